@@ -1,0 +1,609 @@
+"""General-symmetry basis on the device.
+
+Host mirror of ``basis_general`` (src/quspin/basis/basis_general/
+base_general.py:101-1181) and ``hcb_basis_general`` (base_hcb.py:15-95).  The
+native ``_core`` object is replaced by ``_Core``, a handle on the C-ABI core
+of ``libquspin_hip.so``; the basis (``uintX[Ns]``, descending), the
+normalisations ``n`` and the prefix blocks live in HBM as torch buffers.
+"""
+
+import warnings
+
+import numpy as np
+
+from .. import _lib as L
+from .._lib import ptr, stream_handle
+
+
+class GeneralBasisWarning(Warning):
+    pass
+
+
+MAXPRINT = 50
+
+
+def _get_index_type(Ns):
+    """int32 unless Ns reaches 2^31-1 (src/quspin/basis/base.py:7-11)."""
+    return np.int32 if Ns < np.iinfo(np.int32).max else np.int64
+
+
+def get_basis_type(N, Np, sps):
+    """Smallest unsigned word holding N sites of ``sps`` states (uint32 up to
+    32 bits, uint64 up to 64; wider words are not supported here)."""
+    bits = int(np.ceil(N * np.log2(sps))) if sps > 1 else N
+    if bits <= 32:
+        return np.uint32
+    if bits <= 64:
+        return np.uint64
+    raise NotImplementedError("bases wider than 64 bits are not supported")
+
+
+def process_map(map, q):
+    """Validate a site map and compute its period (base_general.py:22-51)."""
+    map = np.asarray(map, dtype=np.int32)
+    i_map = map.copy()
+    i_map[map < 0] = -(i_map[map < 0] + 1)
+    s_map = map < 0
+    sites = np.arange(len(map), dtype=np.int32)
+    order = sites.copy()
+    if np.any(np.sort(i_map) - order):
+        raise ValueError("map must be a one-to-one site mapping.")
+    per = 0
+    group = [tuple(order)]
+    while True:
+        sites[s_map] = -(sites[s_map] + 1)
+        sites = sites[i_map]
+        per += 1
+        group.append(tuple(sites))
+        if np.array_equal(order, sites):
+            break
+    if per == 1:
+        warnings.warn("identity mapping found in set of transformations.", GeneralBasisWarning, stacklevel=5)
+    return map, per, q, set(group)
+
+
+def _check_symmetry_maps(item1, item2):
+    grp1, map1, block1 = item1[1][-1], item1[1][0], item1[0]
+    grp2, map2, block2 = item2[1][-1], item2[1][0], item2[0]
+    if grp1 == grp2:
+        warnings.warn("mappings for block {} and block {} produce the same symmetry.".format(block1, block2),
+                      GeneralBasisWarning, stacklevel=5)
+
+    def act(sites, m):
+        i_m = m.copy()
+        i_m[m < 0] = -(i_m[m < 0] + 1)
+        s = sites.copy()
+        s[m < 0] = -(s[m < 0] + 1)
+        return s[i_m]
+
+    a = act(act(np.arange(len(map1)), map1), map2)
+    b = act(act(np.arange(len(map2)), map2), map1)
+    if not np.array_equal(a, b):
+        warnings.warn("using non-commuting symmetries can lead to unwanted behaviour of general basis, make sure "
+                      "that quantum numbers are invariant under non-commuting symmetries!",
+                      GeneralBasisWarning, stacklevel=5)
+
+
+class _Core:
+    """Device basis core (replaces ``*_basis_core_wrap``; C-ABI
+    ``qsx_core_create``)."""
+
+    def __init__(self, kind, N, maps, pers, qs, double_occupancy=True):
+        lib = L.load()
+        self.kind, self.N = kind, N
+        maps = np.ascontiguousarray(maps, dtype=np.int32).reshape(-1, N) if len(pers) else np.zeros((0, N), np.int32)
+        self.maps = maps
+        self.pers = np.ascontiguousarray(pers, dtype=np.int32)
+        self.qs = np.ascontiguousarray(qs, dtype=np.int32)
+        h = L._vp()
+        rc = lib.qsx_core_create(kind, N, maps.ctypes.data_as(L._vp), self.pers.ctypes.data_as(L._vp),
+                                 self.qs.ctypes.data_as(L._vp), len(self.pers), 1 if double_occupancy else 0,
+                                 L.C.byref(h))
+        L.check(rc, "qsx_core_create")
+        self.h = h
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and L._lib is not None:
+                L._lib.qsx_core_destroy(self.h)
+        except Exception:
+            pass
+
+    @property
+    def G(self):
+        return L.load().qsx_core_group_order(self.h)
+
+
+class basis_general:
+    """Shared machinery of the general bases (see module docstring)."""
+
+    _fermion = False
+    _spinful = False
+
+    def __init__(self, N, block_order=None, **kwargs):
+        self._unique_me = True
+        self._check_herm = True
+        self._check_pcon = None
+        self._basis_pcon = None
+        self._get_proj_pcon = False
+        self._made_basis = False
+        self._Ns_block_est = 0
+        self._dev = None
+        self._host_basis = None
+        self._host_n = None
+        if self.__class__ is basis_general:
+            raise TypeError("general_basis class is not to be instantiated.")
+        kwargs = {key: value for key, value in kwargs.items() if value is not None}
+        n_maps = len(kwargs)
+        if n_maps > 32:
+            raise ValueError("general basis can only support up to 32 symmetries.")
+        self._conserved = "custom symmetries" if n_maps > 0 else ""
+        if any((type(m) is not tuple) and (len(m) != 2) for m in kwargs.values()):
+            raise ValueError("blocks must contain tuple: (map,q).")
+        kwargs = {block: process_map(*item) for block, item in kwargs.items()}
+        if block_order is None:
+            sorted_items = sorted(kwargs.items(), key=lambda x: x[1][1])
+        else:
+            block_order = list(block_order)
+            missing = set(kwargs.keys()) - set(block_order)
+            if missing:
+                raise ValueError("{} names found in block names but missing from block_order.".format(missing))
+            missing = set(block_order) - set(kwargs.keys())
+            if missing:
+                raise ValueError("{} names found in block_order but missing from block names.".format(missing))
+            block_order.reverse()
+            sorted_items = [(key, kwargs[key]) for key in block_order]
+        self._blocks = {block: ((-1) ** q if per == 2 else q) for block, (_, per, q, _) in sorted_items}
+        self._maps_dict = {block: m for block, (m, _, _, _) in sorted_items}
+        remove_index = []
+        for i, item1 in enumerate(sorted_items[:-1]):
+            if item1[1][1] == 1:
+                remove_index.append(i)
+            for item2 in sorted_items[i + 1:]:
+                _check_symmetry_maps(item1, item2)
+        remove_index.sort()
+        if sorted_items:
+            _, items = zip(*sorted_items)
+            items = list(items)
+            for i in remove_index:  # ascending pops, as the reference does
+                items.pop(i)
+            maps, pers, qs, _ = zip(*items)
+            self._maps = np.vstack(maps)
+            self._qs = np.asarray(qs, dtype=np.int32)
+            self._pers = np.asarray(pers, dtype=np.int32)
+            if any(m.shape[0] != N for m in self._maps[:]):
+                raise ValueError("size of map is not equal to N.")
+        else:
+            self._maps = np.array([[]], dtype=np.int32)
+            self._qs = np.array([], dtype=np.int32)
+            self._pers = np.array([], dtype=np.int32)
+        nmax = self._pers.prod()
+        self._n_dtype = np.min_scalar_type(nmax)
+
+    # ------------------------------------------------------------------ core
+    def _make_core(self, kind, N, double_occupancy=True):
+        maps = self._maps if len(self._pers) else np.zeros((0, N), np.int32)
+        return _Core(kind, N, maps, self._pers, self._qs, double_occupancy)
+
+    # ------------------------------------------------------------ properties
+    @property
+    def Ns(self):
+        """int: number of states in the (symmetry-reduced) basis."""
+        return self._Ns
+
+    @property
+    def N(self):
+        """int: number of sites (bits) of the lattice."""
+        return self._N
+
+    @property
+    def sps(self):
+        return self._sps
+
+    @property
+    def blocks(self):
+        return dict(self._blocks)
+
+    @property
+    def dtype(self):
+        return self._basis_dtype
+
+    @property
+    def states(self):
+        """numpy array of the representative states, descending."""
+        if self._host_basis is None and self._made_basis:
+            if self._Ns > 0:
+                self._host_basis = L.to_host(self._dev["basis"]).view(self._basis_dtype)
+            else:
+                self._host_basis = np.array([], dtype=self._basis_dtype)
+        return self._host_basis
+
+    @property
+    def _basis(self):
+        return self.states
+
+    @property
+    def _n(self):
+        if self._host_n is None and self._made_basis:
+            if self._Ns > 0:
+                a = L.to_host(self._dev["n"])
+                self._host_n = a.view(np.dtype(self._n_dtype)) if a.dtype.itemsize == np.dtype(self._n_dtype).itemsize else a.astype(self._n_dtype)
+            else:
+                self._host_n = np.array([], dtype=self._n_dtype)
+        return self._host_n
+
+    @property
+    def description(self):
+        blocks = ", ".join("{} = {}".format(k, v) for k, v in self._blocks.items())
+        return "general basis for lattice of N = {0} sites containing {1} states \n\tsymmetries: {2}\n".format(
+            self._N, self._Ns, blocks or "none")
+
+    def __str__(self):
+        s = self.description
+        if self._made_basis:
+            st = self.states
+            n = len(st)
+            idx = range(n) if n <= MAXPRINT else list(range(MAXPRINT // 2)) + list(range(n - MAXPRINT // 2, n))
+            s += "reference states: \n"
+            for i in idx:
+                s += "\t{:>8d}.  {}\n".format(i, self.int_to_state(int(st[i])))
+        return s
+
+    # ---------------------------------------------------------- make basis
+    def _Np_spec(self):
+        """Particle sectors as the int32 array the C-ABI expects."""
+        Np = self._Np
+        if Np is None:
+            return np.zeros(0, dtype=np.int32), 0
+        if self._spinful:
+            arr = np.asarray(Np, dtype=np.int32).reshape(-1, 2)
+            return np.ascontiguousarray(arr.ravel()), arr.shape[0]
+        arr = np.atleast_1d(np.asarray(Np, dtype=np.int32))
+        return np.ascontiguousarray(arr), arr.size
+
+    def make(self, Ns_block_est=None, N_p=None):
+        """Build the representative basis on the device (``make``
+        base_general.py:817-899): enumerate the particle sector(s), keep orbit
+        maxima with positive normalisation, store them descending."""
+        L.require_device()
+        lib = L.load()
+        if Ns_block_est is not None:
+            Ns_cap = Ns_block_est if Ns_block_est > self._Ns_block_est else self._Ns_block_est
+        else:
+            Ns_cap = max([self._Ns, 1000, self._Ns_block_est])
+        Np, nNp = self._Np_spec()
+        st = stream_handle()
+        Ns = lib.qsx_make_basis(self._core.h, Np.ctypes.data_as(L._vp), nNp, None, None, 1, 0, st)
+        L.check(Ns, "qsx_make_basis")
+        if Ns > Ns_cap:
+            raise ValueError(
+                "estimate for size of reduced Hilbert-space is too low, please double check that transformation "
+                "mappings are correct or use 'Ns_block_est' argument to give an upper bound of the block size.")
+        nb = np.dtype(self._n_dtype).itemsize
+        nb = 1 if nb <= 1 else (2 if nb <= 2 else 4)
+        word = np.dtype(self._basis_dtype)
+        basis_d = L.empty(max(Ns, 1), np.int32 if word.itemsize == 4 else np.int64)
+        n_d = L.empty(max(Ns, 1), {1: np.uint8, 2: np.int16, 4: np.int32}[nb])
+        if Ns > 0:
+            rc = lib.qsx_make_basis(self._core.h, Np.ctypes.data_as(L._vp), nNp, ptr(basis_d), ptr(n_d), nb, Ns, st)
+            L.check(rc, "qsx_make_basis")
+        self._dev = {"basis": basis_d[:Ns], "n": n_d[:Ns], "n_bytes": nb}
+        self._host_basis = None
+        self._host_n = None
+        self._Ns = int(Ns)
+        self._Ns_block_est = int(Ns)
+        self._index_type = _get_index_type(self._Ns)
+        self._made_basis = True
+        # n dtype shrinks to what the stored values need (base_general.py:287-290)
+        if Ns > 0 and len(self._pers):
+            nmax = int(self._dev["n"].to(L.torch().int32).max().item())
+            self._n_dtype = np.min_scalar_type(nmax)
+        self.make_basis_blocks(N_p=N_p)
+
+    def make_basis_blocks(self, N_p=None):
+        """Prefix blocks bounding the search for a representative
+        (``make_basis_blocks`` base_general.py:901-950)."""
+        if not self._made_basis:
+            raise ValueError("reference states are not constructed yet. basis must be constructed before "
+                             "calculating blocks")
+        sps = self.sps if self.sps is not None else 2
+        if N_p is None:
+            N_p = int(np.floor(np.log(self._Ns // 2 + 1) / np.log(sps)))
+        else:
+            N_p = int(N_p)
+        if len(self._pers) == 0 and self._Np is None:
+            N_p = 0
+        self._N_p = min(max(N_p, 0), self.N)
+        if self._N_p > 0:
+            P = 1 << self._N_p
+            begin = L.empty(P, np.int64)
+            end = L.empty(P, np.int64)
+            rc = L.load().qsx_make_basis_blocks(self._core.h, self._N_p, ptr(self._dev["basis"]), self._Ns,
+                                                ptr(begin), ptr(end), stream_handle())
+            L.check(rc, "qsx_make_basis_blocks")
+            self._dev["begin"], self._dev["end"] = begin, end
+        else:
+            self._dev["begin"], self._dev["end"] = None, None
+
+    @property
+    def _basis_begin(self):
+        b = self._dev.get("begin")
+        return np.array([], np.intp) if b is None else L.to_host(b)
+
+    @property
+    def _basis_end(self):
+        b = self._dev.get("end")
+        return np.array([], np.intp) if b is None else L.to_host(b)
+
+    def _dev_args(self):
+        """Positional basis arguments of the C-ABI calls."""
+        d = self._dev
+        return (ptr(d["basis"]), ptr(d["n"]), d["n_bytes"], self._Ns, ptr(d.get("begin")), ptr(d.get("end")),
+                self._N_p)
+
+    # ------------------------------------------------------------- queries
+    def _word_dev(self, states):
+        t = L.require_device()
+        a = np.ascontiguousarray(np.atleast_1d(np.asarray(states)).astype(self._basis_dtype))
+        return L.to_device(a), a.shape
+
+    def representative(self, states, out=None, return_g=False, return_sign=False):
+        """Orbit maxima of ``states`` (base_general.py:1035-1117)."""
+        states = np.asarray(states, order="C", dtype=self._basis_dtype)
+        states = np.atleast_1d(states)
+        if states.ndim != 1:
+            raise TypeError("dimension of array_like states must not exceed 1.")
+        s_d, _ = self._word_dev(states)
+        o_d = L.empty(states.shape[0], np.int32 if np.dtype(self._basis_dtype).itemsize == 4 else np.int64)
+        nm = self._qs.shape[0]
+        g_d = L.zeros((states.shape[0], max(nm, 1)), np.int32) if return_g else None
+        sg_d = L.zeros(states.shape[0], np.int8) if return_sign else None
+        rc = L.load().qsx_representative(self._core.h, ptr(s_d), ptr(o_d), ptr(g_d), ptr(sg_d), states.shape[0],
+                                         stream_handle())
+        L.check(rc, "qsx_representative")
+        reps = L.to_host(o_d).view(self._basis_dtype)
+        g_out = L.to_host(g_d)[:, :nm] if return_g else None
+        sign_out = L.to_host(sg_d) if return_sign else None
+        if out is not None:
+            if not isinstance(out, np.ndarray):
+                raise TypeError("out must be a numpy.ndarray")
+            if states.shape != out.shape:
+                raise TypeError("states and out must have same shape.")
+            if out.dtype != self._basis_dtype:
+                raise TypeError("out must have same type as basis")
+            out[...] = reps
+            res = tuple(x for x in (g_out, sign_out) if x is not None)
+            return res if len(res) > 1 else (res[0] if res else None)
+        res = [reps] + [x for x in (g_out, sign_out) if x is not None]
+        return tuple(res) if len(res) > 1 else reps
+
+    def normalization(self, states, out=None):
+        """Character sum over the stabiliser of each state, 0 outside the
+        sector (base_general.py:1119-1181)."""
+        states = np.atleast_1d(np.asarray(states, order="C", dtype=self._basis_dtype))
+        s_d, _ = self._word_dev(states)
+        o_d = L.zeros(states.shape[0], np.int64)
+        rc = L.load().qsx_normalization(self._core.h, ptr(s_d), ptr(o_d), 8, states.shape[0], stream_handle())
+        L.check(rc, "qsx_normalization")
+        vals = L.to_host(o_d).astype(np.uint64)
+        if out is None:
+            return vals.astype(np.min_scalar_type(vals.max(initial=0))).squeeze()
+        if states.shape != out.shape:
+            raise TypeError("states and out must have same shape.")
+        if np.issubdtype(out.dtype, np.signedinteger):
+            raise TypeError("out must have datatype numpy.uint8, numpy.uint16, numpy.uint32, or numpy.uint64.")
+        out[...] = vals.astype(out.dtype)
+
+    def index(self, *args):
+        """Position of a state (int or string) in the basis."""
+        if len(args) == 1:
+            s = args[0]
+        else:
+            s = "".join(str(a) for a in args)
+        if isinstance(s, str):
+            s = int(s.replace("|", "").replace(">", "").replace("<", "").replace(" ", ""), self.sps)
+        st = self.states[::-1]
+        pos = np.searchsorted(st, np.asarray(s, dtype=self._basis_dtype))
+        if pos >= st.size or st[pos] != s:
+            raise ValueError("state {} not found in basis".format(s))
+        return self._Ns - 1 - int(pos)
+
+    def int_to_state(self, state, bracket_notation=True):
+        state = int(state)
+        bits = [(state >> (self.N - 1 - i)) & 1 for i in range(self.N)]
+        s_str = " ".join(str(b) for b in bits)
+        return "|" + s_str + ">" if bracket_notation else s_str.replace(" ", "")
+
+    def state_to_int(self, state):
+        state = state.replace("|", "").replace(">", "").replace("<", "").replace(" ", "")
+        return int(state, 2)
+
+    # ---------------------------------------------------------- operators
+    def _opstr_prepare(self, opstr, indx, J):
+        """Subclasses map (opstr, indx, J) to the core's bit alphabet
+        (Pauli scaling, spinful '|' notation)."""
+        return opstr, indx, J
+
+    def _validate_op(self, opstr, indx):
+        indx = np.asarray(indx, dtype=np.int32)
+        if len(opstr) != len(indx):
+            raise ValueError("length of opstr does not match length of indx")
+        if np.any(indx >= self._N) or np.any(indx < 0):
+            raise ValueError("values in indx falls outside of system")
+        extra_ops = set(opstr) - self._allowed_ops
+        if extra_ops:
+            raise ValueError("unrecognized characters {} in operator string.".format(extra_ops))
+
+    def Op(self, opstr, indx, J, dtype):
+        """(ME, row, col) of one operator string in the reduced basis
+        (``_Op`` base_general.py:292-352): one element per basis state,
+        vanishing elements removed."""
+        if not self._made_basis:
+            raise AttributeError("this function requires the basis to be constructed first; use basis.make().")
+        opstr, indx, J = self._opstr_prepare(opstr, indx, J)
+        self._validate_op(opstr, indx)
+        dtype = np.dtype(dtype)
+        if self._Ns <= 0:
+            return (np.array([], dtype=dtype), np.array([], dtype=self._index_type),
+                    np.array([], dtype=self._index_type))
+        idx_bytes = np.dtype(self._index_type).itemsize
+        row = L.empty(self._Ns, self._index_type)
+        col = L.empty(self._Ns, self._index_type)
+        me = L.empty(self._Ns, np.complex128)
+        indx = np.ascontiguousarray(indx, dtype=np.int32)
+        Jc = complex(J)
+        rc = L.load().qsx_op(self._core.h, opstr.encode(), indx.ctypes.data_as(L._vp), len(opstr), Jc.real, Jc.imag,
+                             *self._dev_args(), ptr(row), ptr(col), idx_bytes, ptr(me), stream_handle())
+        L.check(rc, "qsx_op")
+        ME = L.to_host(me)
+        if dtype.kind != "c":
+            if np.any(ME.imag != 0):
+                raise TypeError("attempting to use real dtype for complex matrix elements")
+            ME = ME.real
+        ME = ME.astype(dtype)
+        mask = ME != 0
+        return ME[mask], L.to_host(row)[mask], L.to_host(col)[mask]
+
+    def _compile_oplist(self, op_list):
+        """Device operator list (``qsx_oplist_create``) of prepared terms."""
+        opstrs, lens, idx, Js = [], [], [], []
+        for opstr, indx, J in op_list:
+            opstr, indx, J = self._opstr_prepare(opstr, indx, J)
+            self._validate_op(opstr, indx)
+            opstrs.append(opstr)
+            lens.append(len(opstr))
+            idx.extend(int(i) for i in indx)
+            Jc = complex(J)
+            Js.extend([Jc.real, Jc.imag])
+        return _OpList(self._core, "".join(opstrs), np.asarray(lens, np.int32), np.asarray(idx, np.int32),
+                       np.asarray(Js, np.float64))
+
+    def inplace_Op(self, v_in, op_list, dtype, transposed=False, conjugated=False, a=1.0, v_out=None):
+        """v_out += a * Op v_in for the operator list (``inplace_Op``
+        src/quspin/basis/base.py:164-223 -> base_general.py:354-410), all
+        terms fused in one device launch."""
+        if not self._made_basis:
+            raise AttributeError("this function requires the basis to be constructed first; use basis.make().")
+        on_dev = L.is_device_tensor(v_in)
+        if on_dev:
+            vin_np_dtype = L.np_dtype(v_in.dtype)
+        else:
+            v_in = np.asanyarray(v_in)
+            vin_np_dtype = v_in.dtype
+        result_dtype = np.result_type(vin_np_dtype, dtype)
+        shape = tuple(v_in.shape)
+        if shape[0] != self.Ns:
+            raise ValueError("dimension mismatch")
+        if v_out is not None:
+            out_dt = L.np_dtype(v_out.dtype) if L.is_device_tensor(v_out) else v_out.dtype
+            if out_dt != result_dtype:
+                raise TypeError("v_out does not have the correct data type.")
+            if tuple(v_out.shape) != shape:
+                raise ValueError("invalid shape for v_out and v_in: v_in.shape != v_out.shape")
+            if not L.is_device_tensor(v_out) and not v_out.flags["CARRAY"]:
+                raise ValueError("v_out is not a writable C-contiguous array")
+        vin_d = L.to_device(v_in, result_dtype)
+        if v_out is None:
+            vout_d = L.zeros(shape, result_dtype)
+        elif L.is_device_tensor(v_out):
+            vout_d = v_out
+        else:
+            vout_d = L.to_device(v_out, result_dtype)
+        nvec = int(np.prod(shape[1:])) if len(shape) > 1 else 1
+        ol = self._compile_oplist(op_list)
+        ac = complex(a)
+        rc = L.load().qsx_hv(self._core.h, ol.h, *self._dev_args(), ptr(vin_d), ptr(vout_d),
+                             L.dtype_code(result_dtype), nvec, int(bool(transposed)), int(bool(conjugated)),
+                             int(bool(conjugated)), ac.real, ac.imag, ac.real, ac.imag, 1, None, 0,
+                             stream_handle())
+        L.check(rc, "qsx_hv")
+        if on_dev or L.is_device_tensor(v_out):
+            return vout_d.squeeze() if v_out is None else v_out
+        res = L.to_host(vout_d)
+        if v_out is not None:
+            v_out[...] = res
+            return v_out.squeeze()
+        return res.squeeze()
+
+    # --------------------------------------------- hermiticity/symmetry checks
+    def _expand_term(self, opstr, indx, J):
+        return [(opstr, tuple(indx), J)]
+
+    def check_hermitian(self, static, dynamic):
+        from ._checks import check_hermitian
+        check_hermitian(self, static, dynamic)
+
+    def check_symm(self, static, dynamic):
+        from ._checks import check_symm
+        check_symm(self, static, dynamic)
+
+    def check_pcon(self, static, dynamic):
+        from ._checks import check_pcon
+        check_pcon(self, static, dynamic)
+
+
+class _OpList:
+    """Device operator list handle."""
+
+    def __init__(self, core, opstrs, lens, idx, Js):
+        self.core = core
+        h = L._vp()
+        rc = L.load().qsx_oplist_create(core.h, len(lens), opstrs.encode(), lens.ctypes.data_as(L._vp),
+                                        idx.ctypes.data_as(L._vp), Js.ctypes.data_as(L._vp), L.C.byref(h))
+        L.check(rc, "qsx_oplist_create")
+        self.h = h
+        no, nd, cx = L._i32(), L._i32(), L._i32()
+        L.load().qsx_oplist_info(h, L.C.byref(no), L.C.byref(nd), L.C.byref(cx))
+        self.n_off, self.n_diag, self.complex_me = no.value, nd.value, cx.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and L._lib is not None:
+                L._lib.qsx_oplist_destroy(self.h)
+        except Exception:
+            pass
+
+
+class hcb_basis_general(basis_general):
+    """Hard-core bosons / spin-1/2 (base_hcb.py:15-95)."""
+
+    def __init__(self, N, Nb=None, Ns_block_est=None, _Np=None, _make_basis=True, block_order=None, **kwargs):
+        from scipy.special import comb
+        basis_general.__init__(self, N, block_order=block_order, **kwargs)
+        self._check_pcon = False
+        self._count_particles = False
+        if Nb is None:
+            self._Ns = 1 << N
+        elif type(Nb) is int:
+            self._check_pcon = True
+            self._get_proj_pcon = True
+            self._Ns = comb(N, Nb, exact=True)
+        else:
+            try:
+                iter(Nb)
+            except TypeError:
+                raise TypeError("Nb must be integer or iteratable object.")
+            Nb = list(Nb)
+            self._Ns = 0
+            for b in Nb:
+                if b > N or b < 0:
+                    raise ValueError("particle number Nb must satisfy: 0 <= Nb <= N")
+                self._Ns += comb(N, b, exact=True)
+        if len(self._pers) > 0:
+            if Ns_block_est is None:
+                self._Ns = int(float(self._Ns) / np.multiply.reduce(self._pers)) * 2
+            else:
+                if type(Ns_block_est) is not int:
+                    raise TypeError("Ns_block_est must be integer value.")
+                self._Ns = Ns_block_est
+        self._basis_dtype = get_basis_type(N, None, 2)
+        self._N = N
+        self._Ns_block_est = self._Ns
+        self._Np = Nb
+        self._sps = 2
+        self._allowed_ops = set(["I", "x", "y", "z", "+", "-", "n"])
+        self._core = self._make_core(L.KIND_HCB, N)
+        if _make_basis:
+            self.make()
+        else:
+            self._Ns = 1

@@ -1,0 +1,562 @@
+// Operator lists, fused H.v, per-term inplace_op, COO op and CSR assembly.
+//
+// Replaces ``_core.inplace_op`` (base_general.py:395-408), ``_core.op``
+// (base_general.py:324-336) and the COO->CSR assembly of ``_make_matrix``
+// (src/quspin/basis/base.py:265-304).
+#include <hipcub/hipcub.hpp>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "hv_impl.h"
+
+namespace {
+
+int op_code(char ch) {
+  switch (ch) {
+    case 'I': return OP_I;
+    case 'z': return OP_Z;
+    case 'n': return OP_N;
+    case '+': return OP_P;
+    case '-': return OP_M;
+    case 'x': return OP_X;
+    case 'y': return OP_Y;
+    default: return -1;
+  }
+}
+
+bool is_flip(int c) { return c == OP_P || c == OP_M || c == OP_X || c == OP_Y; }
+
+// Compile opstr/indx (character k acts on site indx[k]; rightmost first) into
+// a TermDev; hc = 1 builds the Hermitian conjugate (reversed order, + <-> -,
+// conj(J)).
+int compile_term(const qsx_core* c, const char* opstr, const int32_t* indx, int n_op, double jr, double ji, int hc,
+                 TermDev& T, int& n_y) {
+  if (n_op > QSX_MAX_OPS) { qsx_set_error("operator string longer than 16 characters"); return QSX_ERR_UNSUPPORTED; }
+  std::memset(&T, 0, sizeof(T));
+  const int N = c->h.N;
+  T.nops = n_op;
+  T.jr = jr;
+  T.ji = hc ? -ji : ji;
+  n_y = 0;
+  for (int k = 0; k < n_op; ++k) {
+    // application order: original -> characters n_op-1 ... 0; hc -> 0 ... n_op-1
+    const int src = hc ? k : (n_op - 1 - k);
+    int code = op_code(opstr[src]);
+    if (code < 0) {
+      qsx_set_error(std::string("unrecognized characters {'") + opstr[src] + "'} in operator string.");
+      return QSX_ERR_ARG;
+    }
+    if (hc) { if (code == OP_P) code = OP_M; else if (code == OP_M) code = OP_P; }
+    const int site = indx[src];
+    if (site < 0 || site >= N) { qsx_set_error("values in indx falls outside of system"); return QSX_ERR_ARG; }
+    T.code[k] = (uint8_t)code;
+    T.pos[k] = (uint8_t)(N - 1 - site);
+    if (is_flip(code)) T.flip ^= 1ull << (N - 1 - site);
+    if (code == OP_Y) ++n_y;
+  }
+  return QSX_OK;
+}
+
+}  // namespace
+
+struct qsx_oplist_full {
+  qsx_oplist base;
+  TermDev* off_hc = nullptr;  // device, Hermitian-conjugate off-diagonal terms
+  std::vector<TermDev> off_hc_h;
+  int real_ok = 1;            // all factors real for real dtypes
+};
+
+extern "C" int qsx_oplist_create(qsx_core* c, int n_terms, const char* opstrs, const int32_t* lens,
+                                 const int32_t* indx, const double* J, qsx_oplist** out) {
+  if (!c || !out || n_terms < 0 || n_terms > QSX_MAX_TERMS) {
+    qsx_set_error("qsx_oplist_create: bad arguments (at most 4096 terms)");
+    return QSX_ERR_ARG;
+  }
+  auto* L = new qsx_oplist_full();
+  int pos_s = 0, pos_i = 0;
+  for (int t = 0; t < n_terms; ++t) {
+    TermDev T, Th;
+    int ny = 0, ny2 = 0;
+    int rc = compile_term(c, opstrs + pos_s, indx + pos_i, lens[t], J[2 * t], J[2 * t + 1], 0, T, ny);
+    if (rc == QSX_OK) rc = compile_term(c, opstrs + pos_s, indx + pos_i, lens[t], J[2 * t], J[2 * t + 1], 1, Th, ny2);
+    if (rc != QSX_OK) { delete L; return rc; }
+    pos_s += lens[t];
+    pos_i += lens[t];
+    if (J[2 * t + 1] != 0.0 || (ny & 1)) L->real_ok = 0;
+    if (ny > 0 || J[2 * t + 1] != 0.0) L->base.complex_me = 1;
+    if (T.flip == 0) L->base.diag_h.push_back(T);
+    else { L->base.off_h.push_back(T); L->off_hc_h.push_back(Th); }
+  }
+  L->base.n_off = (int)L->base.off_h.size();
+  L->base.n_diag = (int)L->base.diag_h.size();
+  auto up = [](std::vector<TermDev>& v, TermDev** d) -> hipError_t {
+    if (v.empty()) return hipSuccess;
+    hipError_t e = hipMalloc(d, sizeof(TermDev) * v.size());
+    if (e == hipSuccess) e = hipMemcpy(*d, v.data(), sizeof(TermDev) * v.size(), hipMemcpyHostToDevice);
+    return e;
+  };
+  hipError_t e = up(L->base.off_h, &L->base.off);
+  if (e == hipSuccess) e = up(L->off_hc_h, &L->off_hc);
+  if (e == hipSuccess) e = up(L->base.diag_h, &L->base.diag);
+  if (e != hipSuccess) {
+    qsx_set_error(std::string("qsx_oplist_create: ") + hipGetErrorString(e));
+    delete L;
+    return QSX_ERR_HIP;
+  }
+  *out = &L->base;
+  return QSX_OK;
+}
+
+extern "C" void qsx_oplist_destroy(qsx_oplist* l) {
+  if (!l) return;
+  auto* L = reinterpret_cast<qsx_oplist_full*>(l);
+  if (L->base.off) (void)hipFree(L->base.off);
+  if (L->off_hc) (void)hipFree(L->off_hc);
+  if (L->base.diag) (void)hipFree(L->base.diag);
+  delete L;
+}
+
+extern "C" int qsx_oplist_info(const qsx_oplist* l, int* n_off, int* n_diag, int* complex_me) {
+  if (!l) return QSX_ERR_ARG;
+  *n_off = l->n_off;
+  *n_diag = l->n_diag;
+  *complex_me = l->complex_me;
+  return QSX_OK;
+}
+
+static bool core_chars_real(const qsx_core* c) {
+  for (const auto& p : c->phase_h) if (p.y != 0.0) return false;
+  return true;
+}
+
+static int pick_lookup(const qsx_core* c, int64_t Ns) {
+  if (c->h.nmaps == 0 && c->rank_ok && c->rank_h.Ns == Ns) return LOOKUP_RANK;
+  return LOOKUP_SEARCH;
+}
+
+static int dispatch_hv(const HvArgs& A, int dtype, int N, int nmaps, bool fm, int lookup, hipStream_t st) {
+  switch (dtype) {
+    case QSX_F32: return qsx_hv_launch_f32(A, N, nmaps, fm, lookup, st);
+    case QSX_F64: return qsx_hv_launch_f64(A, N, nmaps, fm, lookup, st);
+    case QSX_C64: return qsx_hv_launch_c64(A, N, nmaps, fm, lookup, st);
+    case QSX_C128: return qsx_hv_launch_c128(A, N, nmaps, fm, lookup, st);
+    default: qsx_set_error("unknown dtype code"); return QSX_ERR_ARG;
+  }
+}
+
+// v_out = beta*v_out + a_diag*(D + diag_arr) v_in + a_off*Op v_in
+// (Op = sum of off-diagonal terms, transposed/conjugated per flags; D = sum of
+// diagonal terms, conjugated iff diag_conj).
+extern "C" int qsx_hv(qsx_core* c, qsx_oplist* l, const void* basis, const void* nrm, int n_bytes, int64_t Ns,
+                      const int64_t* begin, const int64_t* end, int N_p, const void* v_in, void* v_out, int dtype,
+                      int64_t nvec, int transposed, int conjugated, int diag_conj, double a_off_re,
+                      double a_off_im, double a_diag_re, double a_diag_im, int beta, const void* diag_arr,
+                      int diag_arr_cplx, void* stream) {
+  if (!c || !l || nvec < 1 || Ns < 0) { qsx_set_error("qsx_hv: bad arguments"); return QSX_ERR_ARG; }
+  if (Ns == 0) return QSX_OK;
+  auto* L = reinterpret_cast<qsx_oplist_full*>(l);
+  const bool real = (dtype == QSX_F32 || dtype == QSX_F64);
+  if (real && (!L->real_ok || !core_chars_real(c) || a_off_im != 0.0 || a_diag_im != 0.0 ||
+               (diag_arr && diag_arr_cplx))) {
+    qsx_set_error("real dtype requested for an operator with complex matrix elements");
+    return QSX_ERR_COMPLEX;
+  }
+  if (c->h.nmaps > 0 && n_bytes != 1 && n_bytes != 2 && n_bytes != 4) {
+    qsx_set_error("qsx_hv: n_bytes must be 1, 2 or 4");
+    return QSX_ERR_ARG;
+  }
+  HvArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.C = c->d;
+  A.ph = c->phase;
+  A.R = c->rank_d;
+  A.off = transposed ? L->base.off : L->off_hc;
+  A.n_off = L->base.n_off;
+  A.off_conj = (transposed == conjugated) ? 1 : 0;
+  A.dg = L->base.diag;
+  A.n_diag = L->base.n_diag;
+  A.diag_conj = diag_conj ? 1 : 0;
+  A.diag_arr = diag_arr;
+  A.diag_arr_cplx = diag_arr_cplx;
+  A.basis = basis;
+  A.nrm = nrm;
+  A.n_bytes = n_bytes;
+  A.begin = begin;
+  A.end = end;
+  A.N_p = N_p;
+  A.v_in = v_in;
+  A.v_out = v_out;
+  A.Ns = Ns;
+  A.nvec = nvec;
+  A.alpha_off = make_double2(a_off_re, a_off_im);
+  A.alpha_diag = make_double2(a_diag_re, a_diag_im);
+  A.beta = beta ? 1 : 0;
+  return dispatch_hv(A, dtype, c->h.N, c->h.nmaps, c->h.fermion != 0, pick_lookup(c, Ns), (hipStream_t)stream);
+}
+
+// Reference-compatible single term: v_out += J Op v_in (Op^T / Op^* per flags).
+extern "C" int qsx_inplace_op(qsx_core* c, const void* v_in, void* v_out, int dtype, int64_t nvec, int conjugated,
+                              int transposed, const char* opstr, const int32_t* indx, int n_op, double J_re,
+                              double J_im, const void* basis, const void* nrm, int n_bytes, int64_t Ns,
+                              const int64_t* begin, const int64_t* end, int N_p, void* stream) {
+  if (!c) { qsx_set_error("qsx_inplace_op: null core"); return QSX_ERR_ARG; }
+  hipStream_t st = (hipStream_t)stream;
+  TermDev T, Th;
+  int ny = 0, ny2 = 0;
+  int rc = compile_term(c, opstr, indx, n_op, J_re, J_im, 0, T, ny);
+  if (rc == QSX_OK) rc = compile_term(c, opstr, indx, n_op, J_re, J_im, 1, Th, ny2);
+  if (rc != QSX_OK) return rc;
+  if (Ns == 0) return QSX_OK;
+  const bool real = (dtype == QSX_F32 || dtype == QSX_F64);
+  if (real && (J_im != 0.0 || (ny & 1) || !core_chars_real(c))) {
+    qsx_set_error("real dtype requested for an operator with complex matrix elements");
+    return QSX_ERR_COMPLEX;
+  }
+  TermDev* d = nullptr;
+  QSX_CHECK_HIP(hipMallocAsync((void**)&d, sizeof(TermDev), st));
+  const bool diag = (T.flip == 0);
+  const TermDev& use = diag ? T : (transposed ? T : Th);
+  QSX_CHECK_HIP(hipMemcpyAsync(d, &use, sizeof(TermDev), hipMemcpyHostToDevice, st));
+  HvArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.C = c->d;
+  A.ph = c->phase;
+  A.R = c->rank_d;
+  if (diag) {
+    A.dg = d; A.n_diag = 1; A.diag_conj = conjugated ? 1 : 0;
+  } else {
+    A.off = d; A.n_off = 1; A.off_conj = (transposed == conjugated) ? 1 : 0;
+  }
+  A.basis = basis; A.nrm = nrm; A.n_bytes = n_bytes; A.begin = begin; A.end = end; A.N_p = N_p;
+  A.v_in = v_in; A.v_out = v_out; A.Ns = Ns; A.nvec = nvec;
+  A.alpha_off = make_double2(1.0, 0.0);
+  A.alpha_diag = make_double2(1.0, 0.0);
+  A.beta = 1;
+  rc = dispatch_hv(A, dtype, c->h.N, c->h.nmaps, c->h.fermion != 0, pick_lookup(c, Ns), st);
+  (void)hipFreeAsync(d, st);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// COO of one term, one element per basis state (column): ``_core.op``.
+// ---------------------------------------------------------------------------
+template <typename I, int M, bool FERM, int LK>
+__global__ void __launch_bounds__(256) k_op_coo(HvArgs A, const TermDev* __restrict__ T, void* row, void* col,
+                                                int idx_bytes, double2* __restrict__ me) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= A.Ns) return;
+  const I s = ((const I*)A.basis)[i];
+  double rsq = 1.0;
+  if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, i));
+  int64_t rr = i;
+  double2 v = make_double2(0.0, 0.0);
+  term_element<I, M, FERM, LK, true>(A, T, s, rsq, 0, [&](int64_t c, double2 coef) { rr = c; v = coef; });
+  if (idx_bytes == 4) { ((int32_t*)row)[i] = (int32_t)rr; ((int32_t*)col)[i] = (int32_t)i; }
+  else { ((int64_t*)row)[i] = rr; ((int64_t*)col)[i] = i; }
+  me[i] = v;
+}
+
+template <typename I>
+static void launch_op_coo(const HvArgs& A, const TermDev* T, void* row, void* col, int idx_bytes, double2* me,
+                          int nmaps, bool fm, int lookup, hipStream_t st) {
+  const dim3 grid((unsigned)qsx_grid(A.Ns, 256)), blk(256);
+  if (nmaps == 0) {
+    if (lookup == LOOKUP_RANK) hipLaunchKernelGGL((k_op_coo<I, 0, false, LOOKUP_RANK>), grid, blk, 0, st, A, T, row, col, idx_bytes, me);
+    else hipLaunchKernelGGL((k_op_coo<I, 0, false, LOOKUP_SEARCH>), grid, blk, 0, st, A, T, row, col, idx_bytes, me);
+  } else if (fm) {
+    QSX_HV_MSWITCH(nmaps, hipLaunchKernelGGL((k_op_coo<I, MM, true, LOOKUP_SEARCH>), grid, blk, 0, st, A, T, row, col, idx_bytes, me));
+  } else {
+    QSX_HV_MSWITCH(nmaps, hipLaunchKernelGGL((k_op_coo<I, MM, false, LOOKUP_SEARCH>), grid, blk, 0, st, A, T, row, col, idx_bytes, me));
+  }
+}
+
+// me_out: complex128 (double2) per basis state; zero where the element
+// vanishes.  The caller casts and masks (base_general.py:338-350).
+extern "C" int qsx_op(qsx_core* c, const char* opstr, const int32_t* indx, int n_op, double J_re, double J_im,
+                      const void* basis, const void* nrm, int n_bytes, int64_t Ns, const int64_t* begin,
+                      const int64_t* end, int N_p, void* row, void* col, int idx_bytes, void* me_out,
+                      void* stream) {
+  if (!c || (idx_bytes != 4 && idx_bytes != 8)) { qsx_set_error("qsx_op: bad arguments"); return QSX_ERR_ARG; }
+  hipStream_t st = (hipStream_t)stream;
+  TermDev T;
+  int ny = 0;
+  int rc = compile_term(c, opstr, indx, n_op, J_re, J_im, 0, T, ny);
+  if (rc != QSX_OK) return rc;
+  if (Ns == 0) return QSX_OK;
+  TermDev* d = nullptr;
+  QSX_CHECK_HIP(hipMallocAsync((void**)&d, sizeof(TermDev), st));
+  QSX_CHECK_HIP(hipMemcpyAsync(d, &T, sizeof(TermDev), hipMemcpyHostToDevice, st));
+  HvArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.C = c->d; A.ph = c->phase; A.R = c->rank_d;
+  A.basis = basis; A.nrm = nrm; A.n_bytes = n_bytes; A.begin = begin; A.end = end; A.N_p = N_p; A.Ns = Ns;
+  const int lk = pick_lookup(c, Ns);
+  if (c->h.N <= 32) launch_op_coo<uint32_t>(A, d, row, col, idx_bytes, (double2*)me_out, c->h.nmaps, c->h.fermion != 0, lk, st);
+  else launch_op_coo<uint64_t>(A, d, row, col, idx_bytes, (double2*)me_out, c->h.nmaps, c->h.fermion != 0, lk, st);
+  QSX_LAUNCH_CHECK();
+  (void)hipFreeAsync(d, st);
+  return QSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CSR assembly: rows gathered with the Hermitian-conjugate terms, columns
+// sorted, duplicates summed, zeros dropped (scipy canonical CSR).
+// ---------------------------------------------------------------------------
+template <typename I, int M, bool FERM, int LK, int PASS>
+__global__ void __launch_bounds__(256) k_csr_rows(HvArgs A, int64_t* __restrict__ cnt, const int64_t* __restrict__ offs,
+                                                  int64_t* __restrict__ rc, double2* __restrict__ rv) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= A.Ns) return;
+  const I s = ((const I*)A.basis)[r];
+  double rsq = 1.0;
+  if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, r));
+  const double2 d = row_diag<I, M, FERM, LK, true>(A, r, s);
+  int64_t k = 0;
+  const int64_t o = (PASS == 1) ? offs[r] : 0;
+  if (d.x != 0.0 || d.y != 0.0) {
+    if constexpr (PASS == 1) { rc[o + k] = r; rv[o + k] = d; }
+    ++k;
+  }
+  for (int t = 0; t < A.n_off; ++t) {
+    term_element<I, M, FERM, LK, true>(A, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
+      coef = cmul(coef, A.alpha_off);
+      if constexpr (PASS == 1) { rc[o + k] = c; rv[o + k] = coef; }
+      ++k;
+    });
+  }
+  if constexpr (PASS == 0) cnt[r] = k;
+}
+
+// per row: insertion sort by column, merge duplicates, drop zeros
+__global__ void k_csr_canon(int64_t Ns, const int64_t* __restrict__ offs, const int64_t* __restrict__ cnt,
+                            int64_t* __restrict__ rc, double2* __restrict__ rv, int64_t* __restrict__ fcnt) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= Ns) return;
+  const int64_t o = offs[r], n = cnt[r];
+  for (int64_t i = 1; i < n; ++i) {
+    const int64_t kc = rc[o + i];
+    const double2 kv = rv[o + i];
+    int64_t j = i - 1;
+    while (j >= 0 && rc[o + j] > kc) { rc[o + j + 1] = rc[o + j]; rv[o + j + 1] = rv[o + j]; --j; }
+    rc[o + j + 1] = kc;
+    rv[o + j + 1] = kv;
+  }
+  int64_t w = 0;
+  for (int64_t i = 0; i < n;) {
+    const int64_t cc = rc[o + i];
+    double2 acc = rv[o + i];
+    int64_t j = i + 1;
+    while (j < n && rc[o + j] == cc) { acc = cadd(acc, rv[o + j]); ++j; }
+    if (acc.x != 0.0 || acc.y != 0.0) { rc[o + w] = cc; rv[o + w] = acc; ++w; }
+    i = j;
+  }
+  fcnt[r] = w;
+}
+
+__global__ void k_csr_compact(int64_t Ns, const int64_t* __restrict__ offs, const int64_t* __restrict__ fcnt,
+                              const int64_t* __restrict__ foffs, const int64_t* __restrict__ rc,
+                              const double2* __restrict__ rv, void* indptr, void* indices, void* data,
+                              int idx_bytes, int dtype) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r > Ns) return;
+  if (r == Ns) {
+    const int64_t tot = (Ns > 0) ? foffs[Ns - 1] + fcnt[Ns - 1] : 0;
+    if (idx_bytes == 4) ((int32_t*)indptr)[Ns] = (int32_t)tot; else ((int64_t*)indptr)[Ns] = tot;
+    return;
+  }
+  const int64_t o = offs[r], fo = foffs[r], n = fcnt[r];
+  if (idx_bytes == 4) ((int32_t*)indptr)[r] = (int32_t)fo; else ((int64_t*)indptr)[r] = fo;
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t cc = rc[o + i];
+    const double2 v = rv[o + i];
+    if (idx_bytes == 4) ((int32_t*)indices)[fo + i] = (int32_t)cc; else ((int64_t*)indices)[fo + i] = cc;
+    switch (dtype) {
+      case QSX_F32: ((float*)data)[fo + i] = (float)v.x; break;
+      case QSX_F64: ((double*)data)[fo + i] = v.x; break;
+      case QSX_C64: ((float2*)data)[fo + i] = make_float2((float)v.x, (float)v.y); break;
+      default: ((double2*)data)[fo + i] = v; break;
+    }
+  }
+}
+
+struct qsx_csr_builder {
+  int64_t Ns = 0, nnz = 0;
+  int64_t *offs = nullptr, *cnt = nullptr, *fcnt = nullptr, *foffs = nullptr, *rc = nullptr;
+  double2* rv = nullptr;
+  int complex_vals = 0;
+};
+
+template <typename I, int M, bool FERM, int LK>
+static void launch_csr_rows(const HvArgs& A, int pass, qsx_csr_builder* B, hipStream_t st) {
+  const dim3 grid((unsigned)qsx_grid(A.Ns, 256)), blk(256);
+  if (pass == 0) hipLaunchKernelGGL((k_csr_rows<I, M, FERM, LK, 0>), grid, blk, 0, st, A, B->cnt, B->offs, B->rc, B->rv);
+  else hipLaunchKernelGGL((k_csr_rows<I, M, FERM, LK, 1>), grid, blk, 0, st, A, B->cnt, B->offs, B->rc, B->rv);
+}
+
+template <typename I>
+static void csr_rows_dispatch(const HvArgs& A, int pass, qsx_csr_builder* B, int nmaps, bool fm, int lk, hipStream_t st) {
+  if (nmaps == 0) {
+    if (lk == LOOKUP_RANK) launch_csr_rows<I, 0, false, LOOKUP_RANK>(A, pass, B, st);
+    else launch_csr_rows<I, 0, false, LOOKUP_SEARCH>(A, pass, B, st);
+  } else if (fm) {
+    QSX_HV_MSWITCH(nmaps, (launch_csr_rows<I, MM, true, LOOKUP_SEARCH>(A, pass, B, st)));
+  } else {
+    QSX_HV_MSWITCH(nmaps, (launch_csr_rows<I, MM, false, LOOKUP_SEARCH>(A, pass, B, st)));
+  }
+}
+
+extern "C" void qsx_csr_builder_destroy(qsx_csr_builder* B) {
+  if (!B) return;
+  (void)hipFree(B->offs); (void)hipFree(B->cnt); (void)hipFree(B->fcnt); (void)hipFree(B->foffs);
+  (void)hipFree(B->rc); (void)hipFree(B->rv);
+  delete B;
+}
+
+// Assemble a*Op (all terms of l; transposed/conjugated per flags) as CSR.
+// Returns nnz through *nnz; finish with qsx_csr_builder_finish.
+extern "C" int qsx_csr_builder_create(qsx_core* c, qsx_oplist* l, const void* basis, const void* nrm, int n_bytes,
+                                      int64_t Ns, const int64_t* begin, const int64_t* end, int N_p, int transposed,
+                                      int conjugated, double a_re, double a_im, qsx_csr_builder** out,
+                                      int64_t* nnz, void* stream) {
+  if (!c || !l || !out || !nnz) { qsx_set_error("qsx_csr_builder_create: bad arguments"); return QSX_ERR_ARG; }
+  hipStream_t st = (hipStream_t)stream;
+  auto* L = reinterpret_cast<qsx_oplist_full*>(l);
+  auto* B = new qsx_csr_builder();
+  B->Ns = Ns;
+  B->complex_vals = (!L->real_ok || !core_chars_real(c) || a_im != 0.0) ? 1 : 0;
+  *out = B;
+  *nnz = 0;
+  if (Ns == 0) return QSX_OK;
+  HvArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.C = c->d; A.ph = c->phase; A.R = c->rank_d;
+  A.off = transposed ? L->base.off : L->off_hc;
+  A.n_off = L->base.n_off;
+  A.off_conj = (transposed == conjugated) ? 1 : 0;
+  A.dg = L->base.diag; A.n_diag = L->base.n_diag; A.diag_conj = conjugated ? 1 : 0;
+  A.basis = basis; A.nrm = nrm; A.n_bytes = n_bytes; A.begin = begin; A.end = end; A.N_p = N_p; A.Ns = Ns;
+  A.alpha_off = make_double2(a_re, a_im);
+  A.alpha_diag = make_double2(a_re, a_im);
+  const int lk = pick_lookup(c, Ns);
+  QSX_CHECK_HIP(hipMalloc(&B->cnt, sizeof(int64_t) * Ns));
+  QSX_CHECK_HIP(hipMalloc(&B->offs, sizeof(int64_t) * Ns));
+  QSX_CHECK_HIP(hipMalloc(&B->fcnt, sizeof(int64_t) * Ns));
+  QSX_CHECK_HIP(hipMalloc(&B->foffs, sizeof(int64_t) * Ns));
+  if (c->h.N <= 32) csr_rows_dispatch<uint32_t>(A, 0, B, c->h.nmaps, c->h.fermion != 0, lk, st);
+  else csr_rows_dispatch<uint64_t>(A, 0, B, c->h.nmaps, c->h.fermion != 0, lk, st);
+  QSX_LAUNCH_CHECK();
+  void* tmp = nullptr;
+  size_t tb = 0;
+  QSX_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, B->cnt, B->offs, (int)Ns, st));
+  QSX_CHECK_HIP(hipMalloc(&tmp, tb + 16));
+  QSX_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, B->cnt, B->offs, (int)Ns, st));
+  int64_t lc = 0, lo = 0;
+  QSX_CHECK_HIP(hipMemcpyAsync(&lc, B->cnt + Ns - 1, 8, hipMemcpyDeviceToHost, st));
+  QSX_CHECK_HIP(hipMemcpyAsync(&lo, B->offs + Ns - 1, 8, hipMemcpyDeviceToHost, st));
+  QSX_CHECK_HIP(hipStreamSynchronize(st));
+  const int64_t raw = lc + lo;
+  QSX_CHECK_HIP(hipMalloc(&B->rc, sizeof(int64_t) * (raw + 1)));
+  QSX_CHECK_HIP(hipMalloc(&B->rv, sizeof(double2) * (raw + 1)));
+  if (c->h.N <= 32) csr_rows_dispatch<uint32_t>(A, 1, B, c->h.nmaps, c->h.fermion != 0, lk, st);
+  else csr_rows_dispatch<uint64_t>(A, 1, B, c->h.nmaps, c->h.fermion != 0, lk, st);
+  QSX_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_csr_canon, dim3((unsigned)qsx_grid(Ns, 256)), dim3(256), 0, st, Ns, B->offs, B->cnt, B->rc,
+                     B->rv, B->fcnt);
+  QSX_LAUNCH_CHECK();
+  QSX_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, B->fcnt, B->foffs, (int)Ns, st));
+  QSX_CHECK_HIP(hipMemcpyAsync(&lc, B->fcnt + Ns - 1, 8, hipMemcpyDeviceToHost, st));
+  QSX_CHECK_HIP(hipMemcpyAsync(&lo, B->foffs + Ns - 1, 8, hipMemcpyDeviceToHost, st));
+  QSX_CHECK_HIP(hipStreamSynchronize(st));
+  (void)hipFree(tmp);
+  B->nnz = lc + lo;
+  *nnz = B->nnz;
+  return QSX_OK;
+}
+
+extern "C" int qsx_csr_builder_complex(const qsx_csr_builder* B) { return B ? B->complex_vals : -1; }
+
+extern "C" int qsx_csr_builder_finish(qsx_csr_builder* B, int dtype, int idx_bytes, void* indptr, void* indices,
+                                      void* data, void* stream) {
+  if (!B || (idx_bytes != 4 && idx_bytes != 8)) { qsx_set_error("qsx_csr_builder_finish: bad arguments"); return QSX_ERR_ARG; }
+  hipStream_t st = (hipStream_t)stream;
+  if ((dtype == QSX_F32 || dtype == QSX_F64) && B->complex_vals) {
+    qsx_set_error("real dtype requested for an operator with complex matrix elements");
+    return QSX_ERR_COMPLEX;
+  }
+  if (B->Ns == 0) {
+    if (idx_bytes == 4) { int32_t z = 0; QSX_CHECK_HIP(hipMemcpyAsync(indptr, &z, 4, hipMemcpyHostToDevice, st)); }
+    else { int64_t z = 0; QSX_CHECK_HIP(hipMemcpyAsync(indptr, &z, 8, hipMemcpyHostToDevice, st)); }
+    QSX_CHECK_HIP(hipStreamSynchronize(st));
+    return QSX_OK;
+  }
+  hipLaunchKernelGGL(k_csr_compact, dim3((unsigned)qsx_grid(B->Ns + 1, 256)), dim3(256), 0, st, B->Ns, B->offs,
+                     B->fcnt, B->foffs, B->rc, B->rv, indptr, indices, data, idx_bytes, dtype);
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// reusable H.v context (fixed operator, basis, dtype, nvec and flags)
+// ---------------------------------------------------------------------------
+struct qsx_hv_ctx {
+  HvArgs A;
+  int dtype, N, nmaps, lookup;
+  bool fm;
+};
+
+extern "C" int qsx_hv_ctx_create(qsx_core* c, qsx_oplist* l, const void* basis, const void* nrm, int n_bytes,
+                                 int64_t Ns, const int64_t* begin, const int64_t* end, int N_p, int dtype,
+                                 int64_t nvec, int transposed, int conjugated, int diag_conj, const void* diag_arr,
+                                 int diag_arr_cplx, qsx_hv_ctx** out) {
+  if (!c || !l || !out || nvec < 1) { qsx_set_error("qsx_hv_ctx_create: bad arguments"); return QSX_ERR_ARG; }
+  auto* L = reinterpret_cast<qsx_oplist_full*>(l);
+  const bool real = (dtype == QSX_F32 || dtype == QSX_F64);
+  if (real && (!L->real_ok || !core_chars_real(c) || (diag_arr && diag_arr_cplx))) {
+    qsx_set_error("real dtype requested for an operator with complex matrix elements");
+    return QSX_ERR_COMPLEX;
+  }
+  auto* X = new qsx_hv_ctx();
+  std::memset(&X->A, 0, sizeof(HvArgs));
+  HvArgs& A = X->A;
+  A.C = c->d; A.ph = c->phase; A.R = c->rank_d;
+  A.off = transposed ? L->base.off : L->off_hc;
+  A.n_off = L->base.n_off;
+  A.off_conj = (transposed == conjugated) ? 1 : 0;
+  A.dg = L->base.diag; A.n_diag = L->base.n_diag; A.diag_conj = diag_conj ? 1 : 0;
+  A.diag_arr = diag_arr; A.diag_arr_cplx = diag_arr_cplx;
+  A.basis = basis; A.nrm = nrm; A.n_bytes = n_bytes; A.begin = begin; A.end = end; A.N_p = N_p;
+  A.Ns = Ns; A.nvec = nvec;
+  A.alpha_off = make_double2(1, 0); A.alpha_diag = make_double2(1, 0);
+  X->dtype = dtype; X->N = c->h.N; X->nmaps = c->h.nmaps; X->fm = c->h.fermion != 0;
+  X->lookup = pick_lookup(c, Ns);
+  *out = X;
+  return QSX_OK;
+}
+
+extern "C" void qsx_hv_ctx_destroy(qsx_hv_ctx* X) { delete X; }
+
+// y = beta*y + a_diag*D x + a_off*Op x
+extern "C" int qsx_hv_ctx_apply(qsx_hv_ctx* X, const void* x, void* y, double a_off_re, double a_off_im,
+                                double a_diag_re, double a_diag_im, int beta, void* stream) {
+  if (!X) { qsx_set_error("qsx_hv_ctx_apply: null context"); return QSX_ERR_ARG; }
+  if (X->A.Ns == 0) return QSX_OK;
+  const bool real = (X->dtype == QSX_F32 || X->dtype == QSX_F64);
+  if (real && (a_off_im != 0.0 || a_diag_im != 0.0)) { qsx_set_error("real vectors with a complex scale"); return QSX_ERR_COMPLEX; }
+  HvArgs A = X->A;
+  A.v_in = x; A.v_out = y;
+  A.alpha_off = make_double2(a_off_re, a_off_im);
+  A.alpha_diag = make_double2(a_diag_re, a_diag_im);
+  A.beta = beta ? 1 : 0;
+  return dispatch_hv(A, X->dtype, X->N, X->nmaps, X->fm, X->lookup, (hipStream_t)stream);
+}
+
+int qsx_hv_expm_term(void* ctx, const void* x, void* y, double2 coef, double2 shift, const int* skip,
+                     hipStream_t st) {
+  qsx_hv_ctx* X = (qsx_hv_ctx*)ctx;
+  HvArgs A = X->A;
+  A.v_in = x; A.v_out = y;
+  A.alpha_off = coef; A.alpha_diag = coef; A.shift = shift; A.skip = skip; A.beta = 0;
+  return dispatch_hv(A, X->dtype, X->N, X->nmaps, X->fm, X->lookup, st);
+}

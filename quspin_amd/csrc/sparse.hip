@@ -1,0 +1,640 @@
+// CSR H.v, trace / 1-norm, device-resident Taylor-series expm and the
+// FP64-accumulating vector kernels used by Lanczos.
+//
+// Replaces parallel_sparse_tools ``_matvec`` (hamiltonian_core.py:607-612,
+// :741-772), ``_wrapper_csr_trace`` / ``_wrapper_csr_1_norm`` (EMP.html:196-202)
+// and ``_wrapper_expm_multiply[_batch]`` (EMP.html:347-372), whose loop is
+// Al-Mohy & Higham Alg. 3.2 as in scipy ``_expm_multiply_simple_core``
+// (scipy/sparse/linalg/_expm_multiply.py:280-303).
+#include <cstring>
+#include <string>
+
+#include "sparse_impl.h"
+
+// ---------------------------------------------------------------------------
+// CSR SpMV dispatch
+// ---------------------------------------------------------------------------
+template <typename MT, typename VT, bool EPI>
+static void launch_csr_g(const CsrArgs& A, int G, hipStream_t st) {
+  const int64_t lanes = A.Ns * G;
+  const dim3 grid((unsigned)qsx_grid(lanes, 256)), blk(256);
+  switch (G) {
+    case 1: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 1, EPI>), grid, blk, 0, st, A); break;
+    case 2: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 2, EPI>), grid, blk, 0, st, A); break;
+    case 4: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 4, EPI>), grid, blk, 0, st, A); break;
+    case 8: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 8, EPI>), grid, blk, 0, st, A); break;
+    case 16: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 16, EPI>), grid, blk, 0, st, A); break;
+    case 32: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 32, EPI>), grid, blk, 0, st, A); break;
+    default: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 64, EPI>), grid, blk, 0, st, A); break;
+  }
+}
+
+template <bool EPI>
+static int launch_csr(const CsrArgs& A, int mdt, int vdt, int G, hipStream_t st) {
+  // matrix dtype must be the vector dtype or its real counterpart
+  if (vdt == QSX_F32 && mdt == QSX_F32) launch_csr_g<float, float, EPI>(A, G, st);
+  else if (vdt == QSX_F64 && mdt == QSX_F64) launch_csr_g<double, double, EPI>(A, G, st);
+  else if (vdt == QSX_C64 && mdt == QSX_F32) launch_csr_g<float, float2, EPI>(A, G, st);
+  else if (vdt == QSX_C64 && mdt == QSX_C64) launch_csr_g<float2, float2, EPI>(A, G, st);
+  else if (vdt == QSX_C128 && mdt == QSX_F64) launch_csr_g<double, double2, EPI>(A, G, st);
+  else if (vdt == QSX_C128 && mdt == QSX_C128) launch_csr_g<double2, double2, EPI>(A, G, st);
+  else { qsx_set_error("unsupported (matrix, vector) dtype pair"); return QSX_ERR_ARG; }
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+static int auto_group(int64_t Ns, int64_t nnz) {
+  const double avg = Ns > 0 ? (double)nnz / (double)Ns : 1.0;
+  int G = 1;
+  while (G < 64 && G < avg) G <<= 1;
+  return G > 1 ? G / 2 : 1;
+}
+
+// y = a*A*x (+ y when accumulate) ; x, y: (Ns, nvec) row-major.
+extern "C" int qsx_csr_matvec(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
+                              int64_t Ns, int64_t nnz, const void* x, void* y, int vec_dtype, int64_t nvec,
+                              double a_re, double a_im, int accumulate, int group, void* stream) {
+  if (Ns < 0 || nvec < 1) { qsx_set_error("qsx_csr_matvec: bad arguments"); return QSX_ERR_ARG; }
+  if (Ns == 0) return QSX_OK;
+  if ((vec_dtype == QSX_F32 || vec_dtype == QSX_F64) && a_im != 0.0) {
+    qsx_set_error("real vectors with a complex scale"); return QSX_ERR_COMPLEX;
+  }
+  CsrArgs A;
+  std::memset(&A, 0, sizeof(A));
+  A.indptr = indptr; A.indices = indices; A.data = data; A.idx64 = idx64; A.Ns = Ns; A.nvec = nvec;
+  A.x = x; A.y = y; A.a = make_double2(a_re, a_im); A.mu = make_double2(0, 0); A.beta = accumulate ? 1 : 0;
+  const int G = group > 0 ? group : auto_group(Ns, nnz);
+  return launch_csr<false>(A, mat_dtype, vec_dtype, G, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// deterministic block reductions (last block sums the partials in order)
+// ---------------------------------------------------------------------------
+template <int OP>  // 0: sum into out[0..1], 1: max into out[0]
+__device__ __forceinline__ void block_finish(double vr, double vi, double* partials, unsigned int* counter,
+                                             double* out, double post_scale_sqrt) {
+  __shared__ double sr[256], si[256];
+  __shared__ bool last;
+  sr[threadIdx.x] = vr;
+  si[threadIdx.x] = vi;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) {
+      if (OP == 0) { sr[threadIdx.x] += sr[threadIdx.x + o]; si[threadIdx.x] += si[threadIdx.x + o]; }
+      else sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + o]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = sr[0];
+    partials[2 * blockIdx.x + 1] = si[0];
+    __threadfence();
+    const unsigned int t = atomicAdd(counter, 1u);
+    last = (t == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (last && threadIdx.x == 0) {
+    __threadfence();
+    double a = 0.0, b = 0.0;
+    for (unsigned int k = 0; k < gridDim.x; ++k) {
+      const double pr = ((volatile double*)partials)[2 * k];
+      const double pi = ((volatile double*)partials)[2 * k + 1];
+      if (OP == 0) { a += pr; b += pi; } else a = fmax(a, pr);
+    }
+    if (post_scale_sqrt != 0.0) a = sqrt(a);
+    out[0] = a;
+    out[1] = b;
+    *counter = 0u;
+  }
+}
+
+constexpr int RED_BLOCKS = 512;
+
+// out = sum conj(x) y  (complex: out[0]+i out[1])
+template <typename VT>
+__global__ void __launch_bounds__(256) k_vdot(const VT* __restrict__ x, const VT* __restrict__ y, int64_t n,
+                                              double* partials, unsigned int* counter, double* out) {
+  double ar = 0.0, ai = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 a = VTr<VT>::ld(x + i), b = VTr<VT>::ld(y + i);
+    ar += a.x * b.x + a.y * b.y;
+    ai += a.x * b.y - a.y * b.x;
+  }
+  block_finish<0>(ar, ai, partials, counter, out, 0.0);
+}
+
+// y += a x with a = sgn * s[0] (+ i s[1]) read from device memory (or host value)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_axpy_dev(const VT* __restrict__ x, VT* __restrict__ y, int64_t n,
+                                                  const double* __restrict__ sdev, double sgn, double2 ahost,
+                                                  int cplx_scale) {
+  double2 a = ahost;
+  if (sdev) a = make_double2(sgn * sdev[0], cplx_scale ? sgn * sdev[1] : 0.0);
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = VTr<VT>::ld(y + i);
+    const double2 u = VTr<VT>::ld(x + i);
+    if constexpr (VTr<VT>::cplx) v = cadd(v, cmul(a, u));
+    else v.x += a.x * u.x;
+    VTr<VT>::st(y + i, v);
+  }
+}
+
+// Lanczos: r -= b_prev * v (b_prev device scalar, may be null);
+// a = Re <q, r>  -> aout[0]
+template <typename VT>
+__global__ void __launch_bounds__(256) k_lz_axpy_dot(const VT* __restrict__ v, const double* __restrict__ bprev,
+                                                     const VT* __restrict__ q, VT* __restrict__ r, int64_t n,
+                                                     double* partials, unsigned int* counter, double* aout) {
+  const double b = bprev ? bprev[0] : 0.0;
+  double ar = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 rv = VTr<VT>::ld(r + i);
+    if (bprev) {
+      const double2 vv = VTr<VT>::ld(v + i);
+      rv.x -= b * vv.x;
+      rv.y -= b * vv.y;
+      VTr<VT>::st(r + i, rv);
+      rv = VTr<VT>::ld(r + i);
+    }
+    const double2 qq = VTr<VT>::ld(q + i);
+    ar += qq.x * rv.x + qq.y * rv.y;
+  }
+  block_finish<0>(ar, 0.0, partials, counter, aout, 0.0);
+}
+
+// Lanczos: r -= a q (a device scalar); b = ||r|| -> bout[0]
+template <typename VT>
+__global__ void __launch_bounds__(256) k_lz_axpy_nrm(const double* __restrict__ adev, const VT* __restrict__ q,
+                                                     VT* __restrict__ r, int64_t n, double* partials,
+                                                     unsigned int* counter, double* bout) {
+  const double a = adev[0];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 rv = VTr<VT>::ld(r + i);
+    const double2 qq = VTr<VT>::ld(q + i);
+    rv.x -= a * qq.x;
+    rv.y -= a * qq.y;
+    VTr<VT>::st(r + i, rv);
+    rv = VTr<VT>::ld(r + i);
+    s += rv.x * rv.x + rv.y * rv.y;
+  }
+  block_finish<0>(s, 0.0, partials, counter, bout, 1.0);
+}
+
+// q = r / b (b device scalar)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_lz_div(const VT* __restrict__ r, const double* __restrict__ bdev,
+                                                VT* __restrict__ q, int64_t n) {
+  const double b = bdev[0];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 v = VTr<VT>::ld(r + i);
+    VTr<VT>::st(q + i, make_double2(v.x / b, v.y / b));
+  }
+}
+
+// ||x||_2 (FP64 accumulation)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_nrm2(const VT* __restrict__ x, int64_t n, double* partials,
+                                              unsigned int* counter, double* out) {
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = VTr<VT>::ld(x + i);
+    s += v.x * v.x + v.y * v.y;
+  }
+  block_finish<0>(s, 0.0, partials, counter, out, 1.0);
+}
+
+// out[j] = sum_i c[i] Q[i, j]  (Q: m x n rows), c complex double
+template <typename VT>
+__global__ void __launch_bounds__(256) k_lincomb(const VT* __restrict__ Q, int64_t m, int64_t n,
+                                                 const double2* __restrict__ c, VT* __restrict__ out) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int64_t i = 0; i < m; ++i) {
+      const double2 q = VTr<VT>::ld(Q + i * n + j);
+      if constexpr (VTr<VT>::cplx) acc = cadd(acc, cmul(c[i], q));
+      else acc.x += c[i].x * q.x;
+    }
+    VTr<VT>::st(out + j, acc);
+  }
+}
+
+// scratch for reductions: partials (2*RED_BLOCKS doubles) + counter
+struct RedScratch {
+  double* partials = nullptr;
+  unsigned int* counter = nullptr;
+};
+
+static int get_red_scratch(RedScratch& R, hipStream_t st) {
+  (void)st;
+  static thread_local double* p = nullptr;
+  static thread_local unsigned int* cnt = nullptr;
+  static thread_local int dev = -1;
+  int cur = 0;
+  QSX_CHECK_HIP(hipGetDevice(&cur));
+  if (!p || dev != cur) {
+    QSX_CHECK_HIP(hipMalloc(&p, sizeof(double) * 2 * RED_BLOCKS * 8));
+    QSX_CHECK_HIP(hipMalloc(&cnt, sizeof(unsigned int) * 8));
+    QSX_CHECK_HIP(hipMemset(cnt, 0, sizeof(unsigned int) * 8));
+    dev = cur;
+  }
+  R.partials = p;
+  R.counter = cnt;
+  return QSX_OK;
+}
+
+static int red_grid(int64_t n) {
+  int64_t g = qsx_grid(n, 256);
+  return (int)(g < RED_BLOCKS ? (g < 1 ? 1 : g) : RED_BLOCKS);
+}
+
+#define QSX_VT_SWITCH(dt, BODY)                                   \
+  switch (dt) {                                                   \
+    case QSX_F32: { using VT = float; BODY; } break;              \
+    case QSX_F64: { using VT = double; BODY; } break;             \
+    case QSX_C64: { using VT = float2; BODY; } break;             \
+    case QSX_C128: { using VT = double2; BODY; } break;           \
+    default: qsx_set_error("unknown dtype"); return QSX_ERR_ARG;  \
+  }
+
+// out[0] + i out[1] = sum conj(x) y   (device output, 2 doubles)
+extern "C" int qsx_vdot(int dtype, const void* x, const void* y, int64_t n, double* out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  RedScratch R;
+  int rc = get_red_scratch(R, st);
+  if (rc) return rc;
+  QSX_VT_SWITCH(dtype, hipLaunchKernelGGL(k_vdot<VT>, dim3(red_grid(n)), dim3(256), 0, st, (const VT*)x, (const VT*)y, n,
+                                          R.partials, R.counter, out));
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+extern "C" int qsx_nrm2(int dtype, const void* x, int64_t n, double* out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  RedScratch R;
+  int rc = get_red_scratch(R, st);
+  if (rc) return rc;
+  QSX_VT_SWITCH(dtype, hipLaunchKernelGGL(k_nrm2<VT>, dim3(red_grid(n)), dim3(256), 0, st, (const VT*)x, n, R.partials,
+                                          R.counter, out));
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// y += a x ; a = (a_re, a_im) or, when sdev != NULL, sgn*sdev[0] (+i sgn*sdev[1] if cplx_scale)
+extern "C" int qsx_axpy(int dtype, const void* x, void* y, int64_t n, double a_re, double a_im, const double* sdev,
+                        double sgn, int cplx_scale, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  QSX_VT_SWITCH(dtype, hipLaunchKernelGGL(k_axpy_dev<VT>, dim3(red_grid(n) * 4), dim3(256), 0, st, (const VT*)x, (VT*)y, n,
+                                          sdev, sgn, make_double2(a_re, a_im), cplx_scale));
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// One Lanczos recurrence update after r = A q_i:
+//   r -= b_prev v ; a_i = Re<q_i, r> ; r -= a_i q_i ; b_i = ||r||
+// (src/quspin/tools/lanczos/_lanczos_utils.py:191-201).  b_prev / v may be
+// NULL for the first step.  a_out, b_out: device doubles (2 slots each).
+extern "C" int qsx_lanczos_update(int dtype, const void* v, const double* b_prev, const void* q, void* r, int64_t n,
+                                  double* a_out, double* b_out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  RedScratch R;
+  int rc = get_red_scratch(R, st);
+  if (rc) return rc;
+  QSX_VT_SWITCH(dtype, {
+    hipLaunchKernelGGL(k_lz_axpy_dot<VT>, dim3(red_grid(n)), dim3(256), 0, st, (const VT*)v, b_prev, (const VT*)q,
+                       (VT*)r, n, R.partials, R.counter, a_out);
+    hipLaunchKernelGGL(k_lz_axpy_nrm<VT>, dim3(red_grid(n)), dim3(256), 0, st, (const double*)a_out, (const VT*)q,
+                       (VT*)r, n, R.partials + 2 * RED_BLOCKS, R.counter + 1, b_out);
+  });
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// q = r / b (device scalar)
+extern "C" int qsx_lanczos_div(int dtype, const void* r, const double* b, void* q, int64_t n, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  QSX_VT_SWITCH(dtype, hipLaunchKernelGGL(k_lz_div<VT>, dim3(red_grid(n) * 4), dim3(256), 0, st, (const VT*)r, b, (VT*)q, n));
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// out = sum_i c_i Q[i]  (c: m complex doubles, device)
+extern "C" int qsx_lincomb(int dtype, const void* Q, int64_t m, int64_t n, const double* c, void* out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  QSX_VT_SWITCH(dtype, hipLaunchKernelGGL(k_lincomb<VT>, dim3(red_grid(n) * 4), dim3(256), 0, st, (const VT*)Q, m, n,
+                                          (const double2*)c, (VT*)out));
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// CSR trace and 1-norm
+// ---------------------------------------------------------------------------
+template <typename MT>
+__global__ void __launch_bounds__(256) k_csr_trace(const void* indptr, const void* indices, const MT* __restrict__ data,
+                                                   int idx64, int64_t Ns, double* partials, unsigned int* counter,
+                                                   double* out) {
+  double tr = 0.0, ti = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < Ns; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k0 = ld_idx(indptr, idx64, r), k1 = ld_idx(indptr, idx64, r + 1);
+    for (int64_t k = k0; k < k1; ++k)
+      if (ld_idx(indices, idx64, k) == r) { const double2 v = MTr<MT>::ld(data + k); tr += v.x; ti += v.y; }
+  }
+  block_finish<0>(tr, ti, partials, counter, out, 0.0);
+}
+
+template <typename MT>
+__global__ void __launch_bounds__(256) k_csr_colabs(const void* indptr, const void* indices, const MT* __restrict__ data,
+                                                    int idx64, int64_t Ns, double2 shift, double* __restrict__ colsum) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < Ns; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k0 = ld_idx(indptr, idx64, r), k1 = ld_idx(indptr, idx64, r + 1);
+    bool has_diag = false;
+    for (int64_t k = k0; k < k1; ++k) {
+      const int64_t c = ld_idx(indices, idx64, k);
+      double2 v = MTr<MT>::ld(data + k);
+      if (c == r) { v = cadd(v, shift); has_diag = true; }
+      atomicAdd(colsum + c, sqrt(v.x * v.x + v.y * v.y));
+    }
+    if (!has_diag) atomicAdd(colsum + r, sqrt(shift.x * shift.x + shift.y * shift.y));
+  }
+}
+
+__global__ void __launch_bounds__(256) k_max_reduce(const double* __restrict__ x, int64_t n, double* partials,
+                                                    unsigned int* counter, double* out) {
+  double m = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    m = fmax(m, x[i]);
+  block_finish<1>(m, 0.0, partials, counter, out, 0.0);
+}
+
+__global__ void k_fill_f64(double* p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+#define QSX_MT_SWITCH(dt, BODY)                                   \
+  switch (dt) {                                                   \
+    case QSX_F32: { using MT = float; BODY; } break;              \
+    case QSX_F64: { using MT = double; BODY; } break;             \
+    case QSX_C64: { using MT = float2; BODY; } break;             \
+    case QSX_C128: { using MT = double2; BODY; } break;           \
+    default: qsx_set_error("unknown dtype"); return QSX_ERR_ARG;  \
+  }
+
+// out (device, 2 doubles) = trace
+extern "C" int qsx_csr_trace(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
+                             int64_t Ns, double* out, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  RedScratch R;
+  int rc = get_red_scratch(R, st);
+  if (rc) return rc;
+  QSX_MT_SWITCH(mat_dtype, hipLaunchKernelGGL(k_csr_trace<MT>, dim3(red_grid(Ns)), dim3(256), 0, st, indptr, indices,
+                                              (const MT*)data, idx64, Ns, R.partials, R.counter, out));
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// out (device) = max_j sum_i |A_ij + shift delta_ij| ; colsum: Ns device doubles (scratch)
+extern "C" int qsx_csr_1_norm(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
+                              int64_t Ns, double shift_re, double shift_im, double* colsum, double* out,
+                              void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  RedScratch R;
+  int rc = get_red_scratch(R, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_fill_f64, dim3(red_grid(Ns)), dim3(256), 0, st, colsum, Ns, 0.0);
+  QSX_MT_SWITCH(mat_dtype, hipLaunchKernelGGL(k_csr_colabs<MT>, dim3(red_grid(Ns) * 4), dim3(256), 0, st, indptr,
+                                              indices, (const MT*)data, idx64, Ns, make_double2(shift_re, shift_im),
+                                              colsum));
+  hipLaunchKernelGGL(k_max_reduce, dim3(red_grid(Ns)), dim3(256), 0, st, colsum, Ns, R.partials, R.counter, out);
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Taylor-series expm: F <- e^{a(A - mu I)} ... e^{a mu} F, in place
+// ---------------------------------------------------------------------------
+// ctl layout (device doubles): [0] done flag (as int in the first 4 bytes),
+// [1] tol, then normB[s*(m+1)], normF[s*(m+1)].
+template <typename VT>
+__global__ void __launch_bounds__(256) k_expm_start(const VT* __restrict__ F, VT* __restrict__ B, int64_t n,
+                                                    int64_t nvec, double* __restrict__ normB0) {
+  double mx = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double rs = 0.0;
+    for (int64_t j = 0; j < nvec; ++j) {
+      const double2 v = VTr<VT>::ld(F + r * nvec + j);
+      VTr<VT>::st(B + r * nvec + j, v);
+      rs += VTr<VT>::cplx ? sqrt(v.x * v.x + v.y * v.y) : fabs(v.x);
+    }
+    mx = fmax(mx, rs);
+  }
+  __shared__ double sm[256];
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomic_max_nonneg(normB0, sm[0]);
+}
+
+// F *= eta ; B = F ; ||F|| -> normB0 of the next step ; done = 0
+template <typename VT>
+__global__ void __launch_bounds__(256) k_expm_scale(VT* __restrict__ F, VT* __restrict__ B, int64_t n, int64_t nvec,
+                                                    double2 eta, double* __restrict__ normB0, ExpmCtl* ctl) {
+  double mx = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double rs = 0.0;
+    for (int64_t j = 0; j < nvec; ++j) {
+      double2 v = VTr<VT>::ld(F + r * nvec + j);
+      if constexpr (VTr<VT>::cplx) v = cmul(eta, v); else v.x *= eta.x;
+      VTr<VT>::st(F + r * nvec + j, v);
+      VTr<VT>::st(B + r * nvec + j, v);
+      const double2 w = VTr<VT>::ld(F + r * nvec + j);
+      rs += VTr<VT>::cplx ? sqrt(w.x * w.x + w.y * w.y) : fabs(w.x);
+    }
+    mx = fmax(mx, rs);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = 0;
+  if (!normB0) return;
+  __shared__ double sm[256];
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomic_max_nonneg(normB0, sm[0]);
+}
+
+__global__ void k_expm_init_ctl(ExpmCtl* ctl, double tol, double* slots, int64_t nslots) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += (int64_t)gridDim.x * blockDim.x)
+    slots[i] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { ctl->done = 0; ctl->tol = tol; }
+}
+
+// Matrix-free epilogue: test previous term, F += B, norms.
+__global__ void k_expm_check(ExpmCtl* ctl, const double* c1, const double* c2, const double* nf) {
+  if (ctl->done) return;
+  if (*c1 + *c2 <= ctl->tol * *nf) ctl->done = 1;
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(256) k_expm_epi(const VT* __restrict__ B, VT* __restrict__ F, int64_t n, int64_t nvec,
+                                                  const ExpmCtl* ctl, double* normB, double* normF) {
+  if (ctl->done) return;
+  double mb = 0.0, mf = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double rb = 0.0, rf = 0.0;
+    for (int64_t j = 0; j < nvec; ++j) {
+      const double2 b = VTr<VT>::ld(B + r * nvec + j);
+      double2 f = cadd(VTr<VT>::ld(F + r * nvec + j), b);
+      VTr<VT>::st(F + r * nvec + j, f);
+      f = VTr<VT>::ld(F + r * nvec + j);
+      rb += VTr<VT>::cplx ? sqrt(b.x * b.x + b.y * b.y) : fabs(b.x);
+      rf += VTr<VT>::cplx ? sqrt(f.x * f.x + f.y * f.y) : fabs(f.x);
+    }
+    mb = fmax(mb, rb);
+    mf = fmax(mf, rf);
+  }
+  __shared__ double sb[256], sf[256];
+  sb[threadIdx.x] = mb;
+  sf[threadIdx.x] = mf;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) { sb[threadIdx.x] = fmax(sb[threadIdx.x], sb[threadIdx.x + o]); sf[threadIdx.x] = fmax(sf[threadIdx.x], sf[threadIdx.x + o]); }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { atomic_max_nonneg(normB, sb[0]); atomic_max_nonneg(normF, sf[0]); }
+}
+
+struct ExpmPlan {
+  int s, m;
+  double tol;
+  double2 a, mu;
+  int64_t Ns, nvec;
+  int vdt;
+};
+
+static int expm_ctl_alloc(const ExpmPlan& P, ExpmCtl** ctl, double** slots, hipStream_t st) {
+  const int64_t nsl = (int64_t)P.s * (P.m + 1) * 2;
+  QSX_CHECK_HIP(hipMallocAsync((void**)ctl, sizeof(ExpmCtl) + sizeof(double) * (nsl + 2), st));
+  *slots = (double*)((char*)*ctl + sizeof(ExpmCtl));
+  hipLaunchKernelGGL(k_expm_init_ctl, dim3(qsx_grid(nsl, 256) > 0 ? (unsigned)qsx_grid(nsl, 256) : 1u), dim3(256), 0, st,
+                     *ctl, P.tol, *slots, nsl);
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// CSR Taylor loop: v (Ns, nvec) is F and is overwritten; work has 2*Ns*nvec.
+extern "C" int qsx_expm_multiply_csr(const void* indptr, const void* indices, const void* data, int idx64,
+                                     int mat_dtype, int64_t Ns, int64_t nnz, int s, int m_star, double tol,
+                                     double mu_re, double mu_im, double a_re, double a_im, void* v, void* work,
+                                     int vec_dtype, int64_t nvec, int group, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (Ns <= 0) return QSX_OK;
+  if (s < 1 || m_star < 0) { qsx_set_error("qsx_expm_multiply_csr: bad partition"); return QSX_ERR_ARG; }
+  const bool real = (vec_dtype == QSX_F32 || vec_dtype == QSX_F64);
+  if (real && (a_im != 0.0 || mu_im != 0.0)) { qsx_set_error("real vectors with complex a or mu"); return QSX_ERR_COMPLEX; }
+  ExpmPlan P{s, m_star, tol, make_double2(a_re, a_im), make_double2(mu_re, mu_im), Ns, nvec, vec_dtype};
+  ExpmCtl* ctl = nullptr;
+  double* slots = nullptr;
+  int rc = expm_ctl_alloc(P, &ctl, &slots, st);
+  if (rc) return rc;
+  const int64_t stride = (int64_t)(m_star + 1);
+  double* normB = slots;
+  double* normF = slots + (int64_t)s * stride;
+  size_t esz = (vec_dtype == QSX_F32) ? 4 : (vec_dtype == QSX_F64 || vec_dtype == QSX_C64) ? 8 : 16;
+  void* Bb[2] = {work, (char*)work + esz * Ns * nvec};
+  const int G = group > 0 ? group : auto_group(Ns, nnz);
+  const unsigned rg = (unsigned)red_grid(Ns) * 4;
+  // eta = exp(a mu / s)
+  const double er = a_re * mu_re - a_im * mu_im, ei = a_re * mu_im + a_im * mu_re;
+  const double mag = std::exp(er / s);
+  const double2 eta = make_double2(mag * std::cos(ei / s), mag * std::sin(ei / s));
+  QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_start<VT>, dim3(rg), dim3(256), 0, st, (const VT*)v, (VT*)Bb[0], Ns,
+                                              nvec, normB));
+  QSX_LAUNCH_CHECK();
+  for (int i = 0; i < s; ++i) {
+    for (int j = 0; j < m_star; ++j) {
+      CsrArgs A;
+      std::memset(&A, 0, sizeof(A));
+      A.indptr = indptr; A.indices = indices; A.data = data; A.idx64 = idx64; A.Ns = Ns; A.nvec = nvec;
+      A.x = Bb[j & 1]; A.y = Bb[(j + 1) & 1];
+      const double cf = 1.0 / ((double)s * (double)(j + 1));
+      A.a = make_double2(a_re * cf, a_im * cf);
+      A.mu = P.mu;
+      A.beta = 0;
+      A.F = v;
+      A.ctl = ctl;
+      A.normB = normB + i * stride + j + 1;
+      A.normF = normF + i * stride + j + 1;
+      if (j >= 1) {
+        A.chk_c1 = normB + i * stride + j - 1;
+        A.chk_c2 = normB + i * stride + j;
+        A.chk_F = normF + i * stride + j;
+      }
+      rc = launch_csr<true>(A, mat_dtype, vec_dtype, G, st);
+      if (rc) return rc;
+    }
+    double* nb0 = (i + 1 < s) ? normB + (i + 1) * stride : nullptr;
+    QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_scale<VT>, dim3(rg), dim3(256), 0, st, (VT*)v, (VT*)Bb[0], Ns,
+                                                nvec, eta, nb0, ctl));
+    QSX_LAUNCH_CHECK();
+  }
+  (void)hipFreeAsync(ctl, st);
+  return QSX_OK;
+}
+
+// Matrix-free Taylor loop: same recurrence with H.v from the fused kernel.
+int qsx_hv_expm_term(void* hv_ctx, const void* x, void* y, double2 coef, double2 shift, const int* skip,
+                     hipStream_t st);
+
+extern "C" int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star, double tol, double mu_re,
+                                    double mu_im, double a_re, double a_im, void* v, void* work, int vec_dtype,
+                                    int64_t nvec, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (Ns <= 0) return QSX_OK;
+  if (s < 1 || m_star < 0) { qsx_set_error("qsx_expm_multiply_mf: bad partition"); return QSX_ERR_ARG; }
+  ExpmPlan P{s, m_star, tol, make_double2(a_re, a_im), make_double2(mu_re, mu_im), Ns, nvec, vec_dtype};
+  ExpmCtl* ctl = nullptr;
+  double* slots = nullptr;
+  int rc = expm_ctl_alloc(P, &ctl, &slots, st);
+  if (rc) return rc;
+  const int64_t stride = (int64_t)(m_star + 1);
+  double* normB = slots;
+  double* normF = slots + (int64_t)s * stride;
+  size_t esz = (vec_dtype == QSX_F32) ? 4 : (vec_dtype == QSX_F64 || vec_dtype == QSX_C64) ? 8 : 16;
+  void* Bb[2] = {work, (char*)work + esz * Ns * nvec};
+  const unsigned rg = (unsigned)red_grid(Ns) * 4;
+  const double er = a_re * mu_re - a_im * mu_im, ei = a_re * mu_im + a_im * mu_re;
+  const double mag = std::exp(er / s);
+  const double2 eta = make_double2(mag * std::cos(ei / s), mag * std::sin(ei / s));
+  QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_start<VT>, dim3(rg), dim3(256), 0, st, (const VT*)v, (VT*)Bb[0], Ns,
+                                              nvec, normB));
+  QSX_LAUNCH_CHECK();
+  for (int i = 0; i < s; ++i) {
+    for (int j = 0; j < m_star; ++j) {
+      if (j >= 1) {
+        hipLaunchKernelGGL(k_expm_check, dim3(1), dim3(1), 0, st, ctl, normB + i * stride + j - 1,
+                           normB + i * stride + j, normF + i * stride + j);
+      }
+      const double cf = 1.0 / ((double)s * (double)(j + 1));
+      const double2 coef = make_double2(a_re * cf, a_im * cf);
+      // shift = -coef * mu
+      const double2 shift = make_double2(-(coef.x * mu_re - coef.y * mu_im), -(coef.x * mu_im + coef.y * mu_re));
+      rc = qsx_hv_expm_term(hv_ctx, Bb[j & 1], Bb[(j + 1) & 1], coef, shift, &ctl->done, st);
+      if (rc) return rc;
+      QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_epi<VT>, dim3(rg), dim3(256), 0, st, (const VT*)Bb[(j + 1) & 1],
+                                                  (VT*)v, Ns, nvec, ctl, normB + i * stride + j + 1,
+                                                  normF + i * stride + j + 1));
+      QSX_LAUNCH_CHECK();
+    }
+    double* nb0 = (i + 1 < s) ? normB + (i + 1) * stride : nullptr;
+    QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_scale<VT>, dim3(rg), dim3(256), 0, st, (VT*)v, (VT*)Bb[0], Ns,
+                                                nvec, eta, nb0, ctl));
+    QSX_LAUNCH_CHECK();
+  }
+  (void)hipFreeAsync(ctl, st);
+  return QSX_OK;
+}

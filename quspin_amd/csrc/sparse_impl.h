@@ -1,0 +1,133 @@
+// CSR kernels shared by hamiltonian.dot and the Taylor-series expm.
+#pragma once
+#include "hv_impl.h"
+
+template <typename MT> struct MTr;
+template <> struct MTr<float> { static constexpr bool cplx = false; __device__ static double2 ld(const float* p) { return make_double2((double)*p, 0.0); } };
+template <> struct MTr<double> { static constexpr bool cplx = false; __device__ static double2 ld(const double* p) { return make_double2(*p, 0.0); } };
+template <> struct MTr<float2> { static constexpr bool cplx = true; __device__ static double2 ld(const float2* p) { float2 v = *p; return make_double2((double)v.x, (double)v.y); } };
+template <> struct MTr<double2> { static constexpr bool cplx = true; __device__ static double2 ld(const double2* p) { return *p; } };
+
+template <int G>
+__device__ __forceinline__ double grp_sum(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
+// Atomic max of a non-negative double through its bit pattern.
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
+  atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
+}
+
+__device__ __forceinline__ double cabs2(double2 z) { return sqrt(z.x * z.x + z.y * z.y); }
+
+// Taylor-loop control block (one per expm call).
+struct ExpmCtl {
+  int done;      // set when the truncation test of the current outer step passed
+  int pad;
+  double tol;
+  // normB[i*(m+1)+j], normF[i*(m+1)+j] follow in device memory
+};
+
+struct CsrArgs {
+  const void* indptr;   // int32 or int64
+  const void* indices;  // int32 or int64
+  const void* data;
+  int idx64;
+  int64_t Ns;
+  int64_t nvec;
+  const void* x;
+  void* y;
+  double2 a;            // y = a*(A x - mu x) [+ y if beta]
+  double2 mu;
+  int beta;
+  // fused Taylor epilogue (expm): F += y, norms, early exit
+  void* F;
+  ExpmCtl* ctl;
+  double* normB;        // slot for ||y||_inf (this term)
+  double* normF;        // slot for ||F||_inf (this term)
+  const double* chk_c1; // previous test inputs: c1, c2, ||F|| (null: no test)
+  const double* chk_c2;
+  const double* chk_F;
+};
+
+__device__ __forceinline__ int64_t ld_idx(const void* p, int idx64, int64_t i) {
+  return idx64 ? ((const int64_t*)p)[i] : (int64_t)((const int32_t*)p)[i];
+}
+
+// G lanes per row.  EPI: fused expm epilogue.
+template <typename MT, typename VT, int G, bool EPI>
+__global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
+  if constexpr (EPI) {
+    if (A.ctl->done) return;
+    if (A.chk_c1) {
+      // truncation test of the previous term (SCIPY:298-300)
+      const double c1 = *A.chk_c1, c2 = *A.chk_c2, nf = *A.chk_F;
+      if (c1 + c2 <= A.ctl->tol * nf) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl->done = 1;
+        return;
+      }
+    }
+  }
+  constexpr bool CPLX = VTr<VT>::cplx;
+  const int lane = threadIdx.x % G;
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
+  double rowB = 0.0, rowF = 0.0;
+  if (row < A.Ns) {
+    const int64_t k0 = ld_idx(A.indptr, A.idx64, row), k1 = ld_idx(A.indptr, A.idx64, row + 1);
+    const MT* __restrict__ val = (const MT*)A.data;
+    const VT* __restrict__ x = (const VT*)A.x;
+    VT* __restrict__ y = (VT*)A.y;
+    for (int64_t j = 0; j < A.nvec; ++j) {
+      double sr = 0.0, si = 0.0;
+      for (int64_t k = k0 + lane; k < k1; k += G) {
+        const int64_t c = ld_idx(A.indices, A.idx64, k);
+        const double2 m = MTr<MT>::ld(val + k);
+        const double2 v = VTr<VT>::ld(x + c * A.nvec + j);
+        if constexpr (CPLX) { sr += m.x * v.x - m.y * v.y; si += m.x * v.y + m.y * v.x; }
+        else sr += m.x * v.x;
+      }
+      sr = grp_sum<G>(sr);
+      if constexpr (CPLX) si = grp_sum<G>(si);
+      if (lane == 0) {
+        const int64_t o = row * A.nvec + j;
+        double2 acc = make_double2(sr, si);
+        const double2 xr = VTr<VT>::ld(x + o);
+        double2 res;
+        if constexpr (CPLX) {
+          acc = cadd(acc, make_double2(-(A.mu.x * xr.x - A.mu.y * xr.y), -(A.mu.x * xr.y + A.mu.y * xr.x)));
+          res = cmul(A.a, acc);
+        } else {
+          res = make_double2(A.a.x * (acc.x - A.mu.x * xr.x), 0.0);
+        }
+        if (A.beta) res = cadd(res, VTr<VT>::ld(y + o));
+        VTr<VT>::st(y + o, res);
+        if constexpr (EPI) {
+          VT* F = (VT*)A.F;
+          double2 f = VTr<VT>::ld(F + o);
+          // the stored (possibly rounded) B enters F, as in the reference
+          const double2 rs = VTr<VT>::ld(y + o);
+          f = cadd(f, rs);
+          VTr<VT>::st(F + o, f);
+          rowB += CPLX ? cabs2(rs) : fabs(rs.x);
+          const double2 fs = VTr<VT>::ld(F + o);
+          rowF += CPLX ? cabs2(fs) : fabs(fs.x);
+        }
+      }
+    }
+  }
+  if constexpr (EPI) {
+    // block max of the row sums, then one atomic per block
+    __shared__ double sB[256 / G], sF[256 / G];
+    const int gid = threadIdx.x / G;
+    if (lane == 0) { sB[gid] = rowB; sF[gid] = rowF; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double mb = 0.0, mf = 0.0;
+      for (int i = 0; i < 256 / G; ++i) { mb = fmax(mb, sB[i]); mf = fmax(mf, sF[i]); }
+      atomic_max_nonneg(A.normB, mb);
+      atomic_max_nonneg(A.normF, mf);
+    }
+  }
+}

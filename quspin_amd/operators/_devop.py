@@ -1,0 +1,266 @@
+"""Device operators used by ``hamiltonian``, ``quantum_LinearOperator`` and
+the tools: a CSR matrix in HBM and the matrix-free fused H.v, both exposing
+
+    apply(x, y, a=1, beta=0)   y <- beta*y + a*A x   (device tensors)
+    trace(), one_norm(shift), expm_inplace(...)
+
+All arithmetic runs in ``libquspin_hip.so``; torch only holds the buffers.
+"""
+
+import ctypes as C
+
+import numpy as np
+import scipy.sparse as sp
+
+from .. import _lib as L
+from .._lib import ptr, stream_handle
+
+
+def _vec_dtype_code(t):
+    return L.dtype_code(L.np_dtype(t.dtype))
+
+
+class DeviceCSR:
+    """CSR matrix resident in HBM (int32 or int64 indices)."""
+
+    is_matrix_free = False
+
+    def __init__(self, indptr, indices, data, shape, dtype):
+        self.indptr, self.indices, self.data = indptr, indices, data
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.idx64 = 1 if indptr.dtype == L.torch().int64 else 0
+        self.nnz = int(indices.shape[0])
+        self._host = None
+
+    @property
+    def Ns(self):
+        return self.shape[0]
+
+    @classmethod
+    def from_scipy(cls, A, dtype=None):
+        A = sp.csr_matrix(A)
+        if dtype is not None:
+            A = A.astype(dtype)
+        A.sum_duplicates()
+        dt = A.dtype
+        if dt not in (np.float32, np.float64, np.complex64, np.complex128):
+            dt = np.result_type(dt, np.float64)
+            A = A.astype(dt)
+        idx = np.int64 if (A.nnz >= 2 ** 31 - 1 or A.shape[0] >= 2 ** 31 - 1) else np.int32
+        ip = L.to_device(A.indptr.astype(idx))
+        ix = L.to_device(A.indices.astype(idx)) if A.nnz else L.empty(1, idx)[:0]
+        da = L.to_device(A.data) if A.nnz else L.empty(1, dt)[:0]
+        M = cls(ip, ix, da, A.shape, dt)
+        return M
+
+    def to_scipy(self):
+        if self._host is None:
+            self._host = sp.csr_matrix((L.to_host(self.data), L.to_host(self.indices), L.to_host(self.indptr)),
+                                       shape=self.shape)
+        return self._host
+
+    def apply(self, x, y, a=1.0, beta=0):
+        """y = a*A*x (+ y when beta)."""
+        vdt = _vec_dtype_code(x)
+        mdt = L.dtype_code(self.dtype)
+        nvec = int(np.prod(x.shape[1:])) if x.dim() > 1 else 1
+        ac = complex(a)
+        data = self._data_for(L.np_dtype(x.dtype))
+        mdt_use = L.dtype_code(L.np_dtype(data.dtype))
+        rc = L.load().qsx_csr_matvec(ptr(self.indptr), ptr(self.indices), ptr(data), self.idx64, mdt_use,
+                                     self.shape[0], self.nnz, ptr(x), ptr(y), vdt, nvec, ac.real, ac.imag,
+                                     int(bool(beta)), 0, stream_handle())
+        L.check(rc, "qsx_csr_matvec")
+
+    def _data_for(self, vdt):
+        """Values in the matrix dtype the kernel pairs with vector dtype
+        ``vdt``: the vector dtype itself, or its real part for a real matrix."""
+        vdt = np.dtype(vdt)
+        if self.dtype.kind == "c":
+            tgt = vdt if vdt.kind == "c" else np.dtype(np.complex128 if vdt == np.float64 else np.complex64)
+        else:
+            tgt = np.dtype(np.float64) if vdt in (np.float64, np.complex128) else np.dtype(np.float32)
+        if tgt == self.dtype:
+            return self.data
+        if not hasattr(self, "_casts"):
+            self._casts = {}
+        if tgt not in self._casts:
+            self._casts[tgt] = self.data.to(L.torch_dtype(tgt))
+        return self._casts[tgt]
+
+    def trace(self):
+        out = L.zeros(2, np.float64)
+        rc = L.load().qsx_csr_trace(ptr(self.indptr), ptr(self.indices), ptr(self.data), self.idx64,
+                                    L.dtype_code(self.dtype), self.shape[0], ptr(out), stream_handle())
+        L.check(rc, "qsx_csr_trace")
+        o = L.to_host(out)
+        return complex(o[0], o[1]) if self.dtype.kind == "c" else float(o[0])
+
+    def one_norm(self, shift=0.0):
+        colsum = L.empty(max(self.shape[0], 1), np.float64)
+        out = L.zeros(2, np.float64)
+        sc = complex(shift)
+        rc = L.load().qsx_csr_1_norm(ptr(self.indptr), ptr(self.indices), ptr(self.data), self.idx64,
+                                     L.dtype_code(self.dtype), self.shape[0], sc.real, sc.imag, ptr(colsum),
+                                     ptr(out), stream_handle())
+        L.check(rc, "qsx_csr_1_norm")
+        return float(L.to_host(out)[0])
+
+    def expm_inplace(self, v, work, s, m_star, tol, mu, a):
+        """Taylor loop of ``_wrapper_expm_multiply[_batch]`` on device."""
+        vdt = _vec_dtype_code(v)
+        nvec = int(np.prod(v.shape[1:])) if v.dim() > 1 else 1
+        data = self._data_for(L.np_dtype(v.dtype))
+        mdt = L.dtype_code(L.np_dtype(data.dtype))
+        mu, a = complex(mu), complex(a)
+        rc = L.load().qsx_expm_multiply_csr(ptr(self.indptr), ptr(self.indices), ptr(data), self.idx64, mdt,
+                                            self.shape[0], self.nnz, int(s), int(m_star), float(tol), mu.real,
+                                            mu.imag, a.real, a.imag, ptr(v), ptr(work), vdt, nvec, 0,
+                                            stream_handle())
+        L.check(rc, "qsx_expm_multiply_csr")
+
+
+class MatrixFreeOp:
+    """Fused matrix-free H.v over a device basis and operator list.
+
+    ``transposed``/``conjugated`` act on the off-diagonal terms, ``diag_conj``
+    on the diagonal terms; ``diag`` is an optional extra device diagonal."""
+
+    is_matrix_free = True
+
+    def __init__(self, basis, oplist, dtype, transposed=False, conjugated=False, diag_conj=False, diag=None):
+        self.basis = basis
+        self.oplist = oplist
+        self.dtype = np.dtype(dtype)
+        self.transposed, self.conjugated, self.diag_conj = bool(transposed), bool(conjugated), bool(diag_conj)
+        self.diag = diag
+        self._ctx = {}
+        self.shape = (basis.Ns, basis.Ns)
+
+    @property
+    def Ns(self):
+        return self.basis.Ns
+
+    def _context(self, vdt, nvec):
+        key = (vdt, nvec)
+        if key not in self._ctx:
+            h = L._vp()
+            dcplx = 0
+            if self.diag is not None:
+                dcplx = 1 if L.np_dtype(self.diag.dtype).kind == "c" else 0
+            rc = L.load().qsx_hv_ctx_create(self.basis._core.h, self.oplist.h, *self.basis._dev_args(),
+                                            L.dtype_code(vdt), nvec, int(self.transposed), int(self.conjugated),
+                                            int(self.diag_conj), ptr(self.diag), dcplx, C.byref(h))
+            L.check(rc, "qsx_hv_ctx_create")
+            self._ctx[key] = _Ctx(h)
+        return self._ctx[key]
+
+    def apply(self, x, y, a=1.0, beta=0, a_diag=None):
+        vdt = L.np_dtype(x.dtype)
+        nvec = int(np.prod(x.shape[1:])) if x.dim() > 1 else 1
+        ctx = self._context(vdt, nvec)
+        ac = complex(a)
+        ad = ac if a_diag is None else complex(a_diag)
+        rc = L.load().qsx_hv_ctx_apply(ctx.h, ptr(x), ptr(y), ac.real, ac.imag, ad.real, ad.imag,
+                                       int(bool(beta)), stream_handle())
+        L.check(rc, "qsx_hv_ctx_apply")
+
+    def diagonal(self, dtype=np.complex128):
+        """Diagonal of the operator as a device vector."""
+        ones = L.torch().ones(self.Ns, dtype=L.torch_dtype(dtype), device="cuda")
+        out = L.zeros(self.Ns, dtype)
+        self.apply(ones, out, a=0.0, a_diag=1.0)
+        return out
+
+    def trace(self):
+        d = self.diagonal(np.complex128)
+        ones = L.torch().ones(self.Ns, dtype=L.torch().complex128, device="cuda")
+        return vdot_dev(ones, d)
+
+    def one_norm(self, shift=0.0):
+        """max_c sum_r |A_rc + shift delta_rc| through an explicit CSR build
+        (cheap next to the expm it feeds)."""
+        csr = self.to_csr()
+        return csr.one_norm(shift)
+
+    def to_csr(self, dtype=np.complex128):
+        return build_csr(self.basis, self.oplist, dtype, self.transposed, self.conjugated, diag=self.diag)
+
+    def expm_inplace(self, v, work, s, m_star, tol, mu, a):
+        vdt = L.np_dtype(v.dtype)
+        nvec = int(np.prod(v.shape[1:])) if v.dim() > 1 else 1
+        ctx = self._context(vdt, nvec)
+        mu, a = complex(mu), complex(a)
+        rc = L.load().qsx_expm_multiply_mf(ctx.h, self.Ns, int(s), int(m_star), float(tol), mu.real, mu.imag,
+                                           a.real, a.imag, ptr(v), ptr(work), L.dtype_code(vdt), nvec,
+                                           stream_handle())
+        L.check(rc, "qsx_expm_multiply_mf")
+
+
+class _Ctx:
+    def __init__(self, h):
+        self.h = h
+
+    def __del__(self):
+        try:
+            if L._lib is not None:
+                L._lib.qsx_hv_ctx_destroy(self.h)
+        except Exception:
+            pass
+
+
+def build_csr(basis, oplist, dtype, transposed=False, conjugated=False, a=1.0, diag=None):
+    """Assemble a*Op as a canonical device CSR (``qsx_csr_builder_*``)."""
+    lib = L.load()
+    dtype = np.dtype(dtype)
+    h = L._vp()
+    nnz = L._i64()
+    ac = complex(a)
+    rc = lib.qsx_csr_builder_create(basis._core.h, oplist.h, *basis._dev_args(), int(transposed), int(conjugated),
+                                    ac.real, ac.imag, C.byref(h), C.byref(nnz), stream_handle())
+    L.check(rc, "qsx_csr_builder_create")
+    try:
+        n = nnz.value
+        Ns = basis.Ns
+        idx = np.int64 if (n >= 2 ** 31 - 1 or Ns >= 2 ** 31 - 1) else np.int32
+        indptr = L.empty(Ns + 1, idx)
+        indices = L.empty(max(n, 1), idx)
+        data = L.empty(max(n, 1), dtype)
+        rc = lib.qsx_csr_builder_finish(h, L.dtype_code(dtype), np.dtype(idx).itemsize, ptr(indptr), ptr(indices),
+                                        ptr(data), stream_handle())
+        L.check(rc, "qsx_csr_builder_finish")
+    finally:
+        lib.qsx_csr_builder_destroy(h)
+    M = DeviceCSR(indptr, indices[:n], data[:n], (basis.Ns, basis.Ns), dtype)
+    if diag is not None:
+        D = sp.diags(L.to_host(diag).astype(dtype), format="csr")
+        M = DeviceCSR.from_scipy(M.to_scipy() + D, dtype)
+    return M
+
+
+def as_device_operator(A, dtype=None):
+    """Device operator for the tools: our operators, scipy sparse or dense."""
+    if hasattr(A, "_qsx_operator"):
+        return A._qsx_operator()
+    if isinstance(A, (DeviceCSR, MatrixFreeOp)):
+        return A
+    if sp.issparse(A) or isinstance(A, np.ndarray):
+        return DeviceCSR.from_scipy(sp.csr_matrix(A), dtype)
+    return None
+
+
+def vdot_dev(x, y):
+    """sum conj(x) y on the device (FP64 accumulation, deterministic)."""
+    out = L.zeros(2, np.float64)
+    rc = L.load().qsx_vdot(_vec_dtype_code(x), ptr(x), ptr(y), int(x.numel()), ptr(out), stream_handle())
+    L.check(rc, "qsx_vdot")
+    o = L.to_host(out)
+    return complex(o[0], o[1])
+
+
+def nrm2_dev(x):
+    out = L.zeros(2, np.float64)
+    rc = L.load().qsx_nrm2(_vec_dtype_code(x), ptr(x), int(x.numel()), ptr(out), stream_handle())
+    L.check(rc, "qsx_nrm2")
+    return float(L.to_host(out)[0])

