@@ -128,3 +128,45 @@ def check_symm(basis, static, dynamic):
         if not _equal(d, t):
             raise TypeError("Symmetry check failed! To turn off check, use check_symm=False in hamiltonian.")
     print("Symmetry checks passed!")
+
+
+def is_symmetric(basis, op_list):
+    """True when the (opstr, indx, J) list commutes with every symmetry map
+    of ``basis`` (then the gather form of H.v is exact); False when it does
+    not or when the check cannot decide."""
+    if not len(basis._pers):
+        return True
+    static = [[opstr, [[J] + list(indx)]] for opstr, indx, J in op_list]
+    try:
+        d = _canonical(basis, static, [])
+    except Exception:
+        return False
+    for m in basis._maps:
+        m = np.asarray(m)
+        inv = m < 0
+        perm = np.where(inv, -(m + 1), m)
+        pinv = np.empty_like(perm)
+        pinv[perm] = np.arange(len(perm))
+        t = {}
+        for (s, ix, key), c in d.items():
+            s2, ix2 = [], []
+            cc = c
+            for ch, j in zip(s, ix):
+                i = int(pinv[j])
+                if inv[i]:
+                    if ch == "z":
+                        cc = -cc
+                    elif ch == "+":
+                        ch = "-"
+                    elif ch == "-":
+                        ch = "+"
+                    elif ch == "n":
+                        return False
+                s2.append(ch)
+                ix2.append(i)
+            s3, ix3, c3 = basis._sort_opstr(("".join(s2), tuple(ix2), cc))
+            k = (s3, tuple(ix3), key)
+            t[k] = t.get(k, 0) + c3
+        if not _equal(d, t):
+            return False
+    return True

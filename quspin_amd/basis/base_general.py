@@ -335,6 +335,33 @@ class basis_general:
         b = self._dev.get("end")
         return np.array([], np.intp) if b is None else L.to_host(b)
 
+    def lookup_table(self, max_bytes=None):
+        """Device table mapping every state of the particle sector (by
+        combinadic rank) to (basis index, group element) of its
+        representative, built on first use; None when the basis has no
+        symmetry, is not a single sector, or the table would exceed
+        ``max_bytes`` (default: 40 GiB)."""
+        if not self._made_basis or not len(self._pers):
+            return None
+        if "table" in self._dev:
+            return self._dev["table"]
+        lib = L.load()
+        ent, eb, ib = L._i64(), L._i32(), L._i32()
+        rc = lib.qsx_table_info(self._core.h, self._Ns, L.C.byref(ent), L.C.byref(eb), L.C.byref(ib))
+        if rc != 0:
+            self._dev["table"] = None
+            return None
+        nbytes = ent.value * eb.value
+        if nbytes > (max_bytes if max_bytes is not None else 40 * 2 ** 30):
+            return None
+        tab = L.empty(max(ent.value, 1), np.int32 if eb.value == 4 else np.int64)
+        rc = lib.qsx_table_build(self._core.h, ptr(self._dev["basis"]), self._Ns, ptr(self._dev.get("begin")),
+                                 ptr(self._dev.get("end")), self._N_p, ptr(tab), eb.value, ib.value,
+                                 stream_handle())
+        L.check(rc, "qsx_table_build")
+        self._dev["table"] = (tab, eb.value, ib.value)
+        return self._dev["table"]
+
     def _dev_args(self):
         """Positional basis arguments of the C-ABI calls."""
         d = self._dev
@@ -475,8 +502,10 @@ class basis_general:
             idx.extend(int(i) for i in indx)
             Jc = complex(J)
             Js.extend([Jc.real, Jc.imag])
+        from ._checks import is_symmetric
+        sym = is_symmetric(self, op_list)
         return _OpList(self._core, "".join(opstrs), np.asarray(lens, np.int32), np.asarray(idx, np.int32),
-                       np.asarray(Js, np.float64))
+                       np.asarray(Js, np.float64), sym)
 
     def inplace_Op(self, v_in, op_list, dtype, transposed=False, conjugated=False, a=1.0, v_out=None):
         """v_out += a * Op v_in for the operator list (``inplace_Op``
@@ -545,11 +574,13 @@ class basis_general:
 class _OpList:
     """Device operator list handle."""
 
-    def __init__(self, core, opstrs, lens, idx, Js):
+    def __init__(self, core, opstrs, lens, idx, Js, symmetric=True):
         self.core = core
+        self.symmetric = bool(symmetric)
         h = L._vp()
         rc = L.load().qsx_oplist_create(core.h, len(lens), opstrs.encode(), lens.ctypes.data_as(L._vp),
-                                        idx.ctypes.data_as(L._vp), Js.ctypes.data_as(L._vp), L.C.byref(h))
+                                        idx.ctypes.data_as(L._vp), Js.ctypes.data_as(L._vp), int(symmetric),
+                                        L.C.byref(h))
         L.check(rc, "qsx_oplist_create")
         self.h = h
         no, nd, cx = L._i32(), L._i32(), L._i32()

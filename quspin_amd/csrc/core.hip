@@ -454,14 +454,15 @@ extern "C" int64_t qsx_make_basis(qsx_core* c, const int32_t* Np, int n_Np, void
   std::memset(&E, 0, sizeof(E));
   RankDev& R = c->rank_h;
   c->rank_ok = 0;
+  c->sector_ok = 0;
   if (c->h.kind != KIND_SPINFUL) {
     if (n_Np == 1 && Np[0] >= 0 && Np[0] <= N && Np[0] <= 32) {
       E.mode = 0;
       E.Np = Np[0];
       E.total = binom_host(N, Np[0]);
-      if (c->h.nmaps == 0) {
-        c->rank_ok = 1; R.mode = 0; R.Np = Np[0]; R.Ns = E.total;
-      }
+      R.mode = 0; R.Np = Np[0]; R.Ns = E.total;
+      c->sector_ok = 1; c->sector_total = E.total;
+      if (c->h.nmaps == 0) c->rank_ok = 1;
     } else {
       if (N > 40) { qsx_set_error("qsx_make_basis: several or no particle sectors need N <= 40"); return QSX_ERR_UNSUPPORTED; }
       E.mode = 1;
@@ -471,7 +472,11 @@ extern "C" int64_t qsx_make_basis(qsx_core* c, const int32_t* Np, int n_Np, void
         if (Np[i] < 0 || Np[i] > N) { qsx_set_error("particle number Nb must satisfy: 0 <= Nb <= N"); return QSX_ERR_ARG; }
         E.popmask |= 1ull << Np[i];
       }
-      if (c->h.nmaps == 0 && n_Np == 0 && N <= 32) { c->rank_ok = 1; R.mode = 2; R.Ns = E.total; }
+      if (n_Np == 0 && N <= 32) {
+        R.mode = 2; R.Ns = E.total;
+        c->sector_ok = 1; c->sector_total = E.total;
+        if (c->h.nmaps == 0) c->rank_ok = 1;
+      }
     }
   } else {
     if (n_Np == 1) {
@@ -479,9 +484,9 @@ extern "C" int64_t qsx_make_basis(qsx_core* c, const int32_t* Np, int n_Np, void
       E.Nup = Np[0]; E.Ndn = Np[1];
       if (E.Nup < 0 || E.Nup > L || E.Ndn < 0 || E.Ndn > L) { qsx_set_error("particle numbers in Nf must satisfy: 0 <= n <= N"); return QSX_ERR_ARG; }
       E.total = binom_host(L, E.Nup) * binom_host(L, E.Ndn);
-      if (c->h.nmaps == 0 && c->h.double_occ) {
-        c->rank_ok = 1; R.mode = 1; R.Nup = E.Nup; R.Ndn = E.Ndn; R.Ns = E.total; R.cdn = binom_host(L, E.Ndn);
-      }
+      R.mode = 1; R.Nup = E.Nup; R.Ndn = E.Ndn; R.Ns = E.total; R.cdn = binom_host(L, E.Ndn);
+      c->sector_ok = 1; c->sector_total = E.total;
+      if (c->h.nmaps == 0 && c->h.double_occ) c->rank_ok = 1;
     } else {
       if (N > 40) { qsx_set_error("qsx_make_basis: several or no particle sectors need 2L <= 40"); return QSX_ERR_UNSUPPORTED; }
       E.mode = 3;
@@ -630,6 +635,113 @@ extern "C" int qsx_normalization(qsx_core* c, const void* states, void* out, int
   } else {
     if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_normalization<uint64_t, MM, true>), dim3(qsx_grid(n, blk)), dim3(blk), 0, st, c->d, c->phase, (const uint64_t*)states, out, out_bytes, n)); }
     else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_normalization<uint64_t, MM, false>), dim3(qsx_grid(n, blk)), dim3(blk), 0, st, c->d, c->phase, (const uint64_t*)states, out, out_bytes, n)); }
+  }
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// sector lookup table: for every state of the enumerated particle sector (in
+// rank order) the basis index of its representative and the code of the
+// group element reaching it; absent states hold all ones.
+// ---------------------------------------------------------------------------
+template <typename I>
+__device__ __forceinline__ I sector_state(const RankDev* __restrict__ R, int64_t rk) {
+  if (R->mode == 0) return R->Np == 0 ? (I)0 : unrank_comb<I>(R, rk, R->Np, R->N);
+  if (R->mode == 1) {
+    const int64_t iu = rk / R->cdn, id = rk % R->cdn;
+    const I up = R->Nup == 0 ? (I)0 : unrank_comb<I>(R, iu, R->Nup, R->L);
+    const I dn = R->Ndn == 0 ? (I)0 : unrank_comb<I>(R, id, R->Ndn, R->L);
+    return (up << R->L) | dn;
+  }
+  return (I)rk;
+}
+
+template <typename I>
+__device__ __forceinline__ I sector_next(const RankDev* __restrict__ R, I s, int64_t rk_next) {
+  if (R->mode == 0) return R->Np == 0 ? s : gosper_next<I>(s);
+  if (R->mode == 1) {
+    if (rk_next % R->cdn == 0) return sector_state<I>(R, rk_next);
+    const I mlo = (((I)1) << R->L) - 1;
+    return (s & ~mlo) | gosper_next<I>(s & mlo);
+  }
+  return s + 1;
+}
+
+template <typename I, int M, bool FERM>
+__global__ void __launch_bounds__(256) k_table(const CoreDev* __restrict__ C, const RankDev* __restrict__ R,
+                                               const I* __restrict__ basis, int64_t Ns, const int64_t* __restrict__ begin,
+                                               const int64_t* __restrict__ end, int N_p, int64_t total, int64_t chunk,
+                                               void* __restrict__ table, int u64, int idx_bits) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t a = t * chunk;
+  if (a >= total) return;
+  const int64_t b = min(total, a + chunk);
+  I s = sector_state<I>(R, a);
+  for (int64_t rk = a; rk < b; ++rk) {
+    OrbitRes O;
+    const I rep = orbit_max<I, M, FERM>(C, s, O);
+    int64_t lo = 0, hi = Ns;
+    if (N_p > 0) {
+      const int64_t p = (int64_t)((uint64_t)rep >> (C->N - N_p));
+      lo = begin[p];
+      hi = end[p];
+    }
+    const int64_t idx = bsearch_desc<I>(basis, lo, hi, rep);
+    if (u64) {
+      uint64_t e = ~0ull;
+      if (idx >= 0) e = (uint64_t)idx | ((uint64_t)O.code << 40) | (O.sign < 0 ? (1ull << 63) : 0ull);
+      ((uint64_t*)table)[rk] = e;
+    } else {
+      uint32_t e = 0xFFFFFFFFu;
+      if (idx >= 0) e = (uint32_t)idx | ((uint32_t)O.code << idx_bits);
+      ((uint32_t*)table)[rk] = e;
+    }
+    if (rk + 1 < b) s = sector_next<I>(R, s, rk + 1);
+  }
+}
+
+// Size and layout of the lookup table for the sector of the last make_basis:
+// entries, bytes per entry (4 or 8) and index bits of the 4-byte layout.
+// Returns QSX_ERR_UNSUPPORTED when the basis was not built from one sector.
+extern "C" int qsx_table_info(const qsx_core* c, int64_t Ns, int64_t* entries, int* entry_bytes, int* idx_bits) {
+  if (!c || !c->sector_ok || c->h.nmaps == 0) {
+    qsx_set_error("lookup table needs a symmetric basis made from one particle sector");
+    return QSX_ERR_UNSUPPORTED;
+  }
+  int ib = 1;
+  while (ib < 40 && ((int64_t)1 << ib) <= Ns) ++ib;
+  int cb = 0;
+  while ((1 << cb) < c->h.G) ++cb;
+  *entries = c->sector_total;
+  *idx_bits = ib;
+  *entry_bytes = (ib + cb <= 31 && !c->h.fermion) ? 4 : 8;
+  if (*entry_bytes == 8 && (Ns >= (1ll << 40) || c->h.G > (1 << 23))) {
+    qsx_set_error("lookup table: basis or group too large");
+    return QSX_ERR_UNSUPPORTED;
+  }
+  return QSX_OK;
+}
+
+extern "C" int qsx_table_build(qsx_core* c, const void* basis, int64_t Ns, const int64_t* begin, const int64_t* end,
+                               int N_p, void* table, int entry_bytes, int idx_bits, void* stream) {
+  if (!c || !c->sector_ok || (entry_bytes != 4 && entry_bytes != 8)) {
+    qsx_set_error("qsx_table_build: bad arguments");
+    return QSX_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t total = c->sector_total;
+  const int64_t chunk = 16;
+  const int64_t nth = (total + chunk - 1) / chunk;
+  const dim3 grid((unsigned)qsx_grid(nth, 256)), blk(256);
+  const int u64 = entry_bytes == 8;
+  const bool ferm = c->h.fermion != 0;
+  if (c->h.N <= 32) {
+    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits)); }
+    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits)); }
+  } else {
+    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits)); }
+    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits)); }
   }
   QSX_LAUNCH_CHECK();
   return QSX_OK;

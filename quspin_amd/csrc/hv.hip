@@ -66,15 +66,17 @@ struct qsx_oplist_full {
   TermDev* off_hc = nullptr;  // device, Hermitian-conjugate off-diagonal terms
   std::vector<TermDev> off_hc_h;
   int real_ok = 1;            // all factors real for real dtypes
+  int symmetric = 1;          // list commutes with the symmetry group (gather form exact)
 };
 
 extern "C" int qsx_oplist_create(qsx_core* c, int n_terms, const char* opstrs, const int32_t* lens,
-                                 const int32_t* indx, const double* J, qsx_oplist** out) {
+                                 const int32_t* indx, const double* J, int symmetric, qsx_oplist** out) {
   if (!c || !out || n_terms < 0 || n_terms > QSX_MAX_TERMS) {
     qsx_set_error("qsx_oplist_create: bad arguments (at most 4096 terms)");
     return QSX_ERR_ARG;
   }
   auto* L = new qsx_oplist_full();
+  L->symmetric = symmetric ? 1 : 0;
   int pos_s = 0, pos_i = 0;
   for (int t = 0; t < n_terms; ++t) {
     TermDev T, Th;
@@ -136,14 +138,46 @@ static int pick_lookup(const qsx_core* c, int64_t Ns) {
   return LOOKUP_SEARCH;
 }
 
-static int dispatch_hv(const HvArgs& A, int dtype, int N, int nmaps, bool fm, int lookup, hipStream_t st) {
+static int dispatch_hv(const HvArgs& A, int dtype, int N, int nmaps, bool fm, int lookup, int scatter,
+                       hipStream_t st) {
   switch (dtype) {
-    case QSX_F32: return qsx_hv_launch_f32(A, N, nmaps, fm, lookup, st);
-    case QSX_F64: return qsx_hv_launch_f64(A, N, nmaps, fm, lookup, st);
-    case QSX_C64: return qsx_hv_launch_c64(A, N, nmaps, fm, lookup, st);
-    case QSX_C128: return qsx_hv_launch_c128(A, N, nmaps, fm, lookup, st);
+    case QSX_F32: return qsx_hv_launch_f32(A, N, nmaps, fm, lookup, scatter, st);
+    case QSX_F64: return qsx_hv_launch_f64(A, N, nmaps, fm, lookup, scatter, st);
+    case QSX_C64: return qsx_hv_launch_c64(A, N, nmaps, fm, lookup, scatter, st);
+    case QSX_C128: return qsx_hv_launch_c128(A, N, nmaps, fm, lookup, scatter, st);
     default: qsx_set_error("unknown dtype code"); return QSX_ERR_ARG;
   }
+}
+
+// Launch v_out = beta v_out + (a_diag D + shift) v_in + a_off Op v_in.  A
+// carries everything except the off-diagonal terms.  Symmetric lists (and
+// every transposed product) use the one-launch gather; otherwise the diagonal
+// launch is followed by the scatter launch with the original terms.
+static int hv_launch(const qsx_core* c, const qsx_oplist_full* L, HvArgs A, int transposed, int conjugated,
+                     int dtype, hipStream_t st) {
+  const int lookup = A.table ? LOOKUP_TABLE : pick_lookup(c, A.Ns);
+  const int N = c->h.N, nm = c->h.nmaps;
+  const bool fm = c->h.fermion != 0;
+  const bool scatter = nm > 0 && !L->symmetric && !transposed && L->base.n_off > 0;
+  if (!scatter) {
+    A.off = transposed ? L->base.off : L->off_hc;
+    A.n_off = L->base.n_off;
+    A.off_conj = (transposed == conjugated) ? 1 : 0;
+    return dispatch_hv(A, dtype, N, nm, fm, lookup, 0, st);
+  }
+  HvArgs D = A;
+  D.off = nullptr;
+  D.n_off = 0;
+  int rc = dispatch_hv(D, dtype, N, nm, fm, lookup, 0, st);
+  if (rc) return rc;
+  HvArgs S = A;
+  S.off = L->base.off;
+  S.n_off = L->base.n_off;
+  S.off_conj = conjugated ? 1 : 0;
+  S.dg = nullptr;
+  S.n_diag = 0;
+  S.diag_arr = nullptr;
+  return dispatch_hv(S, dtype, N, nm, fm, lookup, 1, st);
 }
 
 // v_out = beta*v_out + a_diag*(D + diag_arr) v_in + a_off*Op v_in
@@ -172,9 +206,6 @@ extern "C" int qsx_hv(qsx_core* c, qsx_oplist* l, const void* basis, const void*
   A.C = c->d;
   A.ph = c->phase;
   A.R = c->rank_d;
-  A.off = transposed ? L->base.off : L->off_hc;
-  A.n_off = L->base.n_off;
-  A.off_conj = (transposed == conjugated) ? 1 : 0;
   A.dg = L->base.diag;
   A.n_diag = L->base.n_diag;
   A.diag_conj = diag_conj ? 1 : 0;
@@ -193,7 +224,7 @@ extern "C" int qsx_hv(qsx_core* c, qsx_oplist* l, const void* basis, const void*
   A.alpha_off = make_double2(a_off_re, a_off_im);
   A.alpha_diag = make_double2(a_diag_re, a_diag_im);
   A.beta = beta ? 1 : 0;
-  return dispatch_hv(A, dtype, c->h.N, c->h.nmaps, c->h.fermion != 0, pick_lookup(c, Ns), (hipStream_t)stream);
+  return hv_launch(c, L, A, transposed, conjugated, dtype, (hipStream_t)stream);
 }
 
 // Reference-compatible single term: v_out += J Op v_in (Op^T / Op^* per flags).
@@ -214,27 +245,33 @@ extern "C" int qsx_inplace_op(qsx_core* c, const void* v_in, void* v_out, int dt
     qsx_set_error("real dtype requested for an operator with complex matrix elements");
     return QSX_ERR_COMPLEX;
   }
+  // a single term is handled as a one-term list that is not assumed to
+  // commute with the group: transposed -> gather with the term, otherwise
+  // scatter (the reference's v_out[row] += ME v_in[col])
+  qsx_oplist_full one;
+  one.symmetric = 0;
   TermDev* d = nullptr;
-  QSX_CHECK_HIP(hipMallocAsync((void**)&d, sizeof(TermDev), st));
-  const bool diag = (T.flip == 0);
-  const TermDev& use = diag ? T : (transposed ? T : Th);
-  QSX_CHECK_HIP(hipMemcpyAsync(d, &use, sizeof(TermDev), hipMemcpyHostToDevice, st));
+  QSX_CHECK_HIP(hipMallocAsync((void**)&d, 2 * sizeof(TermDev), st));
+  QSX_CHECK_HIP(hipMemcpyAsync(d, &T, sizeof(TermDev), hipMemcpyHostToDevice, st));
+  QSX_CHECK_HIP(hipMemcpyAsync(d + 1, &Th, sizeof(TermDev), hipMemcpyHostToDevice, st));
+  if (T.flip == 0) {
+    one.base.diag = d; one.base.n_diag = 1;
+  } else {
+    one.base.off = d; one.off_hc = d + 1; one.base.n_off = 1;
+  }
   HvArgs A;
   std::memset(&A, 0, sizeof(A));
   A.C = c->d;
   A.ph = c->phase;
   A.R = c->rank_d;
-  if (diag) {
-    A.dg = d; A.n_diag = 1; A.diag_conj = conjugated ? 1 : 0;
-  } else {
-    A.off = d; A.n_off = 1; A.off_conj = (transposed == conjugated) ? 1 : 0;
-  }
+  A.dg = one.base.diag; A.n_diag = one.base.n_diag; A.diag_conj = conjugated ? 1 : 0;
   A.basis = basis; A.nrm = nrm; A.n_bytes = n_bytes; A.begin = begin; A.end = end; A.N_p = N_p;
   A.v_in = v_in; A.v_out = v_out; A.Ns = Ns; A.nvec = nvec;
   A.alpha_off = make_double2(1.0, 0.0);
   A.alpha_diag = make_double2(1.0, 0.0);
   A.beta = 1;
-  rc = dispatch_hv(A, dtype, c->h.N, c->h.nmaps, c->h.fermion != 0, pick_lookup(c, Ns), st);
+  rc = hv_launch(c, &one, A, transposed, conjugated, dtype, st);
+  one.base.off = nullptr; one.off_hc = nullptr; one.base.diag = nullptr;
   (void)hipFreeAsync(d, st);
   return rc;
 }
@@ -423,6 +460,10 @@ extern "C" int qsx_csr_builder_create(qsx_core* c, qsx_oplist* l, const void* ba
   if (!c || !l || !out || !nnz) { qsx_set_error("qsx_csr_builder_create: bad arguments"); return QSX_ERR_ARG; }
   hipStream_t st = (hipStream_t)stream;
   auto* L = reinterpret_cast<qsx_oplist_full*>(l);
+  if (c->h.nmaps > 0 && !L->symmetric && !transposed) {
+    qsx_set_error("CSR rows of a list that does not commute with the symmetry group: assemble the transpose");
+    return QSX_ERR_UNSUPPORTED;
+  }
   auto* B = new qsx_csr_builder();
   B->Ns = Ns;
   B->complex_vals = (!L->real_ok || !core_chars_real(c) || a_im != 0.0) ? 1 : 0;
@@ -501,10 +542,27 @@ extern "C" int qsx_csr_builder_finish(qsx_csr_builder* B, int dtype, int idx_byt
 // reusable H.v context (fixed operator, basis, dtype, nvec and flags)
 // ---------------------------------------------------------------------------
 struct qsx_hv_ctx {
-  HvArgs A;
-  int dtype, N, nmaps, lookup;
-  bool fm;
+  HvArgs A;  // everything but the off-diagonal terms
+  const qsx_core* c;
+  const qsx_oplist_full* L;
+  int dtype, transposed, conjugated;
 };
+
+// Use a sector lookup table (qsx_table_build) instead of the orbit search.
+extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_bytes, int idx_bits) {
+  if (!X || (table && entry_bytes != 4 && entry_bytes != 8)) { qsx_set_error("qsx_hv_ctx_set_table: bad arguments"); return QSX_ERR_ARG; }
+  X->A.table = table;
+  X->A.table_u64 = entry_bytes == 8;
+  X->A.idx_bits = idx_bits;
+  return QSX_OK;
+}
+
+// 0: one lane per row; 1: compacted work queue (single vector)
+extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
+  if (!X || sched < 0 || sched > 1) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
+  X->A.sched = sched;
+  return QSX_OK;
+}
 
 extern "C" int qsx_hv_ctx_create(qsx_core* c, qsx_oplist* l, const void* basis, const void* nrm, int n_bytes,
                                  int64_t Ns, const int64_t* begin, const int64_t* end, int N_p, int dtype,
@@ -521,16 +579,16 @@ extern "C" int qsx_hv_ctx_create(qsx_core* c, qsx_oplist* l, const void* basis, 
   std::memset(&X->A, 0, sizeof(HvArgs));
   HvArgs& A = X->A;
   A.C = c->d; A.ph = c->phase; A.R = c->rank_d;
-  A.off = transposed ? L->base.off : L->off_hc;
-  A.n_off = L->base.n_off;
-  A.off_conj = (transposed == conjugated) ? 1 : 0;
   A.dg = L->base.diag; A.n_diag = L->base.n_diag; A.diag_conj = diag_conj ? 1 : 0;
   A.diag_arr = diag_arr; A.diag_arr_cplx = diag_arr_cplx;
   A.basis = basis; A.nrm = nrm; A.n_bytes = n_bytes; A.begin = begin; A.end = end; A.N_p = N_p;
   A.Ns = Ns; A.nvec = nvec;
   A.alpha_off = make_double2(1, 0); A.alpha_diag = make_double2(1, 0);
-  X->dtype = dtype; X->N = c->h.N; X->nmaps = c->h.nmaps; X->fm = c->h.fermion != 0;
-  X->lookup = pick_lookup(c, Ns);
+  X->c = c;
+  X->L = L;
+  X->dtype = dtype;
+  X->transposed = transposed ? 1 : 0;
+  X->conjugated = conjugated ? 1 : 0;
   *out = X;
   return QSX_OK;
 }
@@ -549,7 +607,7 @@ extern "C" int qsx_hv_ctx_apply(qsx_hv_ctx* X, const void* x, void* y, double a_
   A.alpha_off = make_double2(a_off_re, a_off_im);
   A.alpha_diag = make_double2(a_diag_re, a_diag_im);
   A.beta = beta ? 1 : 0;
-  return dispatch_hv(A, X->dtype, X->N, X->nmaps, X->fm, X->lookup, (hipStream_t)stream);
+  return hv_launch(X->c, X->L, A, X->transposed, X->conjugated, X->dtype, (hipStream_t)stream);
 }
 
 int qsx_hv_expm_term(void* ctx, const void* x, void* y, double2 coef, double2 shift, const int* skip,
@@ -558,5 +616,5 @@ int qsx_hv_expm_term(void* ctx, const void* x, void* y, double2 coef, double2 sh
   HvArgs A = X->A;
   A.v_in = x; A.v_out = y;
   A.alpha_off = coef; A.alpha_diag = coef; A.shift = shift; A.skip = skip; A.beta = 0;
-  return dispatch_hv(A, X->dtype, X->N, X->nmaps, X->fm, X->lookup, st);
+  return hv_launch(X->c, X->L, A, X->transposed, X->conjugated, X->dtype, st);
 }

@@ -16,6 +16,8 @@ struct qsx_core {
   RankDev rank_h;
   RankDev* rank_d = nullptr;
   int rank_ok = 0;           // 1 when the basis is a full single sector
+  int sector_ok = 0;         // 1 when make_basis enumerated one sector (rank_h describes it)
+  int64_t sector_total = 0;  // states in that sector (before symmetry reduction)
 };
 
 struct qsx_oplist {

@@ -13,6 +13,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from .. import _lib as L
+from .. import tuning
 from .._lib import ptr, stream_handle
 
 
@@ -142,19 +143,30 @@ class MatrixFreeOp:
     def Ns(self):
         return self.basis.Ns
 
+    def _new_ctx(self, vdt, nvec):
+        h = L._vp()
+        dcplx = 0
+        if self.diag is not None:
+            dcplx = 1 if L.np_dtype(self.diag.dtype).kind == "c" else 0
+        rc = L.load().qsx_hv_ctx_create(self.basis._core.h, self.oplist.h, *self.basis._dev_args(),
+                                        L.dtype_code(vdt), nvec, int(self.transposed), int(self.conjugated),
+                                        int(self.diag_conj), ptr(self.diag), dcplx, C.byref(h))
+        L.check(rc, "qsx_hv_ctx_create")
+        return _Ctx(h)
+
     def _context(self, vdt, nvec):
         key = (vdt, nvec)
         if key not in self._ctx:
-            h = L._vp()
-            dcplx = 0
-            if self.diag is not None:
-                dcplx = 1 if L.np_dtype(self.diag.dtype).kind == "c" else 0
-            rc = L.load().qsx_hv_ctx_create(self.basis._core.h, self.oplist.h, *self.basis._dev_args(),
-                                            L.dtype_code(vdt), nvec, int(self.transposed), int(self.conjugated),
-                                            int(self.diag_conj), ptr(self.diag), dcplx, C.byref(h))
-            L.check(rc, "qsx_hv_ctx_create")
-            self._ctx[key] = _Ctx(h)
+            ctx = self._new_ctx(vdt, nvec)
+            cfg = tuning.choose(self, ctx, vdt, nvec)
+            ctx.configure(cfg, self.basis)
+            self._ctx[key] = ctx
         return self._ctx[key]
+
+    @property
+    def config(self):
+        """Chosen (lookup, schedule) per (dtype, nvec)."""
+        return {k: v.cfg for k, v in self._ctx.items()}
 
     def apply(self, x, y, a=1.0, beta=0, a_diag=None):
         vdt = L.np_dtype(x.dtype)
@@ -201,6 +213,23 @@ class MatrixFreeOp:
 class _Ctx:
     def __init__(self, h):
         self.h = h
+        self.cfg = {"lookup": "search", "schedule": "rows"}
+
+    def configure(self, cfg, basis):
+        lib = L.load()
+        self.cfg = dict(cfg)
+        if cfg.get("lookup") == "table":
+            tab = basis.lookup_table()
+            if tab is None:
+                self.cfg["lookup"] = "search"
+                L.check(lib.qsx_hv_ctx_set_table(self.h, None, 0, 0), "qsx_hv_ctx_set_table")
+            else:
+                t, eb, ib = tab
+                L.check(lib.qsx_hv_ctx_set_table(self.h, ptr(t), eb, ib), "qsx_hv_ctx_set_table")
+        else:
+            L.check(lib.qsx_hv_ctx_set_table(self.h, None, 0, 0), "qsx_hv_ctx_set_table")
+        L.check(lib.qsx_hv_ctx_set_schedule(self.h, 1 if cfg.get("schedule") == "queue" else 0),
+                "qsx_hv_ctx_set_schedule")
 
     def __del__(self):
         try:
@@ -211,7 +240,15 @@ class _Ctx:
 
 
 def build_csr(basis, oplist, dtype, transposed=False, conjugated=False, a=1.0, diag=None):
-    """Assemble a*Op as a canonical device CSR (``qsx_csr_builder_*``)."""
+    """Assemble a*Op as a canonical device CSR (``qsx_csr_builder_*``).
+    Lists that do not commute with the symmetry group are assembled through
+    their transpose (rows gathered with the original terms)."""
+    if len(basis._pers) and not oplist.symmetric and not transposed:
+        MT = build_csr(basis, oplist, dtype, True, conjugated, a, None)
+        M = DeviceCSR.from_scipy(MT.to_scipy().T.tocsr(), dtype)
+        if diag is not None:
+            M = DeviceCSR.from_scipy(M.to_scipy() + sp.diags(L.to_host(diag).astype(dtype), format="csr"), dtype)
+        return M
     lib = L.load()
     dtype = np.dtype(dtype)
     h = L._vp()

@@ -1,0 +1,123 @@
+"""Launch-configuration autotuner for the fused H.v kernel.
+
+For a matrix-free operator the tuner chooses the representative lookup
+("search": orbit search + prefix-block binary search; "table": sector lookup
+table in HBM) and the schedule ("rows": one lane per row; "queue": compacted
+LDS work queue, one vector only).  Candidates are timed on the operator's own
+basis with HIP events (a few launches each) and the winner is cached per
+(basis signature, dtype, nvec) in memory and in a JSON file
+(``QSX_TUNE_CACHE``, default ``~/.cache/quspin_amd/tune.json``).
+``QSX_TUNE=off`` disables timing and uses the static rule below;
+``QSX_LOOKUP``/``QSX_SCHEDULE`` force a choice.
+"""
+
+import json
+import os
+
+import numpy as np
+
+from . import _lib as L
+
+_mem = {}
+
+
+def _cache_path():
+    return os.environ.get("QSX_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "quspin_amd",
+                                                         "tune.json"))
+
+
+def _load_file():
+    try:
+        with open(_cache_path()) as f:
+            return json.load(f)
+    except Exception:
+        return {}
+
+
+def _save_file(d):
+    try:
+        p = _cache_path()
+        os.makedirs(os.path.dirname(p), exist_ok=True)
+        with open(p, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+    except Exception:
+        pass
+
+
+def signature(op, vdt, nvec):
+    b = op.basis
+    return "N{}_Ns{}_G{}_maps{}_q{}_off{}_diag{}_{}_nv{}_T{}C{}".format(
+        b.N, b.Ns, int(np.prod(b._pers)) if len(b._pers) else 1, len(b._pers), tuple(int(q) for q in b._qs),
+        op.oplist.n_off, op.oplist.n_diag, np.dtype(vdt).name, nvec, int(op.transposed), int(op.conjugated))
+
+
+def static_rule(op, vdt, nvec):
+    b = op.basis
+    G = int(np.prod(b._pers)) if len(b._pers) else 1
+    lookup = "table" if G > 1 else "search"
+    schedule = "queue" if (nvec == 1 and op.oplist.n_off >= 4) else "rows"
+    return {"lookup": lookup, "schedule": schedule}
+
+
+def candidates(op, vdt, nvec):
+    b = op.basis
+    lookups = ["search"]
+    if len(b._pers):
+        lookups.append("table")
+    scheds = ["rows", "queue"] if nvec == 1 else ["rows"]
+    return [{"lookup": lk, "schedule": sc} for lk in lookups for sc in scheds]
+
+
+def choose(op, ctx, vdt, nvec):
+    forced_l, forced_s = os.environ.get("QSX_LOOKUP"), os.environ.get("QSX_SCHEDULE")
+    if forced_l or forced_s:
+        cfg = static_rule(op, vdt, nvec)
+        if forced_l:
+            cfg["lookup"] = forced_l
+        if forced_s:
+            cfg["schedule"] = forced_s
+        return cfg
+    if os.environ.get("QSX_TUNE", "on") == "off" or op.basis.Ns < 4096:
+        return static_rule(op, vdt, nvec)
+    key = signature(op, vdt, nvec)
+    if key in _mem:
+        return _mem[key]
+    disk = _load_file()
+    if key in disk:
+        _mem[key] = disk[key]
+        return disk[key]
+    best = time_candidates(op, ctx, vdt, nvec)
+    _mem[key] = best
+    disk[key] = best
+    _save_file(disk)
+    return best
+
+
+def time_candidates(op, ctx, vdt, nvec, reps=5):
+    t = L.torch()
+    shape = (op.Ns,) if nvec == 1 else (op.Ns, nvec)
+    x = t.ones(shape, dtype=L.torch_dtype(vdt), device="cuda")
+    y = t.empty_like(x)
+    lib = L.load()
+    best, best_ms = None, float("inf")
+    results = []
+    for cfg in candidates(op, vdt, nvec):
+        ctx.configure(cfg, op.basis)
+        if ctx.cfg["lookup"] != cfg["lookup"]:
+            continue
+        for _ in range(2):
+            lib.qsx_hv_ctx_apply(ctx.h, L.ptr(x), L.ptr(y), 1.0, 0.0, 1.0, 0.0, 0, L.stream_handle())
+        st = t.cuda.current_stream()
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            lib.qsx_hv_ctx_apply(ctx.h, L.ptr(x), L.ptr(y), 1.0, 0.0, 1.0, 0.0, 0, L.stream_handle())
+        e1.record(st)
+        t.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        results.append((cfg, ms))
+        if ms < best_ms:
+            best, best_ms = dict(cfg), ms
+    best["ms"] = round(best_ms, 5)
+    best["timings"] = [[c["lookup"], c["schedule"], round(ms, 5)] for c, ms in results]
+    return best

@@ -247,3 +247,36 @@ def test_large_C2_hermiticity_and_linearity():
     H = hamiltonian(static, [], basis=B, dtype=np.float64, check_symm=False, check_herm=False, check_pcon=False)
     Hv2 = H.dot(v)
     assert torch.max(torch.abs(Hv2 - Hv)).item() < 1e-11
+
+
+@pytest.mark.parametrize("lookup", ["search", "table"])
+@pytest.mark.parametrize("schedule", ["rows", "queue"])
+def test_forced_configs_vs_oracle(lookup, schedule, monkeypatch):
+    """Every (lookup, schedule) pair of the fused kernel against the oracle:
+    complex k-sector, real k=0/p/z sector, fermion k-sector."""
+    from quspin_amd.basis import spin_basis_general, spinful_fermion_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    monkeypatch.setenv("QSX_LOOKUP", lookup)
+    monkeypatch.setenv("QSX_SCHEDULE", schedule)
+    print(f"case lookup={lookup} schedule={schedule}", flush=True)
+    L = 14
+    m = chain_maps(L)
+    cases = [(spin_basis_general(L, Nup=7, pauli=0, kblock=(m["T"], 3)), xxz_static(L, Delta=0.3), np.complex128,
+              False, None),
+             (spin_basis_general(L, Nup=7, pauli=0, kblock=(m["T"], 0), pblock=(m["P"], 1), zblock=(m["Z"], 0)),
+              xxz_static(L, Delta=1.0), np.float64, False, None)]
+    Lf = 6
+    T = (np.arange(Lf) + 1) % Lf
+    cases.append((spinful_fermion_basis_general(Lf, Nf=(3, 2), kblock=(T, 1)), hubbard_static(Lf, U=2.0),
+                  np.complex128, True, Lf))
+    for B, static, dt, ferm, sl in cases:
+        Q = quantum_LinearOperator(static, basis=B, dtype=dt, check_symm=False, check_herm=False, check_pcon=False)
+        H = _oracle_H(B, static, fermion=ferm, spinful_L=sl)
+        for nvec in (None, 3):
+            v = _vec(B.Ns, dt, seed=5, nvec=nvec)
+            got = Q.dot(v)
+            ref = H @ v.astype(np.complex128)
+            np.testing.assert_allclose(got, ref if np.dtype(dt).kind == "c" else ref.real, atol=1e-11, rtol=0)
+        cfg = Q._qsx_operator().config
+        if len(B._pers):
+            assert all(c["lookup"] == lookup for c in cfg.values()), cfg

@@ -137,7 +137,10 @@ def test_lanczos_ground_state_and_expm():
     E, V, Q = lanczos_full(H, v0, 50, full_ortho=False)
     assert abs(E[-1] - E_GS) < 1e-10
     Eo, Vo, Qo, ao, bo = O.lanczos_full(lambda x: H.tocsr() @ x, v0, 50)
-    np.testing.assert_allclose(E, Eo, atol=1e-9)
+    # extreme (converged) Ritz values agree; interior ones depend on rounding
+    # once orthogonality is lost, so only the first recurrence steps are compared
+    assert abs(E[0] - Eo[0]) < 1e-9 and abs(E[-1] - Eo[-1]) < 1e-9
+    np.testing.assert_allclose(Q[:8], Qo[:8], atol=1e-10)
     v1 = lin_comb_Q_T(V[:, -1], Q)
     E, V, Q_iter = lanczos_iter(H, v0, 50, return_vec_iter=True)
     v2 = lin_comb_Q_T(V[:, -1], Q_iter)
@@ -159,8 +162,9 @@ def test_lanczos_device_C2_ground_energy():
     rng = np.random.default_rng(0)
     v0 = rng.normal(size=B.Ns)
     E, V, Qm = lanczos_full(Q, torch.from_numpy(v0).cuda(), 100)
-    Hh = Q.T  # Hermitian: same operator
     Eo, _, _, _, _ = O.lanczos_full(lambda x: Q.dot(x), v0 / np.linalg.norm(v0), 100)
     assert abs(E[0] - Eo[0]) < 1e-9
-    # closed-form ground energy of the L=20 Heisenberg ring (known value)
-    assert abs(E[0] / L - (-0.4438)) < 1e-3
+    # exact ground energy of the sector by dense diagonalisation of the oracle matrix
+    from tests.test_gpu_hv import _oracle_H
+    E_exact = np.linalg.eigvalsh(_oracle_H(B, xxz_static(L)).toarray())[0]
+    assert abs(E[0] - E_exact) < 1e-9
