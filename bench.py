@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--lanczos", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="only run the plain copy kernel on this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU kernel "
@@ -54,6 +56,16 @@ def main():
     from quspin_amd.tools.lanczos import lanczos_full
 
     assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    if args.copy_mb:
+        from quspin_amd import _lib
+        nb = args.copy_mb << 20
+        src = torch.ones(nb // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(args.warmup + args.steps):
+            _lib.check(_lib.load().qsx_copy(_lib.ptr(src), _lib.ptr(dst), nb, _lib.stream_handle()), "qsx_copy")
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": nb, "launches": args.warmup + args.steps}))
+        return
     L = args.L
     i = np.arange(L)
     T, P, Z = (i + 1) % L, i[::-1].copy(), -(i + 1)

@@ -78,6 +78,8 @@ SIGNATURES = {
     "qsx_lanczos_update": (_i32, [_i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "qsx_lanczos_div": (_i32, [_i32, _vp, _vp, _vp, _i64, _vp]),
     "qsx_lincomb": (_i32, [_i32, _vp, _i64, _i64, _vp, _vp, _vp]),
+    "qsx_copy": (_i32, [_vp, _vp, _i64, _vp]),
+    "qsx_sector_rank": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp]),
 }
 
 _lib = None
@@ -96,6 +98,10 @@ def load():
         raise ImportError(
             f"{LIB_PATH} not found: build it with quspin_amd._build.build() "
             "(there is no CPU fallback)")
+    # torch ships its own libamdhip64 with SONAME libamdhip64.so.7; loading it
+    # first makes this library bind to the same HIP runtime (one runtime per
+    # process, so torch's streams and allocations are valid here)
+    torch()
     lib = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         f = getattr(lib, name)

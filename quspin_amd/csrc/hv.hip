@@ -56,6 +56,36 @@ int compile_term(const qsx_core* c, const char* opstr, const int32_t* indx, int 
     if (is_flip(code)) T.flip ^= 1ull << (N - 1 - site);
     if (code == OP_Y) ++n_y;
   }
+  if (T.flip) {
+    T.lo = (int16_t)__builtin_ctzll(T.flip);
+    T.hi = (int16_t)(63 - __builtin_clzll(T.flip));
+  }
+  // bit-parallel form (spin / hard-core bosons, every site at most once)
+  if (!c->h.fermion) {
+    uint64_t seen = 0;
+    bool distinct = true;
+    for (int k = 0; k < n_op; ++k) {
+      const uint64_t b = 1ull << T.pos[k];
+      if (seen & b) distinct = false;
+      seen |= b;
+    }
+    if (distinct) {
+      T.fast = 1;
+      T.mag0 = 1.0;
+      for (int k = 0; k < n_op; ++k) {
+        const uint64_t b = 1ull << T.pos[k];
+        switch (T.code[k]) {
+          case OP_Z: T.zmask |= b; T.nz += 1; T.mag0 *= 0.5; break;
+          case OP_N: T.req_mask |= b; T.req_val |= b; break;
+          case OP_P: T.req_mask |= b; break;
+          case OP_M: T.req_mask |= b; T.req_val |= b; break;
+          case OP_X: T.mag0 *= 0.5; break;
+          case OP_Y: T.ymask |= b; T.ny += 1; T.mag0 *= 0.5; break;
+          default: break;
+        }
+      }
+    }
+  }
   return QSX_OK;
 }
 
@@ -282,6 +312,9 @@ extern "C" int qsx_inplace_op(qsx_core* c, const void* v_in, void* v_out, int dt
 template <typename I, int M, bool FERM, int LK>
 __global__ void __launch_bounds__(256) k_op_coo(HvArgs A, const TermDev* __restrict__ T, void* row, void* col,
                                                 int idx_bytes, double2* __restrict__ me) {
+  __shared__ int64_t sb[(LK == LOOKUP_TABLE || LK == LOOKUP_RANK) ? QSX_SB : 1];
+  stage_binom<LK>(A.R, sb);
+  __syncthreads();
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= A.Ns) return;
   const I s = ((const I*)A.basis)[i];
@@ -289,7 +322,8 @@ __global__ void __launch_bounds__(256) k_op_coo(HvArgs A, const TermDev* __restr
   if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, i));
   int64_t rr = i;
   double2 v = make_double2(0.0, 0.0);
-  term_element<I, M, FERM, LK, true>(A, T, s, rsq, 0, [&](int64_t c, double2 coef) { rr = c; v = coef; });
+  const RowRank rk = row_rank_if<I, LK>(A, sb, s);
+  term_element<I, M, FERM, LK, true>(A, sb, rk, T, s, rsq, 0, [&](int64_t c, double2 coef) { rr = c; v = coef; });
   if (idx_bytes == 4) { ((int32_t*)row)[i] = (int32_t)rr; ((int32_t*)col)[i] = (int32_t)i; }
   else { ((int64_t*)row)[i] = rr; ((int64_t*)col)[i] = i; }
   me[i] = v;
@@ -344,12 +378,16 @@ extern "C" int qsx_op(qsx_core* c, const char* opstr, const int32_t* indx, int n
 template <typename I, int M, bool FERM, int LK, int PASS>
 __global__ void __launch_bounds__(256) k_csr_rows(HvArgs A, int64_t* __restrict__ cnt, const int64_t* __restrict__ offs,
                                                   int64_t* __restrict__ rc, double2* __restrict__ rv) {
+  __shared__ int64_t sb[(LK == LOOKUP_TABLE || LK == LOOKUP_RANK) ? QSX_SB : 1];
+  stage_binom<LK>(A.R, sb);
+  __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= A.Ns) return;
   const I s = ((const I*)A.basis)[r];
   double rsq = 1.0;
   if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, r));
   const double2 d = row_diag<I, M, FERM, LK, true>(A, r, s);
+  const RowRank rr = row_rank_if<I, LK>(A, sb, s);
   int64_t k = 0;
   const int64_t o = (PASS == 1) ? offs[r] : 0;
   if (d.x != 0.0 || d.y != 0.0) {
@@ -357,7 +395,7 @@ __global__ void __launch_bounds__(256) k_csr_rows(HvArgs A, int64_t* __restrict_
     ++k;
   }
   for (int t = 0; t < A.n_off; ++t) {
-    term_element<I, M, FERM, LK, true>(A, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
+    term_element<I, M, FERM, LK, true>(A, sb, rr, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
       coef = cmul(coef, A.alpha_off);
       if constexpr (PASS == 1) { rc[o + k] = c; rv[o + k] = coef; }
       ++k;
@@ -557,9 +595,11 @@ extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_
   return QSX_OK;
 }
 
-// 0: one lane per row; 1: compacted work queue (single vector)
+// 0: one lane per row; 1: compacted work queue; 2/3/4: 2/4/8 lanes per row;
+// 5/6: batches of 4/8 terms with rank-based lookups (table or no symmetry).
+// Single vector; several vectors always use one lane per row.
 extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
-  if (!X || sched < 0 || sched > 1) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
+  if (!X || sched < 0 || sched > 6) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
   X->A.sched = sched;
   return QSX_OK;
 }
@@ -617,4 +657,37 @@ int qsx_hv_expm_term(void* ctx, const void* x, void* y, double2 coef, double2 sh
   A.v_in = x; A.v_out = y;
   A.alpha_off = coef; A.alpha_diag = coef; A.shift = shift; A.skip = skip; A.beta = 0;
   return hv_launch(X->c, X->L, A, X->transposed, X->conjugated, X->dtype, st);
+}
+
+// ---------------------------------------------------------------------------
+// sector rank / basis index of arbitrary states (LOOKUP_RANK path), used by
+// tests and by callers that index states without a search
+// ---------------------------------------------------------------------------
+template <typename I>
+__global__ void k_sector_rank(const RankDev* __restrict__ R, const I* __restrict__ states, int64_t n,
+                              int64_t* __restrict__ out) {
+  __shared__ int64_t sb[QSX_SB];
+  stage_binom<LOOKUP_RANK>(R, sb);
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[i] = sector_rank<I>(R, sb, states[i]);
+}
+
+// out[i] = ascending rank of states[i] in the particle sector of the last
+// make_basis (-1 outside it); info (8 int64) = {mode, Np, Nup, Ndn, Ns, sector_ok, rank_ok, N}.
+extern "C" int qsx_sector_rank(qsx_core* c, const void* states, int64_t n, int64_t* out, int64_t* info,
+                               void* stream) {
+  if (!c) { qsx_set_error("qsx_sector_rank: null core"); return QSX_ERR_ARG; }
+  if (info) {
+    info[0] = c->rank_h.mode; info[1] = c->rank_h.Np; info[2] = c->rank_h.Nup; info[3] = c->rank_h.Ndn;
+    info[4] = c->rank_h.Ns; info[5] = c->sector_ok; info[6] = c->rank_ok; info[7] = c->h.N;
+  }
+  if (n <= 0 || !states) return QSX_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const dim3 grid((unsigned)qsx_grid(n, 256)), blk(256);
+  if (c->h.N <= 32) hipLaunchKernelGGL(k_sector_rank<uint32_t>, grid, blk, 0, st, c->rank_d, (const uint32_t*)states, n, out);
+  else hipLaunchKernelGGL(k_sector_rank<uint64_t>, grid, blk, 0, st, c->rank_d, (const uint64_t*)states, n, out);
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
 }

@@ -43,6 +43,32 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
+// sqrt of a small non-negative integer (stabiliser sizes): exact table for
+// n < 64, hardware sqrt above.
+__constant__ double c_sqrt_tab[64] = {
+    0.0, 1.0, 1.4142135623730951, 1.7320508075688772, 2.0, 2.23606797749979, 2.449489742783178,
+    2.6457513110645907, 2.8284271247461903, 3.0, 3.1622776601683795, 3.3166247903554, 3.4641016151377544,
+    3.605551275463989, 3.7416573867739413, 3.872983346207417, 4.0, 4.123105625617661, 4.242640687119285,
+    4.358898943540674, 4.47213595499958, 4.58257569495584, 4.69041575982343, 4.795831523312719,
+    4.898979485566356, 5.0, 5.0990195135927845, 5.196152422706632, 5.291502622129181, 5.385164807134504,
+    5.477225575051661, 5.5677643628300215, 5.656854249492381, 5.744562646538029, 5.830951894845301,
+    5.916079783099616, 6.0, 6.082762530298219, 6.164414002968976, 6.244997998398398, 6.324555320336759,
+    6.4031242374328485, 6.48074069840786, 6.557438524302, 6.6332495807108, 6.708203932499369,
+    6.782329983125268, 6.855654600401044, 6.928203230275509, 7.0, 7.0710678118654755, 7.14142842854285,
+    7.211102550927978, 7.280109889280518, 7.3484692283495345, 7.416198487095663, 7.483314773547883,
+    7.54983443527075, 7.615773105863909, 7.681145747868608, 7.745966692414834, 7.810249675906654,
+    7.874007874011811, 7.937253933193772};
+
+__device__ __forceinline__ double sqrt_small(int n) { return n < 64 ? c_sqrt_tab[n] : sqrt((double)n); }
+
+// sum over G consecutive lanes (G divides the wavefront size)
+template <int G>
+__device__ __forceinline__ double grp_sum(double v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
+  return v;
+}
+
 // i^p
 __device__ __forceinline__ double2 ipow_c(int p) {
   p &= 3;
@@ -86,10 +112,14 @@ __device__ __forceinline__ double load_n(const void* p, int nb, int64_t i) {
                                                               : (double)((const uint32_t*)p)[i];
 }
 
+template <typename I>
+__device__ __forceinline__ int64_t sector_rank(const RankDev* __restrict__ R, const int64_t* sb, I s);
+
 template <typename I, int LK>
-__device__ __forceinline__ int64_t lookup_index(const HvArgs& A, I rep) {
+__device__ __forceinline__ int64_t lookup_index(const HvArgs& A, const int64_t* sb, I rep) {
   if constexpr (LK == LOOKUP_RANK) {
-    return rank_index<I>(A.R, rep);
+    const int64_t rk = sector_rank<I>(A.R, sb, rep);
+    return rk < 0 ? -1 : A.R->Ns - 1 - rk;
   } else {
     int64_t lo = 0, hi = A.Ns;
     if (A.N_p > 0) {
@@ -101,43 +131,143 @@ __device__ __forceinline__ int64_t lookup_index(const HvArgs& A, I rep) {
   }
 }
 
-// Ascending rank of s inside the particle sector described by R, -1 outside.
-template <typename I>
-__device__ __forceinline__ int64_t sector_rank(const RankDev* __restrict__ R, I s) {
-  if (R->mode == 0) {
-    if (popc<I>(s) != R->Np) return -1;
-    return comb_rank<I>(R, s);
-  } else if (R->mode == 1) {
-    const I mlo = (((I)1) << R->L) - 1;
-    const I up = s >> R->L, dn = s & mlo;
-    if (popc<I>(up) != R->Nup || popc<I>(dn) != R->Ndn) return -1;
-    return comb_rank<I>(R, up) * R->cdn + comb_rank<I>(R, dn);
+// Binomial table C(n, k), n <= 64, k <= 32, staged in LDS by the kernels
+// that rank states (rows of 33 entries).
+constexpr int QSX_SB = 65 * 33;
+
+template <int LK>
+__device__ __forceinline__ void stage_binom(const RankDev* __restrict__ R, int64_t* sb) {
+  if constexpr (LK == LOOKUP_TABLE || LK == LOOKUP_RANK) {
+    for (int i = threadIdx.x; i < QSX_SB; i += blockDim.x) sb[i] = R->binom[i / 33][i % 33];
   }
-  return (int64_t)s;
+}
+
+template <typename I>
+__device__ __forceinline__ int64_t comb_rank_sb(const int64_t* sb, I s) {
+  int64_t r = 0;
+  int j = 1;
+  while (s) {
+    const int p = (sizeof(I) == 8) ? __builtin_ctzll((uint64_t)s) : __builtin_ctz((uint32_t)s);
+    s &= s - 1;
+    r += sb[p * 33 + j];
+    ++j;
+  }
+  return r;
+}
+
+// Ascending rank of s inside the particle sector described by R, -1 outside.
+// Single exit on purpose: an earlier multi-return form of this function was
+// compiled for gfx950 with the mode-2 result never written (the value of the
+// trailing return was lost when the uniform branches were structurised).
+template <typename I>
+__device__ __forceinline__ int64_t sector_rank(const RankDev* __restrict__ R, const int64_t* sb, I s) {
+  const int mode = R->mode;
+  int64_t res = (int64_t)s;
+  if (mode == 0) {
+    res = (popc<I>(s) == R->Np) ? comb_rank_sb<I>(sb, s) : (int64_t)-1;
+  } else if (mode == 1) {
+    const int L = R->L;
+    const I mlo = (I)((((uint64_t)1) << L) - 1);
+    const I up = s >> L, dn = s & mlo;
+    const bool ok = (popc<I>(up) == R->Nup) && (popc<I>(dn) == R->Ndn);
+    res = ok ? comb_rank_sb<I>(sb, up) * R->cdn + comb_rank_sb<I>(sb, dn) : (int64_t)-1;
+  }
+  return res;
+}
+
+// Sum of C(p, j) over the set bits p of x inside [lo, hi], j = number of set
+// bits of x at positions <= p.
+template <typename I>
+__device__ __forceinline__ int64_t window_rank(const int64_t* sb, I x, int lo, int hi) {
+  constexpr int W = 8 * (int)sizeof(I);
+  I w = (x >> lo) << lo;
+  if (hi < W - 1) w &= (((I)2) << hi) - (I)1;
+  int j = (lo > 0) ? popc<I>(x & ((((I)1) << lo) - (I)1)) : 0;
+  int64_t r = 0;
+  while (w) {
+    const int p = (sizeof(I) == 8) ? __builtin_ctzll((uint64_t)w) : __builtin_ctz((uint32_t)w);
+    w &= w - 1;
+    ++j;
+    r += sb[p * 33 + j];
+  }
+  return r;
+}
+
+// Rank data of a row state: full rank (mode 0), or up / down ranks (mode 1).
+struct RowRank {
+  int64_t a, b;
+};
+
+template <typename I>
+__device__ __forceinline__ RowRank row_rank(const RankDev* __restrict__ R, const int64_t* sb, I s) {
+  RowRank rr{0, 0};
+  if (R->mode == 0) {
+    rr.a = comb_rank_sb<I>(sb, s);
+  } else if (R->mode == 1) {
+    const int L = R->L;
+    const I mlo = (I)((((uint64_t)1) << L) - 1);
+    rr.a = comb_rank_sb<I>(sb, s >> L);
+    rr.b = comb_rank_sb<I>(sb, s & mlo);
+  }
+  return rr;
+}
+
+// Rank of s2 = s with the bits of [lo, hi] changed, from the rank of s.
+// Single exit (see sector_rank).
+template <typename I>
+__device__ __forceinline__ int64_t sector_rank_inc(const RankDev* __restrict__ R, const int64_t* sb, I s, RowRank rr,
+                                                   I s2, int lo, int hi) {
+  const int mode = R->mode;
+  int64_t res = (int64_t)s2;
+  if (mode == 0) {
+    const bool ok = popc<I>(s2) == R->Np;
+    res = ok ? rr.a - window_rank<I>(sb, s, lo, hi) + window_rank<I>(sb, s2, lo, hi) : (int64_t)-1;
+  } else if (mode == 1) {
+    const int L = R->L;
+    const I mlo = (I)((((uint64_t)1) << L) - 1);
+    const I up = s2 >> L, dn = s2 & mlo;
+    const bool ok = (popc<I>(up) == R->Nup) && (popc<I>(dn) == R->Ndn);
+    int64_t ra = rr.a, rb = rr.b;
+    if (hi < L) {
+      rb = rr.b - window_rank<I>(sb, s & mlo, lo, hi) + window_rank<I>(sb, dn, lo, hi);
+    } else if (lo >= L) {
+      ra = rr.a - window_rank<I>(sb, s >> L, lo - L, hi - L) + window_rank<I>(sb, up, lo - L, hi - L);
+    } else {
+      ra = comb_rank_sb<I>(sb, up);
+      rb = comb_rank_sb<I>(sb, dn);
+    }
+    res = ok ? ra * R->cdn + rb : (int64_t)-1;
+  }
+  return res;
 }
 
 // Representative of s2, its basis index, element code, stabiliser size and
 // fermion sign.  Returns false when the representative is not in the basis.
+template <typename I>
+__device__ __forceinline__ bool table_entry(const HvArgs& A, int64_t rk, int64_t& c, int& code, int& cnt, int& sign) {
+  if (rk < 0) return false;
+  if (A.table_u64) {
+    const uint64_t e = ((const uint64_t*)A.table)[rk];
+    if (e == ~0ull) return false;
+    c = (int64_t)(e & ((1ull << 40) - 1));
+    code = (int)((e >> 40) & 0x7FFFFF);
+    sign = (e >> 63) ? -1 : 1;
+  } else {
+    const uint32_t e = ((const uint32_t*)A.table)[rk];
+    if (e == 0xFFFFFFFFu) return false;
+    c = (int64_t)(e & ((1u << A.idx_bits) - 1u));
+    code = (int)(e >> A.idx_bits);
+  }
+  cnt = (int)load_n(A.nrm, A.n_bytes, c);
+  return true;
+}
+
 template <typename I, int M, bool FERM, int LK>
-__device__ __forceinline__ bool resolve(const HvArgs& A, I s2, int64_t& c, int& code, int& cnt, int& sign) {
+__device__ __forceinline__ bool resolve(const HvArgs& A, const int64_t* sb, I s2, int64_t& c, int& code, int& cnt,
+                                        int& sign) {
   code = 0; cnt = 1; sign = 1;
   if constexpr (LK == LOOKUP_TABLE) {
-    const int64_t rk = sector_rank<I>(A.R, s2);
-    if (rk < 0) return false;
-    if (A.table_u64) {
-      const uint64_t e = ((const uint64_t*)A.table)[rk];
-      if (e == ~0ull) return false;
-      c = (int64_t)(e & ((1ull << 40) - 1));
-      code = (int)((e >> 40) & 0x7FFFFF);
-      sign = (e >> 63) ? -1 : 1;
-    } else {
-      const uint32_t e = ((const uint32_t*)A.table)[rk];
-      if (e == 0xFFFFFFFFu) return false;
-      c = (int64_t)(e & ((1u << A.idx_bits) - 1u));
-      code = (int)(e >> A.idx_bits);
-    }
-    cnt = (int)load_n(A.nrm, A.n_bytes, c);
-    return true;
+    return table_entry<I>(A, sector_rank<I>(A.R, sb, s2), c, code, cnt, sign);
   } else {
     I rep = s2;
     if constexpr (M != 0) {
@@ -147,26 +277,51 @@ __device__ __forceinline__ bool resolve(const HvArgs& A, I s2, int64_t& c, int& 
       cnt = R.cnt;
       sign = R.sign;
     }
-    c = lookup_index<I, LK>(A, rep);
+    c = lookup_index<I, LK>(A, sb, rep);
     return c >= 0;
   }
+}
+
+// As resolve, with the rank of s2 obtained incrementally from the row state s
+// (rank data rr) when the lookup works by rank.
+template <typename I, int M, bool FERM, int LK>
+__device__ __forceinline__ bool resolve_row(const HvArgs& A, const int64_t* sb, I s, RowRank rr,
+                                            const TermDev* __restrict__ T, I s2, int64_t& c, int& code, int& cnt,
+                                            int& sign) {
+  if constexpr (LK == LOOKUP_TABLE) {
+    code = 0; cnt = 1; sign = 1;
+    return table_entry<I>(A, sector_rank_inc<I>(A.R, sb, s, rr, s2, T->lo, T->hi), c, code, cnt, sign);
+  } else if constexpr (LK == LOOKUP_RANK) {
+    code = 0; cnt = 1; sign = 1;
+    const int64_t rk = sector_rank_inc<I>(A.R, sb, s, rr, s2, T->lo, T->hi);
+    c = rk < 0 ? (int64_t)-1 : A.R->Ns - 1 - rk;
+    return c >= 0;
+  } else {
+    return resolve<I, M, FERM, LK>(A, sb, s2, c, code, cnt, sign);
+  }
+}
+
+template <typename I, int LK>
+__device__ __forceinline__ RowRank row_rank_if(const HvArgs& A, const int64_t* sb, I s) {
+  if constexpr (LK == LOOKUP_TABLE || LK == LOOKUP_RANK) return row_rank<I>(A.R, sb, s);
+  else return RowRank{0, 0};
 }
 
 // Reduced element of term T applied to state s of a row with normalisation
 // 1/sqrt(n_r) = rsq.  Calls f(c, coef) when non-zero.  CPLX=false assumes the
 // host checked that every factor is real.
 template <typename I, int M, bool FERM, int LK, bool CPLX, typename F>
-__device__ __forceinline__ void term_element(const HvArgs& A, const TermDev* __restrict__ T, I s, double rsq,
-                                             int conj, F&& f) {
+__device__ __forceinline__ void term_element(const HvArgs& A, const int64_t* sb, RowRank rr,
+                                             const TermDev* __restrict__ T, I s, double rsq, int conj, F&& f) {
   I s2 = s;
   double mag;
   int ip;
   if (!apply_term<I>(T, A.C->fermion != 0, s2, mag, ip)) return;
   int64_t c;
   int code, cnt, sign;
-  if (!resolve<I, M, FERM, LK>(A, s2, c, code, cnt, sign)) return;
+  if (!resolve_row<I, M, FERM, LK>(A, sb, s, rr, T, s2, c, code, cnt, sign)) return;
   double w = mag * (double)sign;
-  if constexpr (M != 0) w *= sqrt((double)cnt) * rsq;
+  if constexpr (M != 0) w *= sqrt_small(cnt) * rsq;
   if constexpr (CPLX) {
     double2 coef = cmul(make_double2(T->jr, T->ji), ipow_c(ip));
     coef = make_double2(coef.x * w, coef.y * w);
@@ -204,20 +359,24 @@ __device__ __forceinline__ double2 row_diag(const HvArgs& A, int64_t r, I s) {
 // vectors, accumulated in place in the lane's own output row.
 template <typename I, typename VT, int M, bool FERM, int LK, int NV>
 __global__ void __launch_bounds__(256) k_hv_rows(HvArgs A) {
+  __shared__ int64_t sb[(LK == LOOKUP_TABLE || LK == LOOKUP_RANK) ? QSX_SB : 1];
   if (A.skip && *A.skip) return;
+  stage_binom<LK>(A.R, sb);
+  __syncthreads();
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= A.Ns) return;
   constexpr bool CPLX = VTr<VT>::cplx;
   const I s = ((const I*)A.basis)[r];
   double rsq = 1.0;
   if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, r));
+  const RowRank rr = row_rank_if<I, LK>(A, sb, s);
   const VT* __restrict__ vin = (const VT*)A.v_in;
   VT* __restrict__ vout = (VT*)A.v_out;
   const double2 dsum = row_diag<I, M, FERM, LK, CPLX>(A, r, s);
   if constexpr (NV == 1) {
     double2 acc = make_double2(0.0, 0.0);
     for (int t = 0; t < A.n_off; ++t) {
-      term_element<I, M, FERM, LK, CPLX>(A, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
+      term_element<I, M, FERM, LK, CPLX>(A, sb, rr, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
         const double2 x = VTr<VT>::ld(vin + c);
         if constexpr (CPLX) acc = cadd(acc, cmul(coef, x));
         else acc.x += coef.x * x.x;
@@ -240,7 +399,7 @@ __global__ void __launch_bounds__(256) k_hv_rows(HvArgs A) {
       VTr<VT>::st(orow + j, res);
     }
     for (int t = 0; t < A.n_off; ++t) {
-      term_element<I, M, FERM, LK, CPLX>(A, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
+      term_element<I, M, FERM, LK, CPLX>(A, sb, rr, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
         double2 cf;
         if constexpr (CPLX) cf = cmul(A.alpha_off, coef);
         else cf = make_double2(A.alpha_off.x * coef.x, 0.0);
@@ -257,6 +416,158 @@ __global__ void __launch_bounds__(256) k_hv_rows(HvArgs A) {
   }
 }
 
+// One vector, G lanes per row: lane j of a row handles terms j, j+G, ... and
+// the partial sums are combined with cross-lane shuffles (more waves and
+// shorter per-lane dependency chains than one lane per row).
+template <typename I, typename VT, int M, bool FERM, int LK, int G>
+__global__ void __launch_bounds__(256) k_hv_rows_g(HvArgs A) {
+  __shared__ int64_t sb[(LK == LOOKUP_TABLE || LK == LOOKUP_RANK) ? QSX_SB : 1];
+  if (A.skip && *A.skip) return;
+  stage_binom<LK>(A.R, sb);
+  __syncthreads();
+  constexpr bool CPLX = VTr<VT>::cplx;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = gid / G;
+  const int lane = (int)(threadIdx.x % G);
+  const bool live = r < A.Ns;
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ vout = (VT*)A.v_out;
+  double2 acc = make_double2(0.0, 0.0);
+  I s = 0;
+  if (live) {
+    s = ((const I*)A.basis)[r];
+    double rsq = 1.0;
+    if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, r));
+    const RowRank rr = row_rank_if<I, LK>(A, sb, s);
+    for (int t = lane; t < A.n_off; t += G) {
+      term_element<I, M, FERM, LK, CPLX>(A, sb, rr, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
+        const double2 x = VTr<VT>::ld(vin + c);
+        if constexpr (CPLX) acc = cadd(acc, cmul(coef, x));
+        else acc.x += coef.x * x.x;
+      });
+    }
+  }
+  acc.x = grp_sum<G>(acc.x);
+  if constexpr (CPLX) acc.y = grp_sum<G>(acc.y);
+  if (!live || lane != 0) return;
+  const double2 dsum = row_diag<I, M, FERM, LK, CPLX>(A, r, s);
+  double2 res;
+  if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc), cmul(dsum, VTr<VT>::ld(vin + r)));
+  else res = make_double2(A.alpha_off.x * acc.x + dsum.x * VTr<VT>::ld(vin + r).x, 0.0);
+  if (A.beta) res = cadd(res, VTr<VT>::ld(vout + r));
+  VTr<VT>::st(vout + r, res);
+}
+
+// One vector, rank-based lookups (table or no-symmetry rank), terms processed
+// in batches of KB: all KB images and ranks first, then all KB table loads,
+// then all KB (n, v) loads, so each lane keeps KB independent memory requests
+// in flight instead of one dependent chain per term.
+template <typename I, typename VT, int M, bool FERM, int LK, int KB>
+__global__ void __launch_bounds__(256) k_hv_rows_batch(HvArgs A) {
+  static_assert(LK == LOOKUP_TABLE || LK == LOOKUP_RANK, "rank-based lookups only");
+  __shared__ int64_t sb[QSX_SB];
+  if (A.skip && *A.skip) return;
+  stage_binom<LK>(A.R, sb);
+  __syncthreads();
+  constexpr bool CPLX = VTr<VT>::cplx;
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= A.Ns) return;
+  const I s = ((const I*)A.basis)[r];
+  double rsq = 1.0;
+  if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, r));
+  const RowRank rr = row_rank<I>(A.R, sb, s);
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ vout = (VT*)A.v_out;
+  const bool ferm = A.C->fermion != 0;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int t0 = 0; t0 < A.n_off; t0 += KB) {
+    int64_t rk[KB];
+    double mg[KB];
+    int ipw[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      rk[k] = -1;
+      mg[k] = 0.0;
+      ipw[k] = 0;
+      const int t = t0 + k;
+      if (t < A.n_off) {
+        const TermDev* T = A.off + t;
+        I s2 = s;
+        double mag;
+        int ip;
+        if (apply_term<I>(T, ferm, s2, mag, ip)) {
+          rk[k] = sector_rank_inc<I>(A.R, sb, s, rr, s2, T->lo, T->hi);
+          mg[k] = mag;
+          ipw[k] = ip;
+        }
+      }
+    }
+    int64_t c[KB];
+    int code[KB], sg[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      c[k] = -1;
+      code[k] = 0;
+      sg[k] = 1;
+      if constexpr (LK == LOOKUP_TABLE) {
+        if (rk[k] >= 0) {
+          if (A.table_u64) {
+            const uint64_t e = ((const uint64_t*)A.table)[rk[k]];
+            if (e != ~0ull) {
+              c[k] = (int64_t)(e & ((1ull << 40) - 1));
+              code[k] = (int)((e >> 40) & 0x7FFFFF);
+              sg[k] = (e >> 63) ? -1 : 1;
+            }
+          } else {
+            const uint32_t e = ((const uint32_t*)A.table)[rk[k]];
+            if (e != 0xFFFFFFFFu) {
+              c[k] = (int64_t)(e & ((1u << A.idx_bits) - 1u));
+              code[k] = (int)(e >> A.idx_bits);
+            }
+          }
+        }
+      } else {
+        c[k] = rk[k] < 0 ? (int64_t)-1 : A.R->Ns - 1 - rk[k];
+      }
+    }
+    double2 x[KB];
+    double w[KB];
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      x[k] = make_double2(0.0, 0.0);
+      w[k] = 0.0;
+      if (c[k] >= 0) {
+        x[k] = VTr<VT>::ld(vin + c[k]);
+        double ww = mg[k] * (double)sg[k];
+        if constexpr (M != 0) ww *= sqrt_small((int)load_n(A.nrm, A.n_bytes, c[k])) * rsq;
+        w[k] = ww;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KB; ++k) {
+      if (c[k] < 0) continue;
+      const TermDev* T = A.off + t0 + k;
+      if constexpr (CPLX) {
+        double2 coef = cmul(make_double2(T->jr, T->ji), ipow_c(ipw[k]));
+        coef = make_double2(coef.x * w[k], coef.y * w[k]);
+        if constexpr (M != 0) coef = cmul(coef, A.ph[code[k]]);
+        if (A.off_conj) coef = cconj(coef);
+        acc = cadd(acc, cmul(coef, x[k]));
+      } else {
+        double v = T->jr * w[k] * ((ipw[k] & 2) ? -1.0 : 1.0);
+        if constexpr (M != 0) v *= A.ph[code[k]].x;
+        acc.x += v * x[k].x;
+      }
+    }
+  }
+  const double2 dsum = row_diag<I, M, FERM, LK, CPLX>(A, r, s);
+  double2 res;
+  if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc), cmul(dsum, VTr<VT>::ld(vin + r)));
+  else res = make_double2(A.alpha_off.x * acc.x + dsum.x * VTr<VT>::ld(vin + r).x, 0.0);
+  if (A.beta) res = cadd(res, VTr<VT>::ld(vout + r));
+  VTr<VT>::st(vout + r, res);
+}
+
 // One vector, compacted work queue.  A block owns 256 rows.  Per chunk of
 // QK terms every row appends its non-vanishing (row, term) images s' to an
 // LDS queue; the block then resolves the queue densely (representative
@@ -271,7 +582,9 @@ __global__ void __launch_bounds__(QBLK) k_hv_queue(HvArgs A) {
   __shared__ I q_s[QBLK * QK];
   __shared__ double2 q_v[QBLK * QK];
   __shared__ int q_n;
+  __shared__ int64_t sb[(LK == LOOKUP_TABLE || LK == LOOKUP_RANK) ? QSX_SB : 1];
   if (A.skip && *A.skip) return;
+  stage_binom<LK>(A.R, sb);
   const int tid = threadIdx.x;
   const int64_t r = (int64_t)blockIdx.x * QBLK + tid;
   const bool live = r < A.Ns;
@@ -317,9 +630,9 @@ __global__ void __launch_bounds__(QBLK) k_hv_queue(HvArgs A) {
       int64_t c;
       int code, cnt, sign;
       double2 val = make_double2(0.0, 0.0);
-      if (resolve<I, M, FERM, LK>(A, q_s[e], c, code, cnt, sign)) {
+      if (resolve<I, M, FERM, LK>(A, sb, q_s[e], c, code, cnt, sign)) {
         double w = (double)sign;
-        if constexpr (M != 0) w *= sqrt((double)cnt);
+        if constexpr (M != 0) w *= sqrt_small(cnt);
         const double2 x = VTr<VT>::ld(vin + c);
         if constexpr (CPLX) {
           double2 f = make_double2(w, 0.0);
@@ -374,18 +687,22 @@ template <> __device__ __forceinline__ void atomic_acc<double2>(double2* p, doub
 
 template <typename I, typename VT, int M, bool FERM, int LK>
 __global__ void __launch_bounds__(256) k_hv_scatter(HvArgs A) {
+  __shared__ int64_t sb[(LK == LOOKUP_TABLE || LK == LOOKUP_RANK) ? QSX_SB : 1];
   if (A.skip && *A.skip) return;
+  stage_binom<LK>(A.R, sb);
+  __syncthreads();
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= A.Ns) return;
   constexpr bool CPLX = VTr<VT>::cplx;
   const I s = ((const I*)A.basis)[c];
   double rsq = 1.0;
   if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, c));
+  const RowRank rr = row_rank_if<I, LK>(A, sb, s);
   const VT* __restrict__ vin = (const VT*)A.v_in;
   VT* vout = (VT*)A.v_out;
   const int64_t nv = A.nvec;
   for (int t = 0; t < A.n_off; ++t) {
-    term_element<I, M, FERM, LK, CPLX>(A, A.off + t, s, rsq, A.off_conj, [&](int64_t r, double2 coef) {
+    term_element<I, M, FERM, LK, CPLX>(A, sb, rr, A.off + t, s, rsq, A.off_conj, [&](int64_t r, double2 coef) {
       double2 cf;
       if constexpr (CPLX) cf = cmul(A.alpha_off, coef);
       else cf = make_double2(A.alpha_off.x * coef.x, 0.0);
@@ -410,12 +727,26 @@ __global__ void __launch_bounds__(256) k_hv_scatter(HvArgs A) {
   }
 
 // Dispatch over (maps, fermion sign, lookup, schedule, NV).
+// schedule codes: 0 one lane per row, 1 compacted queue, 2/3/4 = 2/4/8 lanes
+// per row, 5/6 = batches of 4/8 terms (rank-based lookups)
 template <typename I, typename VT, int M, bool FERM, int LK>
 void launch_hv_one(const HvArgs& A, hipStream_t st) {
   const int blk = 256;
   const dim3 grid((unsigned)qsx_grid(A.Ns, blk));
   if (A.nvec == 1 && A.sched == 1 && A.n_off > 0)
     hipLaunchKernelGGL((k_hv_queue<I, VT, M, FERM, LK>), grid, dim3(QBLK), 0, st, A);
+  else if (A.nvec == 1 && A.sched == 2)
+    hipLaunchKernelGGL((k_hv_rows_g<I, VT, M, FERM, LK, 2>), dim3((unsigned)qsx_grid(A.Ns * 2, blk)), dim3(blk), 0, st, A);
+  else if (A.nvec == 1 && A.sched == 3)
+    hipLaunchKernelGGL((k_hv_rows_g<I, VT, M, FERM, LK, 4>), dim3((unsigned)qsx_grid(A.Ns * 4, blk)), dim3(blk), 0, st, A);
+  else if (A.nvec == 1 && A.sched == 4)
+    hipLaunchKernelGGL((k_hv_rows_g<I, VT, M, FERM, LK, 8>), dim3((unsigned)qsx_grid(A.Ns * 8, blk)), dim3(blk), 0, st, A);
+  else if (A.nvec == 1 && (A.sched == 5 || A.sched == 6) && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
+    if constexpr (LK == LOOKUP_TABLE || LK == LOOKUP_RANK) {
+      if (A.sched == 5) hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 4>), grid, dim3(blk), 0, st, A);
+      else hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 8>), grid, dim3(blk), 0, st, A);
+    }
+  }
   else if (A.nvec == 1)
     hipLaunchKernelGGL((k_hv_rows<I, VT, M, FERM, LK, 1>), grid, dim3(blk), 0, st, A);
   else

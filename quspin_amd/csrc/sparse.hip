@@ -638,3 +638,19 @@ extern "C" int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star,
   (void)hipFreeAsync(ctl, st);
   return QSX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// plain device copy (16-byte vectors), the reference point for HBM counters
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+extern "C" int qsx_copy(const void* src, void* dst, int64_t bytes, void* stream) {
+  if (bytes % 16) { qsx_set_error("qsx_copy: size must be a multiple of 16 bytes"); return QSX_ERR_ARG; }
+  const int64_t n = bytes / 16;
+  hipLaunchKernelGGL(k_copy16, dim3(2048), dim3(256), 0, (hipStream_t)stream, (const uint4*)src, (uint4*)dst, n);
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}

@@ -8,12 +8,6 @@ template <> struct MTr<double> { static constexpr bool cplx = false; __device__ 
 template <> struct MTr<float2> { static constexpr bool cplx = true; __device__ static double2 ld(const float2* p) { float2 v = *p; return make_double2((double)v.x, (double)v.y); } };
 template <> struct MTr<double2> { static constexpr bool cplx = true; __device__ static double2 ld(const double2* p) { return *p; } };
 
-template <int G>
-__device__ __forceinline__ double grp_sum(double v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, G);
-  return v;
-}
 
 // Atomic max of a non-negative double through its bit pattern.
 __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {

@@ -228,7 +228,7 @@ class _Ctx:
                 L.check(lib.qsx_hv_ctx_set_table(self.h, ptr(t), eb, ib), "qsx_hv_ctx_set_table")
         else:
             L.check(lib.qsx_hv_ctx_set_table(self.h, None, 0, 0), "qsx_hv_ctx_set_table")
-        L.check(lib.qsx_hv_ctx_set_schedule(self.h, 1 if cfg.get("schedule") == "queue" else 0),
+        L.check(lib.qsx_hv_ctx_set_schedule(self.h, tuning.SCHEDULES.get(cfg.get("schedule"), 0)),
                 "qsx_hv_ctx_set_schedule")
 
     def __del__(self):

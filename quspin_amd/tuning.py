@@ -2,8 +2,10 @@
 
 For a matrix-free operator the tuner chooses the representative lookup
 ("search": orbit search + prefix-block binary search; "table": sector lookup
-table in HBM) and the schedule ("rows": one lane per row; "queue": compacted
-LDS work queue, one vector only).  Candidates are timed on the operator's own
+table in HBM) and the schedule ("rows": one lane per row; "rows2/4/8": that
+many lanes per row with shuffle reduction; "queue": compacted LDS work queue;
+"batch4/8": terms in batches with all lookups of a batch in flight, for
+rank-based lookups; all but "rows" for one vector only).  Candidates are timed on the operator's own
 basis with HIP events (a few launches each) and the winner is cached per
 (basis signature, dtype, nvec) in memory and in a JSON file
 (``QSX_TUNE_CACHE``, default ``~/.cache/quspin_amd/tune.json``).
@@ -19,6 +21,9 @@ import numpy as np
 from . import _lib as L
 
 _mem = {}
+
+# schedule name -> launch code of qsx_hv_ctx_set_schedule
+SCHEDULES = {"rows": 0, "queue": 1, "rows2": 2, "rows4": 3, "rows8": 4, "batch4": 5, "batch8": 6}
 
 
 def _cache_path():
@@ -55,7 +60,12 @@ def static_rule(op, vdt, nvec):
     b = op.basis
     G = int(np.prod(b._pers)) if len(b._pers) else 1
     lookup = "table" if G > 1 else "search"
-    schedule = "queue" if (nvec == 1 and op.oplist.n_off >= 4) else "rows"
+    if nvec != 1:
+        schedule = "rows"
+    elif lookup == "table" or _rank_lookup(op):
+        schedule = "batch8"
+    else:
+        schedule = "queue" if op.oplist.n_off >= 4 else "rows"
     return {"lookup": lookup, "schedule": schedule}
 
 
@@ -64,8 +74,23 @@ def candidates(op, vdt, nvec):
     lookups = ["search"]
     if len(b._pers):
         lookups.append("table")
-    scheds = ["rows", "queue"] if nvec == 1 else ["rows"]
-    return [{"lookup": lk, "schedule": sc} for lk in lookups for sc in scheds]
+    out = []
+    for lk in lookups:
+        if nvec != 1:
+            out.append({"lookup": lk, "schedule": "rows"})
+            continue
+        scheds = ["rows", "rows2", "rows4", "queue"]
+        if lk == "table" or _rank_lookup(op):
+            scheds += ["batch4", "batch8"]
+        out += [{"lookup": lk, "schedule": sc} for sc in scheds]
+    return out
+
+
+def _rank_lookup(op):
+    """No-symmetry single-sector bases index states by rank (no search)."""
+    b = op.basis
+    return not len(b._pers) and b._Np is not None and not isinstance(b._Np, (list, tuple)) or \
+        (not len(b._pers) and getattr(b, "_spinful", False) and b._Np is not None and len(b._Np) == 1)
 
 
 def choose(op, ctx, vdt, nvec):

@@ -145,3 +145,27 @@ def test_Ns_block_est_overflow():
     m = chain_maps(20)
     with pytest.raises(ValueError):
         spin_basis_general(20, Nup=10, kblock=(m["T"], 0), Ns_block_est=1001)
+
+
+def test_sector_rank_all_modes():
+    """Device combinadic rank in every sector mode (hcb sector, spinful pair,
+    full space) against direct enumeration; regression test for the
+    full-space branch."""
+    from quspin_amd import _lib as L
+    from quspin_amd.basis import spin_basis_general, spinful_fermion_basis_general
+    cases = [(spin_basis_general(8, Nup=3), 8, lambda s: bin(s).count("1") == 3),
+             (spin_basis_general(7), 7, lambda s: True),
+             (spinful_fermion_basis_general(4, Nf=(2, 1)), 8,
+              lambda s: bin(s >> 4).count("1") == 2 and bin(s & 15).count("1") == 1)]
+    for B, N, member in cases:
+        allst = np.arange(1 << N, dtype=np.uint32)
+        sd = L.to_device(allst)
+        out = L.empty(allst.size, np.int64)
+        info = np.zeros(8, np.int64)
+        L.check(L.load().qsx_sector_rank(B._core.h, L.ptr(sd), allst.size, L.ptr(out),
+                                         info.ctypes.data_as(L._vp), L.stream_handle()))
+        got = L.to_host(out)
+        members = [s for s in range(1 << N) if member(s)]
+        expect = np.full(1 << N, -1, np.int64)
+        expect[members] = np.arange(len(members))
+        np.testing.assert_array_equal(got, expect)

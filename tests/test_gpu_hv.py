@@ -250,7 +250,7 @@ def test_large_C2_hermiticity_and_linearity():
 
 
 @pytest.mark.parametrize("lookup", ["search", "table"])
-@pytest.mark.parametrize("schedule", ["rows", "queue"])
+@pytest.mark.parametrize("schedule", ["rows", "rows2", "rows4", "rows8", "queue", "batch4", "batch8"])
 def test_forced_configs_vs_oracle(lookup, schedule, monkeypatch):
     """Every (lookup, schedule) pair of the fused kernel against the oracle:
     complex k-sector, real k=0/p/z sector, fermion k-sector."""
