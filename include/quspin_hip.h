@@ -1,0 +1,203 @@
+/* libquspin_hip: C-ABI of the MI355X (gfx950) symmetry-reduced H.v path.
+ *
+ * Plain pointers and sizes only.  Every "device" pointer is HIP device
+ * memory; "host" pointers are CPU memory.  `stream` is a hipStream_t (NULL =
+ * default stream).  Functions return 0 (QSX_OK) or a negative status
+ * (QSX_ERR_*); qsx_last_error() gives the message of the calling thread's
+ * last failure.  Vectors are C-contiguous (Ns, nvec) arrays.
+ *
+ * Reference interfaces replaced (paths relative to the QuSpin 1.0.1 tree):
+ *   basis cores   quspin_extensions ... _basis_general_core.*_basis_core_wrap,
+ *                 called from src/quspin/basis/basis_general/base_general.py
+ *   sparse tools  parallel_sparse_tools.matvec._oputils._matvec and
+ *                 expm_multiply_parallel_wrapper._wrapper_* (rendered sources:
+ *                 docs/modules/parallel_sparse_tools/matvec/matvec_core.html,
+ *                 .../expm_multiply_parallel_core/expm_multiply_parallel_core.html)
+ */
+#ifndef QUSPIN_HIP_H
+#define QUSPIN_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { QSX_OK = 0, QSX_ERR_ARG = -1, QSX_ERR_HIP = -2, QSX_ERR_OVERFLOW = -3, QSX_ERR_COMPLEX = -4,
+       QSX_ERR_UNSUPPORTED = -5 };
+/* vector / matrix element types */
+enum { QSX_F32 = 0, QSX_F64 = 1, QSX_C64 = 2, QSX_C128 = 3 };
+/* basis kinds */
+enum { QSX_KIND_HCB = 0, QSX_KIND_SPINLESS_FERMION = 1, QSX_KIND_SPINFUL_FERMION = 2 };
+
+typedef struct qsx_core qsx_core;             /* symmetry group + bit layout of one basis */
+typedef struct qsx_oplist qsx_oplist;         /* compiled operator-string list */
+typedef struct qsx_hv_ctx qsx_hv_ctx;         /* bound H.v (operator, basis, dtype, nvec, flags) */
+typedef struct qsx_csr_builder qsx_csr_builder;
+
+const char* qsx_last_error(void);
+int qsx_version(void);
+
+/* ---- basis core --------------------------------------------------------- */
+
+/* Replaces hcb_basis_core_wrap(basis_dtype, N, maps, pers, qs)
+ * (base_hcb.py:74-76), spinless_fermion_basis_core_wrap (fermion.py:169-171)
+ * and spinful_fermion_basis_core_wrap(..., double_occupancy)
+ * (fermion.py:543-545).  maps: host int32[n_maps][N] (entry -(j+1) = site j
+ * with inversion), pers/qs: host int32[n_maps].  N = number of bits (2L for
+ * spinful fermions), N <= 64, n_maps <= 32. */
+int qsx_core_create(int kind, int N, const int32_t* maps, const int32_t* pers, const int32_t* qs, int n_maps,
+                    int double_occupancy, qsx_core** out);
+void qsx_core_destroy(qsx_core* core);
+int qsx_core_group_order(const qsx_core* core);
+int qsx_core_map_info(const qsx_core* core, int k, int* rev, int* ngroups, int* per);
+
+/* Replaces _core.make_basis(basis, n, Np=, count=) (base_general.py:864-867).
+ * Np: host int32 particle numbers (hcb: n_Np values; spinful: n_Np
+ * (Nup, Ndown) pairs; n_Np = 0: no conservation).  basis_out == NULL counts
+ * only.  Writes the representatives DESCENDING (uint32 for N <= 32, else
+ * uint64) and n (n_bytes = 1, 2 or 4 per entry).  Returns Ns, or -(Ns)-1 when
+ * capacity is too small (the reference's "Ns < 0" overflow signal). */
+int64_t qsx_make_basis(qsx_core* core, const int32_t* Np, int n_Np, void* basis_out, void* n_out, int n_bytes,
+                       int64_t capacity, void* stream);
+
+/* Replaces _core.make_basis_blocks(N_p, basis) (base_general.py:945-947):
+ * device int64 begin/end[2^N_p] bounding the states with each N_p-bit prefix. */
+int qsx_make_basis_blocks(qsx_core* core, int N_p, const void* basis, int64_t Ns, int64_t* begin, int64_t* end,
+                          void* stream);
+
+/* Replaces _core.representative(states, out, g_out=, sign_out=)
+ * (base_general.py:1085-1117): orbit maxima; g_out int32[n][n_maps] and
+ * sign_out int8[n] optional. */
+int qsx_representative(qsx_core* core, const void* states, void* out, int32_t* g_out, int8_t* sign_out, int64_t n,
+                       void* stream);
+
+/* Replaces _core.normalization(states, out) (base_general.py:1158, 1178). */
+int qsx_normalization(qsx_core* core, const void* states, void* out, int out_bytes, int64_t n, void* stream);
+
+/* Sector rank of arbitrary states (ascending, -1 outside the sector of the
+ * last make_basis) and the sector descriptor info[8] = {mode, Np, Nup, Ndown,
+ * sector size, sector_ok, rank_ok, N}.  Internal index of the rank lookups;
+ * no direct counterpart in the reference (its native make_basis_blocks
+ * index plays this role). */
+int qsx_sector_rank(qsx_core* core, const void* states, int64_t n, int64_t* out, int64_t* info, void* stream);
+
+/* Lookup table of the sector (state rank -> basis index | element code):
+ * size query and build.  Trades HBM for the orbit search of the H.v; no
+ * counterpart in the reference. */
+int qsx_table_info(const qsx_core* core, int64_t Ns, int64_t* entries, int* entry_bytes, int* idx_bits);
+int qsx_table_build(qsx_core* core, const void* basis, int64_t Ns, const int64_t* begin, const int64_t* end, int N_p,
+                    void* table, int entry_bytes, int idx_bits, void* stream);
+
+/* ---- operators ----------------------------------------------------------- */
+
+/* Compile an operator list: opstrs = all operator strings concatenated, lens
+ * their lengths, indx the sites concatenated, J (re, im) pairs, all host.
+ * symmetric = 1 when the list commutes with the basis symmetries (enables
+ * the one-launch gather form).  Site/character validation as in
+ * base_general.py:299-311. */
+int qsx_oplist_create(qsx_core* core, int n_terms, const char* opstrs, const int32_t* lens, const int32_t* indx,
+                      const double* J, int symmetric, qsx_oplist** out);
+void qsx_oplist_destroy(qsx_oplist* l);
+int qsx_oplist_info(const qsx_oplist* l, int* n_off, int* n_diag, int* complex_me);
+
+/* Replaces _core.op(row, col, ME, opstr, indx, J, basis, n, begin, end, N_p)
+ * (base_general.py:324-336): one element per basis state (column); ME is
+ * complex128, zero where the element vanishes; row/col int32 or int64. */
+int qsx_op(qsx_core* core, const char* opstr, const int32_t* indx, int n_op, double J_re, double J_im,
+           const void* basis, const void* n, int n_bytes, int64_t Ns, const int64_t* begin, const int64_t* end,
+           int N_p, void* row, void* col, int idx_bytes, void* me_out, void* stream);
+
+/* Replaces _core.inplace_op(v_in, v_out, conjugated, transposed, opstr, indx,
+ * J, basis, n, begin, end, N_p) (base_general.py:395-408), argument order
+ * kept: v_out += J Op v_in (transposed / conjugated). */
+int qsx_inplace_op(qsx_core* core, const void* v_in, void* v_out, int dtype, int64_t nvec, int conjugated,
+                   int transposed, const char* opstr, const int32_t* indx, int n_op, double J_re, double J_im,
+                   const void* basis, const void* n, int n_bytes, int64_t Ns, const int64_t* begin,
+                   const int64_t* end, int N_p, void* stream);
+
+/* Fused form of the loop over terms in base_general.py:392-408 plus the
+ * diagonal product of quantum_LinearOperator._matvec
+ * (quantum_LinearOperator_core.py:520-540), one launch:
+ *   v_out = beta v_out + a_diag (D + diag_arr) v_in + a_off Op v_in. */
+int qsx_hv(qsx_core* core, qsx_oplist* l, const void* basis, const void* n, int n_bytes, int64_t Ns,
+           const int64_t* begin, const int64_t* end, int N_p, const void* v_in, void* v_out, int dtype, int64_t nvec,
+           int transposed, int conjugated, int diag_conj, double a_off_re, double a_off_im, double a_diag_re,
+           double a_diag_im, int beta, const void* diag_arr, int diag_arr_cplx, void* stream);
+
+/* Bound form of qsx_hv for repeated products (Lanczos, expm): create once,
+ * choose the lookup (table) and schedule (0 one lane per row, 1 compacted
+ * queue, 2/3/4 = 2/4/8 lanes per row, 5/6 = batches of 4/8 terms), apply. */
+int qsx_hv_ctx_create(qsx_core* core, qsx_oplist* l, const void* basis, const void* n, int n_bytes, int64_t Ns,
+                      const int64_t* begin, const int64_t* end, int N_p, int dtype, int64_t nvec, int transposed,
+                      int conjugated, int diag_conj, const void* diag_arr, int diag_arr_cplx, qsx_hv_ctx** out);
+void qsx_hv_ctx_destroy(qsx_hv_ctx* ctx);
+int qsx_hv_ctx_set_table(qsx_hv_ctx* ctx, const void* table, int entry_bytes, int idx_bits);
+int qsx_hv_ctx_set_schedule(qsx_hv_ctx* ctx, int sched);
+int qsx_hv_ctx_apply(qsx_hv_ctx* ctx, const void* x, void* y, double a_off_re, double a_off_im, double a_diag_re,
+                     double a_diag_im, int beta, void* stream);
+
+/* Replaces the COO -> CSR assembly of basis._make_matrix (base.py:265-304)
+ * for hamiltonian: canonical CSR (sorted columns, duplicates summed, zeros
+ * dropped) of a*Op.  Two steps: create (returns nnz), finish (writes
+ * indptr/indices/data in the requested types), destroy. */
+int qsx_csr_builder_create(qsx_core* core, qsx_oplist* l, const void* basis, const void* n, int n_bytes, int64_t Ns,
+                           const int64_t* begin, const int64_t* end, int N_p, int transposed, int conjugated,
+                           double a_re, double a_im, qsx_csr_builder** out, int64_t* nnz, void* stream);
+int qsx_csr_builder_complex(const qsx_csr_builder* b);
+int qsx_csr_builder_finish(qsx_csr_builder* b, int dtype, int idx_bytes, void* indptr, void* indices, void* data,
+                           void* stream);
+void qsx_csr_builder_destroy(qsx_csr_builder* b);
+
+/* ---- sparse tools -------------------------------------------------------- */
+
+/* Replaces _matvec(A, x, out=, overwrite_out=, a=) for csr A
+ * (hamiltonian_core.py:607-612, 741-772; matvec_core.html): y = a A x, or
+ * y += a A x when accumulate.  idx64 selects int64 indices; group = lanes
+ * per row (0: automatic).  Matrix dtype = vector dtype or its real part. */
+int qsx_csr_matvec(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype, int64_t Ns,
+                   int64_t nnz, const void* x, void* y, int vec_dtype, int64_t nvec, double a_re, double a_im,
+                   int accumulate, int group, void* stream);
+
+/* Replace _wrapper_csr_trace(indptr, indices, data) and
+ * _wrapper_csr_1_norm(indptr, indices, data, shift) (EMP.html:196-202);
+ * results in device doubles (trace: re, im).  colsum: Ns device doubles. */
+int qsx_csr_trace(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype, int64_t Ns,
+                  double* out, void* stream);
+int qsx_csr_1_norm(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype, int64_t Ns,
+                   double shift_re, double shift_im, double* colsum, double* out, void* stream);
+
+/* Replaces _wrapper_expm_multiply(indptr, indices, data, s, m_star, tol, mu,
+ * a, v, work) and _wrapper_expm_multiply_batch (EMP.html:347-372): v <-
+ * e^{a A} v in place, work = 2*Ns*nvec elements of the vector dtype. */
+int qsx_expm_multiply_csr(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
+                          int64_t Ns, int64_t nnz, int s, int m_star, double tol, double mu_re, double mu_im,
+                          double a_re, double a_im, void* v, void* work, int vec_dtype, int64_t nvec, int group,
+                          void* stream);
+
+/* The same Taylor loop on the matrix-free operator of an H.v context (no
+ * CSR in memory); no counterpart in the reference. */
+int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star, double tol, double mu_re, double mu_im,
+                         double a_re, double a_im, void* v, void* work, int vec_dtype, int64_t nvec, void* stream);
+
+/* Lanczos recurrence pieces (src/quspin/tools/lanczos/_lanczos_utils.py:
+ * 173-204): r -= b_prev v; a = Re<q, r>; r -= a q; b = ||r|| with FP64
+ * accumulation (a_out, b_out device doubles), q = r / b, vdot, nrm2, axpy,
+ * and lin_comb_Q_T (:366-425). */
+int qsx_lanczos_update(int dtype, const void* v, const double* b_prev, const void* q, void* r, int64_t n,
+                       double* a_out, double* b_out, void* stream);
+int qsx_lanczos_div(int dtype, const void* r, const double* b, void* q, int64_t n, void* stream);
+int qsx_vdot(int dtype, const void* x, const void* y, int64_t n, double* out, void* stream);
+int qsx_nrm2(int dtype, const void* x, int64_t n, double* out, void* stream);
+int qsx_axpy(int dtype, const void* x, void* y, int64_t n, double a_re, double a_im, const double* sdev, double sgn,
+             int cplx_scale, void* stream);
+int qsx_lincomb(int dtype, const void* Q, int64_t m, int64_t n, const double* c, void* out, void* stream);
+
+/* Plain 16-byte-vector copy (reference point for HBM counters). */
+int qsx_copy(const void* src, void* dst, int64_t bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* QUSPIN_HIP_H */

@@ -289,6 +289,16 @@ _FERMION = {
 }
 
 
+def popcount64(x):
+    """Bit count of each uint64 (SWAR)."""
+    x = np.asarray(x, dtype=np.uint64).copy()
+    m1, m2, m4 = np.uint64(0x5555555555555555), np.uint64(0x3333333333333333), np.uint64(0x0F0F0F0F0F0F0F0F)
+    x = x - ((x >> np.uint64(1)) & m1)
+    x = (x & m2) + ((x >> np.uint64(2)) & m2)
+    x = (x + (x >> np.uint64(4))) & m4
+    return ((x * np.uint64(0x0101010101010101)) >> np.uint64(56)).astype(np.int64)
+
+
 def apply_term(states, opstr, indx, N, fermion=False):
     """Apply an operator string right-to-left to Fock states.
     hcb/spin-1/2 (``pauli=0``): z=+-1/2, n, +, -, x=sigma^x/2, y=sigma^y/2
@@ -307,8 +317,7 @@ def apply_term(states, opstr, indx, N, fermion=False):
         f0, f1, flip = tab[ch]
         me *= np.where(b, f1, f0)
         if fermion and flip:
-            above = s >> np.uint64(pos + 1)
-            cnt = np.array([bin(int(x)).count("1") for x in above], dtype=np.int64)
+            cnt = popcount64(s >> np.uint64(pos + 1))
             me *= np.where(cnt % 2 == 1, -1.0, 1.0)
         if flip:
             s ^= one << np.uint64(pos)

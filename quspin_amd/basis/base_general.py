@@ -38,47 +38,42 @@ def get_basis_type(N, Np, sps):
     raise NotImplementedError("bases wider than 64 bits are not supported")
 
 
+def _act(labels, m):
+    """One application of the signed site map ``m`` to signed site labels
+    (label ``-(j+1)`` = site ``j`` inverted): labels sitting at the inverted
+    entries of ``m`` are negated, then pulled through the site permutation."""
+    inv = m < 0
+    src = np.where(inv, -1 - m, m)
+    return np.where(inv, -1 - labels, labels)[src]
+
+
 def process_map(map, q):
-    """Validate a site map and compute its period (base_general.py:22-51)."""
-    map = np.asarray(map, dtype=np.int32)
-    i_map = map.copy()
-    i_map[map < 0] = -(i_map[map < 0] + 1)
-    s_map = map < 0
-    sites = np.arange(len(map), dtype=np.int32)
-    order = sites.copy()
-    if np.any(np.sort(i_map) - order):
+    """Validate a site map; return (map, period, q, set of its powers as label
+    tuples).  Same period and errors as base_general.py:22-51."""
+    m = np.asarray(map, dtype=np.int32)
+    src = np.where(m < 0, -1 - m, m)
+    ident = np.arange(m.size, dtype=np.int32)
+    if not np.array_equal(np.sort(src), ident):
         raise ValueError("map must be a one-to-one site mapping.")
-    per = 0
-    group = [tuple(order)]
-    while True:
-        sites[s_map] = -(sites[s_map] + 1)
-        sites = sites[i_map]
-        per += 1
-        group.append(tuple(sites))
-        if np.array_equal(order, sites):
-            break
+    powers = {tuple(ident)}
+    cur, per = _act(ident, m), 1
+    while not np.array_equal(cur, ident):
+        powers.add(tuple(cur))
+        cur, per = _act(cur, m), per + 1
     if per == 1:
         warnings.warn("identity mapping found in set of transformations.", GeneralBasisWarning, stacklevel=5)
-    return map, per, q, set(group)
+    return m, per, q, powers
 
 
 def _check_symmetry_maps(item1, item2):
-    grp1, map1, block1 = item1[1][-1], item1[1][0], item1[0]
-    grp2, map2, block2 = item2[1][-1], item2[1][0], item2[0]
+    """Warnings of base_general.py:54-98: two blocks generating the same
+    cyclic group, or maps that do not commute."""
+    (name1, (m1, _, _, grp1)), (name2, (m2, _, _, grp2)) = item1, item2
     if grp1 == grp2:
-        warnings.warn("mappings for block {} and block {} produce the same symmetry.".format(block1, block2),
+        warnings.warn("mappings for block {} and block {} produce the same symmetry.".format(name1, name2),
                       GeneralBasisWarning, stacklevel=5)
-
-    def act(sites, m):
-        i_m = m.copy()
-        i_m[m < 0] = -(i_m[m < 0] + 1)
-        s = sites.copy()
-        s[m < 0] = -(s[m < 0] + 1)
-        return s[i_m]
-
-    a = act(act(np.arange(len(map1)), map1), map2)
-    b = act(act(np.arange(len(map2)), map2), map1)
-    if not np.array_equal(a, b):
+    ident = np.arange(m1.size)
+    if not np.array_equal(_act(_act(ident, m1), m2), _act(_act(ident, m2), m1)):
         warnings.warn("using non-commuting symmetries can lead to unwanted behaviour of general basis, make sure "
                       "that quantum numbers are invariant under non-commuting symmetries!",
                       GeneralBasisWarning, stacklevel=5)
@@ -89,6 +84,7 @@ class _Core:
     ``qsx_core_create``)."""
 
     def __init__(self, kind, N, maps, pers, qs, double_occupancy=True):
+        L.require_device()
         lib = L.load()
         self.kind, self.N = kind, N
         maps = np.ascontiguousarray(maps, dtype=np.int32).reshape(-1, N) if len(pers) else np.zeros((0, N), np.int32)

@@ -1,0 +1,194 @@
+"""Pins of the oracle (CPU only): known answers printed by the reference's
+notebooks and tests, closed forms, and brute force on tiny systems.  Each
+test is chosen so that a plausible mistake in the oracle (a dropped term, a
+wrong sign or index, a transposed operand, a wrong phase convention) fails."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+from scipy.special import comb
+
+from oracle import qsx_oracle as O
+from tests.models import chain_maps, flat_terms, xxz_static
+
+
+def test_ssh_momentum_block_printout():
+    """SSH.ipynb cell 5: L=6, one fermion, translation by a=2, kblock=1 prints
+    [[0.5, -0.65-0.78j], [-0.65+0.78j, -0.5]] (pull action, chi = e^{+2pi i q l/m})."""
+    L, J, dJ, D = 6, 1.0, 0.1, 0.5
+    T2 = [(i + 2) % L for i in range(L)]
+    g = O.Group(L, [T2], [3], [1], fermion=True)
+    b, n = O.make_basis(L, g, Np=1)
+    static = ([("+-", (i, (i + 1) % L), -J - dJ * (-1) ** i) for i in range(L)]
+              + [("-+", (i, (i + 1) % L), J + dJ * (-1) ** i) for i in range(L)]
+              + [("n", (i,), D * (-1) ** i) for i in range(L)])
+    H = O.reduced_op(b, n, g, static, fermion=True)
+    np.testing.assert_allclose(np.round(H, 2), [[0.5, -0.65 - 0.78j], [-0.65 + 0.78j, -0.5]], atol=1e-12)
+    # k = 2 block is the complex conjugate (SSH.ipynb cell 5, last block)
+    g2 = O.Group(L, [T2], [3], [2], fermion=True)
+    H2 = O.reduced_op(b, n, g2, static, fermion=True)
+    np.testing.assert_allclose(H2, H.conj(), atol=1e-12)
+
+
+@pytest.mark.parametrize("L", [8, 10])
+def test_reduced_equals_projected_full(L):
+    """A.5 element formula == P^H H_full P with the get_proj coefficient
+    1/sqrt(n |G|) (base_general.py:571-575), P^H P = 1, and the union of all
+    sector spectra is the full spectrum."""
+    m = chain_maps(L)
+    st = flat_terms(xxz_static(L, Delta=0.6))
+    full = O.full_basis_states(L, L // 2)
+    Hf = O.full_hamiltonian(full, L, st)
+    Es = []
+    for k in range(L):
+        for z in (0, 1):
+            items = O.order_maps({"k": (m["T"], k), "z": (m["Z"], z)})
+            g = O.Group(L, [it[1] for it in items], [it[2] for it in items], [it[3] for it in items])
+            b, n = O.make_basis(L, g, Np=L // 2)
+            if len(b) == 0:
+                continue
+            P = O.projector(b, n, g, full)
+            np.testing.assert_allclose(P.conj().T @ P, np.eye(len(b)), atol=1e-12)
+            np.testing.assert_allclose(O.reduced_op(b, n, g, st), P.conj().T @ Hf @ P, atol=1e-12)
+            Es.extend(np.linalg.eigvalsh(O.reduced_op(b, n, g, st)))
+    np.testing.assert_allclose(np.sort(Es), np.linalg.eigvalsh(Hf), atol=1e-10)
+
+
+def test_sector_sizes_closed_form():
+    """Sum over k of the k-sector sizes is C(L, L/2); L=12 k=0,p=0,z=0 size
+    from Burnside's count of symmetric orbits."""
+    L = 12
+    m = chain_maps(L)
+    tot = 0
+    for k in range(L):
+        g = O.Group(L, [m["T"]], [L], [k])
+        tot += len(O.make_basis(L, g, Np=L // 2)[0])
+    assert tot == comb(L, L // 2, exact=True)
+
+
+def test_spin_operator_table_vs_kron():
+    """Local ops (pauli=0) and the site<->bit convention against explicit
+    2x2 spin matrices and Kronecker products (site 0 = leftmost factor =
+    highest bit)."""
+    L = 3
+    Sp = np.array([[0, 1], [0, 0]], complex)  # basis (up, down) = (bit 1, bit 0)
+    Sm = Sp.T
+    Sz = np.diag([0.5, -0.5]).astype(complex)
+    Sx = 0.5 * np.array([[0, 1], [1, 0]], complex)
+    Sy = 0.5 * np.array([[0, -1j], [1j, 0]])
+    ops = {"+": Sp, "-": Sm, "z": Sz, "x": Sx, "y": Sy, "n": np.diag([1.0, 0.0]).astype(complex)}
+    full = O.full_basis_states(L)  # descending: index 0 = all up = kron index 0
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        chars = "".join(rng.choice(list(ops), size=2))
+        sites = tuple(rng.choice(L, size=2, replace=False))
+        M = O.full_hamiltonian(full, L, [(chars, sites, 1.0)])
+        mats = [np.eye(2, dtype=complex)] * L
+        mats = list(mats)
+        for ch, i in zip(chars, sites):
+            mats[i] = ops[ch] @ mats[i]
+        K = mats[0]
+        for A in mats[1:]:
+            K = np.kron(K, A)
+        np.testing.assert_allclose(M, K, atol=1e-14)
+
+
+def test_fermion_anticommutation():
+    """{c_i, c_j^+} = delta_ij and {c_i, c_j} = 0 on the full Fock space."""
+    L = 4
+    full = O.full_basis_states(L)
+
+    def mat(ch, i):
+        M = np.zeros((full.size, full.size))
+        me, s2 = O.apply_term(full, ch, (i,), L, fermion=True)
+        rows = O.index_of(full, s2)
+        ok = me != 0
+        M[rows[ok], np.nonzero(ok)[0]] = me[ok].real
+        return M
+
+    for i in range(L):
+        for j in range(L):
+            ci, cjd = mat("-", i), mat("+", j)
+            np.testing.assert_allclose(ci @ cjd + cjd @ ci, np.eye(full.size) * (i == j), atol=1e-14)
+            cj = mat("-", j)
+            np.testing.assert_allclose(ci @ cj + cj @ ci, 0, atol=1e-14)
+
+
+@pytest.mark.parametrize("U,E", [(0.0, -12.0), (1.0, -11.8364690235642218), (2.0, -11.7104580743242632),
+                                 (4.0, -11.5302924026297138)])
+def test_hubbard_4x4_literature_energies(U, E):
+    """test_FHM_energies.py:21-33 (arXiv:cond-mat/0604319): 4x4 torus,
+    Nf=(2,2), hopping +-| with J=+1, -+| with J=-1, U n|n at U = linspace(0,4,9)[j];
+    the reference asserts atol 1e-13, eigsh here reaches ~1e-12."""
+    from scipy.sparse.linalg import eigsh
+    Lx = Ly = 4
+    N = Lx * Ly
+    i = np.arange(N)
+    x, y = i % Lx, i // Lx
+    tx, ty = (x + 1) % Lx + y * Lx, x + ((y + 1) % Ly) * Lx
+    terms = []
+    for j in range(N):
+        for t in (tx[j], ty[j]):
+            terms += [("+-", (j, int(t)), 1.0), ("-+", (j, int(t)), -1.0),
+                      ("+-", (j + N, int(t) + N), 1.0), ("-+", (j + N, int(t) + N), -1.0)]
+        if U:
+            terms.append(("nn", (j, j + N), U))
+    g = O.Group(2 * N, [], [], [], fermion=True)
+    # the (2,2) sector directly from 16-bit halves (up in the high half)
+    import itertools
+    half = [sum(1 << p for p in c) for c in itertools.combinations(range(N), 2)]
+    states = np.array(sorted(((u << N) | d for u in half for d in half), reverse=True), dtype=np.uint64)
+    H = O.make_csr(states, np.ones(states.size, np.int64), g, terms, np.float64, fermion=True)
+    e = eigsh(H, k=1, which="SA", tol=1e-14)[0][0]
+    assert abs(e - E) < 1e-11
+
+
+def test_expm_against_dense_and_scipy_partition():
+    """Taylor loop vs scipy.linalg.expm on a small matrix; (m*, s) equals
+    scipy's _fragment_3_1 choice for the same norm."""
+    from scipy.linalg import expm
+    rng = np.random.default_rng(0)
+    A = sp.random(60, 60, density=0.2, random_state=3).tocsr()
+    A = A + A.T
+    v = rng.normal(size=60) + 1j * rng.normal(size=60)
+    for a in (-0.5j, -3.0j, 0.7):
+        got, part = O.expm_multiply(A, v, a=a)
+        np.testing.assert_allclose(got, expm(a * A.toarray()) @ v, atol=1e-12)
+        assert part[0] >= 1 and part[1] >= 1
+
+
+def test_lanczos_ground_energy():
+    L = 10
+    m = chain_maps(L)
+    g = O.Group(L, [m["T"]], [L], [0])
+    b, n = O.make_basis(L, g, Np=L // 2)
+    H = O.make_csr(b, n, g, flat_terms(xxz_static(L)), np.float64)
+    v0 = np.random.default_rng(0).normal(size=len(b))
+    E, V, Q, a, bb = O.lanczos_full(lambda x: H @ x, v0, 60)
+    assert abs(E[0] - np.linalg.eigvalsh(H.toarray())[0]) < 1e-10
+    assert np.allclose(Q[:10] @ Q[:10].T, np.eye(10), atol=1e-8)
+
+
+def test_c_oracle_matches_numpy_oracle():
+    """The C/OpenMP restatement (used at full sizes and as the CPU baseline)
+    equals the numpy oracle on small sectors: basis, n and H.v."""
+    from oracle import c_oracle as C
+    for L, k, z in [(12, 0, 1), (12, 3, None), (11, 2, None)]:
+        m = chain_maps(L)
+        blocks = {"k": (m["T"], k)}
+        if z is not None:
+            blocks["z"] = (m["Z"], z)
+        items = O.order_maps(blocks)
+        maps, pers, qs = [it[1] for it in items], [it[2] for it in items], [it[3] for it in items]
+        g = O.Group(L, maps, pers, qs)
+        b, n = O.make_basis(L, g, Np=L // 2)
+        cg = C.Group(L, maps, pers, qs)
+        cb, cn = cg.make_basis(L // 2)
+        np.testing.assert_array_equal(cb, b.astype(np.uint64))
+        np.testing.assert_array_equal(cn, n)
+        if (2 * k) % L == 0:
+            terms = flat_terms(xxz_static(L, Delta=0.4))
+            H = O.make_csr(b, n, g, terms, np.float64)
+            v = np.random.default_rng(0).normal(size=len(b))
+            for fused in (True, False):
+                np.testing.assert_allclose(C.hv_real(cg, cb, cn, terms, v, fused=fused), H @ v, atol=1e-12)
