@@ -26,6 +26,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.0, write
+# x1.0); keyed by (L, lookup, schedule, dtype).  Source and commands:
+# profiles/k_hv_rows_heisenberg_L28_kpz.md
+MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 93.5e6}
+
 
 def chain_static(L, J=1.0, Delta=1.0):
     hop = [[0.5 * J, i, (i + 1) % L] for i in range(L)]
@@ -136,6 +142,8 @@ def main():
                "sample": f"{nrep} fused H.v calls on the same L={L} k=0,p=0,z=0 sector ({Ns} states), "
                          f"C/OpenMP restatement (oracle/csrc/qsx_oracle.c); max|GPU-CPU| = {err:.2e}"}
 
+    cfg = next(iter(op.config.values()), {})
+    traffic = MEASURED_TRAFFIC.get((L, cfg.get("lookup"), cfg.get("schedule"), "float64"))
     line = {
         "metric": "H.v matvecs/s (symmetry-reduced matrix-free, fp64)",
         "value": round(value, 2),
@@ -153,7 +161,7 @@ def main():
                                "quantum_LinearOperator.dot on a seeded Gaussian vector",
                    "Ns": Ns, "basis_build_s": round(t_basis, 3), "wall_s": round(t_wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,
         "lanczos": lz,
         "kernel_config": {str(k): v for k, v in op.config.items()},
