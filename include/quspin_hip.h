@@ -196,6 +196,44 @@ int qsx_lincomb(int dtype, const void* Q, int64_t m, int64_t n, const double* c,
 /* Plain 16-byte-vector copy (reference point for HBM counters). */
 int qsx_copy(const void* src, void* dst, int64_t bytes, void* stream);
 
+/* ---- batched sectors x vectors ---------------------------------------------- */
+
+/* Many symmetry sectors (one H.v context each, same dtype and site count, each
+ * with a lookup table or rank lookup) applied to nvec vectors per sector in
+ * ONE launch: vectors of all sectors are stacked row-wise, sector k from row
+ * row_off[k] (qsx_hv_batch_rows).  One wavefront per row, lanes over the
+ * vectors.  No single counterpart in the reference: it batches the
+ * per-sector quantum_LinearOperator.dot / basis.inplace_Op calls
+ * (quantum_LinearOperator_core.py:288-347, base_general.py:354-410). */
+typedef struct qsx_hv_batch qsx_hv_batch;
+int qsx_hv_batch_create(int nsec, qsx_hv_ctx* const* ctxs, int64_t nvec, int chunk_rows, qsx_hv_batch** out);
+/* CSR form of a batch (same apply / expm entry points): sector k is a device
+ * CSR matrix with Ns[k] rows (indptr int64 when idx64[k], int32 indices,
+ * data of the vector dtype) -- the matrices hamiltonian.tocsr() builds
+ * (hamiltonian_core.py:607-612; base.py:265-304), applied to all sectors at
+ * once.  chunk_rows <= 63. */
+int qsx_csr_batch_create(int nsec, const int64_t* Ns, const void* const* indptr, const int32_t* idx64,
+                         const void* const* indices, const void* const* data, int dtype, int64_t nvec,
+                         int chunk_rows, qsx_hv_batch** out);
+void qsx_hv_batch_destroy(qsx_hv_batch* b);
+int qsx_hv_batch_rows(const qsx_hv_batch* b, int64_t* row_off);
+/* Rows per wavefront chunk; xcd_slices = 1 gives each XCD a contiguous slice
+ * of the chunks, 0 keeps dispatch order (tuning only: results are equal). */
+int qsx_hv_batch_set_schedule(qsx_hv_batch* b, int chunk_rows, int xcd_slices);
+/* y = a * H x (+ y if beta) for every sector. */
+int qsx_hv_batch_apply(qsx_hv_batch* b, const void* x, void* y, double a_re, double a_im, int beta, void* stream);
+
+/* e^{a H_k} V_k for every sector k in lockstep: per-sector partition (s[k],
+ * m[k]), tolerance tol[k] and shift mu[2k] (re, im) exactly as
+ * expm_multiply_parallel(H_k, a).dot(V_k) would use them (EMP.html:267-374,
+ * _wrapper_expm_multiply_batch; Al-Mohy & Higham Alg. 3.2, scipy
+ * _expm_multiply.py:280-303).  v = stacked F, overwritten; work = 2 * rows *
+ * nvec elements; nvec <= 64.  use_graph = 1 captures the whole launch
+ * sequence as one hipGraph per (v, work, s_max, m_max) and replays it. */
+int qsx_expm_multiply_batch(qsx_hv_batch* b, const int32_t* s, const int32_t* m, const double* tol,
+                            const double* mu, double a_re, double a_im, void* v, void* work, int use_graph,
+                            void* stream);
+
 #ifdef __cplusplus
 }
 #endif

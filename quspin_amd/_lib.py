@@ -80,6 +80,13 @@ SIGNATURES = {
     "qsx_lincomb": (_i32, [_i32, _vp, _i64, _i64, _vp, _vp, _vp]),
     "qsx_copy": (_i32, [_vp, _vp, _i64, _vp]),
     "qsx_sector_rank": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp]),
+    "qsx_hv_batch_create": (_i32, [_i32, _vp, _i64, _i32, C.POINTER(_vp)]),
+    "qsx_csr_batch_create": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _i32, _i64, _i32, C.POINTER(_vp)]),
+    "qsx_hv_batch_destroy": (None, [_vp]),
+    "qsx_hv_batch_rows": (_i32, [_vp, _vp]),
+    "qsx_hv_batch_set_schedule": (_i32, [_vp, _i32, _i32]),
+    "qsx_hv_batch_apply": (_i32, [_vp, _vp, _vp, _dbl, _dbl, _i32, _vp]),
+    "qsx_expm_multiply_batch": (_i32, [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _i32, _vp]),
 }
 
 _lib = None

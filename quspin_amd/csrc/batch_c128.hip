@@ -1,0 +1,4 @@
+// Instantiations of the batched wavefront-per-row kernels for double2 vectors.
+#include "batch_impl.h"
+
+QSX_DEFINE_WAVE_LAUNCH(qsx_wave_launch_c128, double2)

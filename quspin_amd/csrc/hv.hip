@@ -650,6 +650,36 @@ extern "C" int qsx_hv_ctx_apply(qsx_hv_ctx* X, const void* x, void* y, double a_
   return hv_launch(X->c, X->L, A, X->transposed, X->conjugated, X->dtype, (hipStream_t)stream);
 }
 
+// Kernel arguments of a context for the batched wavefront kernels: the
+// gather-form term list and the lookup mode.  Sectors whose product needs the
+// scatter form, or whose lookup is the orbit search, are not batched.
+int qsx_hv_ctx_export(const qsx_hv_ctx* X, HvArgs* A, int* dtype, int* N, int* lookup, int* fermion) {
+  if (!X || !A) { qsx_set_error("qsx_hv_batch: null context"); return QSX_ERR_ARG; }
+  const qsx_core* c = X->c;
+  const qsx_oplist_full* L = X->L;
+  *A = X->A;
+  const int lk = A->table ? LOOKUP_TABLE : pick_lookup(c, A->Ns);
+  if (lk == LOOKUP_SEARCH) {
+    qsx_set_error("qsx_hv_batch: every sector needs a lookup table (or a symmetry-free single particle sector)");
+    return QSX_ERR_UNSUPPORTED;
+  }
+  if (c->h.nmaps > 0 && !L->symmetric && !X->transposed && L->base.n_off > 0) {
+    qsx_set_error("qsx_hv_batch: operator lists that do not commute with the symmetries are not batched");
+    return QSX_ERR_UNSUPPORTED;
+  }
+  A->off = X->transposed ? L->base.off : L->off_hc;
+  A->n_off = L->base.n_off;
+  A->off_conj = (X->transposed == X->conjugated) ? 1 : 0;
+  A->v_in = nullptr;
+  A->v_out = nullptr;
+  A->skip = nullptr;
+  *dtype = X->dtype;
+  *N = c->h.N;
+  *lookup = lk;
+  *fermion = c->h.fermion != 0;
+  return QSX_OK;
+}
+
 int qsx_hv_expm_term(void* ctx, const void* x, void* y, double2 coef, double2 shift, const int* skip,
                      hipStream_t st) {
   qsx_hv_ctx* X = (qsx_hv_ctx*)ctx;

@@ -210,6 +210,130 @@ class MatrixFreeOp:
         L.check(rc, "qsx_expm_multiply_mf")
 
 
+class SectorBatch:
+    """Many sectors x ``nvec`` vectors in one launch (``qsx_hv_batch_*``).
+
+    ``ops`` are matrix-free operators of the same dtype and site count (one
+    per symmetry sector).  Their vectors are stacked row-wise into one
+    ``(rows, nvec)`` device array, sector k starting at ``row_off[k]``.
+    ``form``: "mf" evaluates the matrix elements on the fly (wavefront per
+    row); "csr" builds every sector's CSR matrix once on the device and
+    streams it (``qsx_csr_batch_create``); None lets the autotuner time both
+    and keep the faster."""
+
+    def __new__(cls, ops, vdt, nvec, form="mf", chunk_rows=None):
+        if form is None:
+            form = tuning.choose_batch_form(cls, ops, vdt, nvec)
+        self = super().__new__(cls)
+        self._form = form
+        return self
+
+    def __init__(self, ops, vdt, nvec, form="mf", chunk_rows=None):
+        if not ops:
+            raise ValueError("at least one sector is needed")
+        for op in ops:
+            if not isinstance(op, MatrixFreeOp):
+                raise TypeError("batched sectors need matrix-free operators")
+        self.ops = list(ops)
+        self.vdt = np.dtype(vdt)
+        self.nvec = int(nvec)
+        self.form = self._form
+        lib = L.load()
+        h = L._vp()
+        n = len(self.ops)
+        if self.form == "mf":
+            self._ctxs = []
+            for op in self.ops:
+                ctx = op._new_ctx(self.vdt, self.nvec)
+                ctx.configure({"lookup": "table" if len(op.basis._pers) else "search", "schedule": "rows"},
+                              op.basis)
+                self._ctxs.append(ctx)
+            arr = (C.c_void_p * n)(*[c.h.value for c in self._ctxs])
+            L.check(lib.qsx_hv_batch_create(n, C.cast(arr, C.c_void_p), self.nvec, int(chunk_rows or 16),
+                                            C.byref(h)), "qsx_hv_batch_create")
+        elif self.form == "csr":
+            self._csr = [op.to_csr(self.vdt) for op in self.ops]
+            for M in self._csr:
+                if L.np_dtype(M.indices.dtype) != np.int32:
+                    raise NotImplementedError("CSR batches need int32 column indices")
+                if L.np_dtype(M.data.dtype) != self.vdt:
+                    raise TypeError("CSR batch values must have the vector dtype")
+            Ns = np.array([M.shape[0] for M in self._csr], np.int64)
+            ip = (C.c_void_p * n)(*[M.indptr.data_ptr() for M in self._csr])
+            i64 = np.array([int(L.np_dtype(M.indptr.dtype) == np.int64) for M in self._csr], np.int32)
+            ix = (C.c_void_p * n)(*[M.indices.data_ptr() for M in self._csr])
+            dt = (C.c_void_p * n)(*[M.data.data_ptr() for M in self._csr])
+            L.check(lib.qsx_csr_batch_create(n, Ns.ctypes.data_as(L._vp), C.cast(ip, C.c_void_p),
+                                             i64.ctypes.data_as(L._vp), C.cast(ix, C.c_void_p),
+                                             C.cast(dt, C.c_void_p), L.dtype_code(self.vdt), self.nvec,
+                                             int(min(chunk_rows or 16, 63)), C.byref(h)), "qsx_csr_batch_create")
+        else:
+            raise ValueError("form must be 'mf', 'csr' or None")
+        self.h = h
+        ro = np.zeros(n + 1, np.int64)
+        lib.qsx_hv_batch_rows(self.h, ro.ctypes.data_as(L._vp))
+        self.row_off = ro
+        self.rows = int(ro[-1])
+        self.chunk_rows, self.xcd_slices = int(chunk_rows or 16), 0
+        if chunk_rows is None:
+            self.set_schedule(*tuning.choose_batch_schedule(self))
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None) is not None and L._lib is not None:
+                L._lib.qsx_hv_batch_destroy(self.h)
+        except Exception:
+            pass
+
+    def set_schedule(self, chunk_rows, xcd_slices=0):
+        """Rows per wavefront chunk and chunk order (XCD slices or dispatch order)."""
+        L.check(L.load().qsx_hv_batch_set_schedule(self.h, int(chunk_rows), int(bool(xcd_slices))),
+                "qsx_hv_batch_set_schedule")
+        self.chunk_rows, self.xcd_slices = int(chunk_rows), int(bool(xcd_slices))
+
+    def stack(self, vs):
+        """One (rows, nvec) device array from per-sector (Ns_k, nvec) arrays."""
+        t = L.torch()
+        parts = [L.to_device(np.asarray(v), self.vdt) if not L.is_device_tensor(v) else v.to(L.torch_dtype(self.vdt))
+                 for v in vs]
+        return t.cat([p.reshape(p.shape[0], self.nvec) for p in parts], 0).contiguous()
+
+    def unstack(self, X):
+        return [X[self.row_off[k]:self.row_off[k + 1]] for k in range(len(self.ops))]
+
+    def empty(self):
+        return L.empty((self.rows, self.nvec), self.vdt)
+
+    def apply(self, x, y, a=1.0, beta=0):
+        """y = a H x (+ y), every sector in one launch."""
+        self._check(x)
+        self._check(y)
+        ac = complex(a)
+        L.check(L.load().qsx_hv_batch_apply(self.h, ptr(x), ptr(y), ac.real, ac.imag, int(bool(beta)),
+                                            stream_handle()), "qsx_hv_batch_apply")
+
+    def expm_inplace(self, v, work, parts, tols, mus, a, graph=True):
+        """v <- e^{a H_k} v_k for every sector (per-sector (s, m), tol, mu)."""
+        self._check(v)
+        if work.numel() != 2 * self.rows * self.nvec or L.np_dtype(work.dtype) != self.vdt:
+            raise ValueError("work must hold 2 * rows * nvec elements of the vector dtype")
+        s = np.ascontiguousarray([p[1] for p in parts], dtype=np.int32)
+        m = np.ascontiguousarray([p[0] for p in parts], dtype=np.int32)
+        tol = np.ascontiguousarray(tols, dtype=np.float64)
+        mu = np.ascontiguousarray(np.column_stack([np.real(mus), np.imag(mus)]).ravel(), dtype=np.float64)
+        ac = complex(a)
+        L.check(L.load().qsx_expm_multiply_batch(self.h, s.ctypes.data_as(L._vp), m.ctypes.data_as(L._vp),
+                                                 tol.ctypes.data_as(L._vp), mu.ctypes.data_as(L._vp), ac.real,
+                                                 ac.imag, ptr(v), ptr(work), int(bool(graph)), stream_handle()),
+                "qsx_expm_multiply_batch")
+
+    def _check(self, x):
+        if tuple(x.shape) not in ((self.rows, self.nvec),) + (((self.rows,),) if self.nvec == 1 else ()):
+            raise ValueError("expected a stacked ({}, {}) array, got {}".format(self.rows, self.nvec, tuple(x.shape)))
+        if L.np_dtype(x.dtype) != self.vdt or not x.is_contiguous():
+            raise ValueError("stacked arrays must be contiguous {}".format(self.vdt))
+
+
 class _Ctx:
     def __init__(self, h):
         self.h = h

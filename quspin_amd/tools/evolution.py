@@ -1,5 +1,6 @@
 """``quspin.tools.evolution.expm_multiply_parallel`` (tools/evolution.py:11-16)."""
 
-from .expm_multiply_parallel_core import expm_multiply_parallel, ExpmMultiplyParallel  # noqa: F401
+from .expm_multiply_parallel_core import (  # noqa: F401
+    ExpmMultiplyParallel, expm_multiply_parallel, expm_multiply_sectors)
 
-__all__ = ["expm_multiply_parallel", "ExpmMultiplyParallel"]
+__all__ = ["expm_multiply_parallel", "ExpmMultiplyParallel", "expm_multiply_sectors"]

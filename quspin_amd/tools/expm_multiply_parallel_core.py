@@ -20,7 +20,7 @@ from scipy.sparse.linalg import LinearOperator, onenormest
 from scipy.sparse.linalg._expm_multiply import _fragment_3_1
 
 from .. import _lib as L
-from ..operators._devop import DeviceCSR, MatrixFreeOp, as_device_operator
+from ..operators._devop import DeviceCSR, MatrixFreeOp, SectorBatch, as_device_operator
 
 
 class LazyOperatorNormInfo:
@@ -201,3 +201,60 @@ class expm_multiply_parallel:
 
 
 ExpmMultiplyParallel = expm_multiply_parallel
+
+
+class expm_multiply_sectors:
+    """e^{a A_k} V_k for many symmetry sectors k and ``nvec`` vectors each,
+    all sectors in one launch per Taylor term (optionally one hipGraph replay
+    for the whole loop).
+
+    No counterpart of its own in the reference: sector k gives what
+    ``expm_multiply_parallel(A_k, a, dtype).dot(V_k)`` gives (EMP.html:117-374),
+    with the same per-sector mu, tolerance and (m*, s).  ``ops`` are
+    ``quantum_LinearOperator`` (or other matrix-free) operators of equal
+    dtype and site count.  ``form``: "mf" (matrix elements on the fly), "csr"
+    (per-sector CSR built once on the device) or None (autotuned)."""
+
+    def __init__(self, ops, a=1.0, dtype=None, nvec=1, chunk_rows=None, graph=True, form=None):
+        self._single = [expm_multiply_parallel(op, a=a, dtype=dtype) for op in ops]
+        self._dtype = self._single[0]._dtype
+        for e in self._single:
+            if e._dtype != self._dtype:
+                raise ValueError("all sectors must share one dtype")
+        mfs = [e._op for e in self._single]
+        self._batch = SectorBatch(mfs, self._dtype, nvec, form=form, chunk_rows=chunk_rows)
+        self._a = a
+        self._graph = bool(graph)
+        self._work = None
+
+    @property
+    def batch(self):
+        return self._batch
+
+    @property
+    def partitions(self):
+        """Per-sector (m_star, s)."""
+        return [e.partition for e in self._single]
+
+    def set_partitions(self, parts):
+        for e, (m, s) in zip(self._single, parts):
+            e.set_partition(m, s)
+
+    def dot(self, V, overwrite_v=False):
+        """``V``: stacked (rows, nvec) device array, or a list of per-sector
+        arrays.  Returns the stacked result (in place when overwrite_v)."""
+        B = self._batch
+        if isinstance(V, (list, tuple)):
+            F = B.stack(V)
+        else:
+            if L.np_dtype(V.dtype) != B.vdt:
+                raise ValueError("stacked input must have dtype {}".format(B.vdt))
+            F = V if overwrite_v else V.clone()
+            F = F.reshape(B.rows, B.nvec)
+        if self._work is None:
+            self._work = L.empty(2 * B.rows * B.nvec, B.vdt)
+        parts = [e.partition for e in self._single]
+        tols = [float(e._tol) for e in self._single]
+        mus = [complex(e._mu) for e in self._single]
+        B.expm_inplace(F, self._work, parts, tols, mus, complex(self._a), graph=self._graph)
+        return F

@@ -146,3 +146,94 @@ def time_candidates(op, ctx, vdt, nvec, reps=5):
     best["ms"] = round(best_ms, 5)
     best["timings"] = [[c["lookup"], c["schedule"], round(ms, 5)] for c, ms in results]
     return best
+
+
+# (rows per wavefront chunk, XCD slices) candidates of the batched sector kernel
+BATCH_SCHEDULES = tuple((c, x) for x in (0, 1) for c in (4, 8, 16, 32))
+
+
+def choose_batch_schedule(batch, reps=3):
+    """Chunk size and chunk order for a sector batch: timed on the batch's own
+    sectors (one H.V launch per candidate and repetition), cached like the
+    single-operator choices.  ``QSX_BATCH_CHUNK`` / ``QSX_BATCH_XCD`` force
+    values."""
+    fc, fx = os.environ.get("QSX_BATCH_CHUNK"), os.environ.get("QSX_BATCH_XCD")
+    if fc or fx:
+        return int(fc or 16), int(fx or 0)
+    if os.environ.get("QSX_TUNE", "on") == "off" or batch.rows < 65536:
+        return 16, 0
+    key = _batch_key(batch.ops, batch.vdt, batch.nvec) + "_" + batch.form
+    if key in _mem:
+        return tuple(_mem[key]["schedule"])
+    disk = _load_file()
+    if key in disk:
+        _mem[key] = disk[key]
+        return tuple(disk[key]["schedule"])
+    t = L.torch()
+    x = t.ones((batch.rows, batch.nvec), dtype=L.torch_dtype(batch.vdt), device="cuda")
+    y = t.empty_like(x)
+    st = t.cuda.current_stream()
+    res = []
+    for ch, xs in BATCH_SCHEDULES:
+        if batch.form == "csr" and ch > 63:
+            continue
+        batch.set_schedule(ch, xs)
+        batch.apply(x, y)
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            batch.apply(x, y)
+        e1.record(st)
+        t.cuda.synchronize()
+        res.append((e0.elapsed_time(e1) / reps, (ch, xs)))
+    ms, best = min(res)
+    entry = {"schedule": list(best), "ms": round(ms, 5), "timings": [[list(c), round(m_, 5)] for m_, c in res]}
+    _mem[key] = entry
+    disk[key] = entry
+    _save_file(disk)
+    return best
+
+
+def _batch_key(ops, vdt, nvec):
+    rows = sum(op.Ns for op in ops)
+    return "batch_rows{}_nsec{}_nv{}_{}_N{}".format(rows, len(ops), int(nvec), np.dtype(vdt).name, ops[0].basis.N)
+
+
+def choose_batch_form(cls, ops, vdt, nvec):
+    """'mf' or 'csr' for a sector batch: both forms are built and timed with
+    their own tuned schedules (cached; ``QSX_BATCH_FORM`` forces one)."""
+    forced = os.environ.get("QSX_BATCH_FORM")
+    if forced:
+        return forced
+    rows = sum(op.Ns for op in ops)
+    if os.environ.get("QSX_TUNE", "on") == "off" or rows < 65536:
+        return "csr" if nvec >= 8 else "mf"
+    key = _batch_key(ops, vdt, nvec) + "_form"
+    if key in _mem:
+        return _mem[key]["form"]
+    disk = _load_file()
+    if key in disk:
+        _mem[key] = disk[key]
+        return disk[key]["form"]
+    t = L.torch()
+    res = []
+    for form in ("mf", "csr"):
+        b = cls(ops, vdt, nvec, form=form)
+        x = t.ones((b.rows, b.nvec), dtype=L.torch_dtype(b.vdt), device="cuda")
+        y = t.empty_like(x)
+        b.apply(x, y)
+        st = t.cuda.current_stream()
+        e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(3):
+            b.apply(x, y)
+        e1.record(st)
+        t.cuda.synchronize()
+        res.append((round(e0.elapsed_time(e1) / 3, 5), form))
+        del b, x, y
+    ms, best = min(res)
+    entry = {"form": best, "timings": [[f, m_] for m_, f in res]}
+    _mem[key] = entry
+    disk[key] = entry
+    _save_file(disk)
+    return best
