@@ -166,6 +166,12 @@ int qsx_csr_trace(const void* indptr, const void* indices, const void* data, int
                   double* out, void* stream);
 int qsx_csr_1_norm(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype, int64_t Ns,
                    double shift_re, double shift_im, double* colsum, double* out, void* stream);
+/* Same with an extra diagonal: ||A + shift I + xs diag(xdiag)||_1, the norm
+ * LazyOperatorNormInfo needs (EMP.html:399-466) for H(t) = A + f(t) D with a
+ * diagonal dynamic part D (xdiag, Ns device doubles). */
+int qsx_csr_1_norm_diag(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
+                        int64_t Ns, double shift_re, double shift_im, const double* xdiag, double xs_re, double xs_im,
+                        double* colsum, double* out, void* stream);
 
 /* Replaces _wrapper_expm_multiply(indptr, indices, data, s, m_star, tol, mu,
  * a, v, work) and _wrapper_expm_multiply_batch (EMP.html:347-372): v <-
@@ -233,6 +239,62 @@ int qsx_hv_batch_apply(qsx_hv_batch* b, const void* x, void* y, double a_re, dou
 int qsx_expm_multiply_batch(qsx_hv_batch* b, const int32_t* s, const int32_t* m, const double* tol,
                             const double* mu, double a_re, double a_im, void* v, void* work, int use_graph,
                             void* stream);
+
+/* Extra diagonal over the stacked rows of a batch: products use
+ * H + (re + i im) diag(xdiag).  For time-dependent diagonal terms. */
+int qsx_hv_batch_set_diag(qsx_hv_batch* b, const double* xdiag, double re, double im);
+
+/* ---- time stepping with learned partitions ------------------------------------ */
+
+/* Device record of one Taylor step of one sector: a, mu, factor of the extra
+ * diagonal, tolerance, partition (s, m).  qsx_expm_record_bytes() gives its
+ * size; field order: double2 a, double2 mu, double2 dscale, double tol,
+ * int s, int m, int pad[2], then padding to a multiple of 16 bytes (80). */
+int qsx_expm_record_bytes(void);
+
+/* nsteps consecutive steps v <- e^{a_t (H + dscale_t D)} v, every parameter
+ * read from the device records es[t*nsec + k]; launch bounds s_max, m_max;
+ * fail[t*nsec + k] set when a stage of step t ran all its m terms without
+ * passing the truncation test (scipy _expm_multiply.py:298-300).  No
+ * counterpart in the reference (whose callers loop on the host over
+ * expm_multiply_parallel(H(t), a).dot); use_graph captures all steps once. */
+int qsx_evolve_steps(qsx_hv_batch* b, int nsteps, void* es, int s_max, int m_max, void* v, void* work, int* fail,
+                     int use_graph, void* stream);
+
+/* Single-matrix form: nsteps steps v <- e^{a_t (A + dscale_t diag(xdiag))} v
+ * with A a device CSR (as qsx_expm_multiply_csr) and the records es[t]
+ * (ExpmSec layout above); the same tail check into fail[t].  ctl_work:
+ * qsx_evolve_csr_ctl_bytes(s_max, m_max) bytes of device scratch owned by the
+ * caller.  use_graph captures the whole sequence once per set of buffers
+ * (qsx_graph_cache_clear drops the captures). */
+int64_t qsx_evolve_csr_ctl_bytes(int s_max, int m_max);
+int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
+                         int64_t Ns, int64_t nnz, const double* xdiag, int nsteps, const void* es, int s_max,
+                         int m_max, void* v, void* work, int vec_dtype, int64_t nvec, int* fail, void* ctl_work,
+                         int64_t ctl_bytes, int group, int use_graph, void* stream);
+void qsx_graph_cache_clear(void);
+
+/* ---- partition predictor (MLP 4 -> 64 -> 64 -> 2, tanh, FP64) ------------------ */
+
+/* Replaces the per-step 1-norm estimate and _fragment_3_1 choice of
+ * ExpmMultiplyParallel._calculate_partition (EMP.html:377-385; scipy
+ * _expm_multiply.py:503-558) for time-dependent H(t).  params: W1[64][4],
+ * b1[64], W2[64][64], b2[64], W3[2][64], b3[2]; norm[12] = input shift[4],
+ * input scale[4], output shift[2], output scale[2].  Inputs per sample:
+ * (t, dt, drive phase, ||H(t)||_1 bound); outputs (m, s). */
+int qsx_mlp_param_count(void);
+int qsx_mlp_forward(const double* params, const double* norm, const double* X, int64_t n, double* Y, void* stream);
+/* Gradient of L = 1/(2n) sum |y_hat - t_hat|^2 (normalised outputs); work:
+ * qsx_mlp_grad_work() device doubles; loss may be null. */
+int64_t qsx_mlp_grad_work(void);
+int qsx_mlp_grad(const double* params, const double* norm, const double* X, const double* T, int64_t n, double* grad,
+                 double* loss, double* work, int64_t work_elems, void* stream);
+int qsx_mlp_adam(double* params, const double* grad, double* m, double* v, int64_t step, double lr, double beta1,
+                 double beta2, double eps, void* stream);
+/* (m, s) of n steps into their records: m = ceil(y_m) + m_margin, s =
+ * round(y_s), clamped to [1, m_cap] x [1, s_cap]. */
+int qsx_mlp_partition(const double* params, const double* norm, const double* X, int64_t n, void* es, int m_margin,
+                      int m_cap, int s_cap, void* stream);
 
 #ifdef __cplusplus
 }

@@ -68,6 +68,7 @@ SIGNATURES = {
                               _vp]),
     "qsx_csr_trace": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _vp, _vp]),
     "qsx_csr_1_norm": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _dbl, _dbl, _vp, _vp, _vp]),
+    "qsx_csr_1_norm_diag": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _dbl, _dbl, _vp, _dbl, _dbl, _vp, _vp, _vp]),
     "qsx_expm_multiply_csr": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _i32, _i32, _dbl, _dbl, _dbl, _dbl,
                                      _dbl, _vp, _vp, _i32, _i64, _i32, _vp]),
     "qsx_expm_multiply_mf": (_i32, [_vp, _i64, _i32, _i32, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _i32, _i64,
@@ -87,6 +88,19 @@ SIGNATURES = {
     "qsx_hv_batch_set_schedule": (_i32, [_vp, _i32, _i32]),
     "qsx_hv_batch_apply": (_i32, [_vp, _vp, _vp, _dbl, _dbl, _i32, _vp]),
     "qsx_expm_multiply_batch": (_i32, [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _i32, _vp]),
+    "qsx_hv_batch_set_diag": (_i32, [_vp, _vp, _dbl, _dbl]),
+    "qsx_expm_record_bytes": (_i32, []),
+    "qsx_evolve_steps": (_i32, [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
+    "qsx_evolve_csr_ctl_bytes": (_i64, [_i32, _i32]),
+    "qsx_evolve_csr_steps": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _i32,
+                                    _i64, _vp, _vp, _i64, _i32, _i32, _vp]),
+    "qsx_graph_cache_clear": (None, []),
+    "qsx_mlp_param_count": (_i32, []),
+    "qsx_mlp_forward": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp]),
+    "qsx_mlp_grad_work": (_i64, []),
+    "qsx_mlp_grad": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _i64, _vp]),
+    "qsx_mlp_adam": (_i32, [_vp, _vp, _vp, _vp, _i64, _dbl, _dbl, _dbl, _dbl, _vp]),
+    "qsx_mlp_partition": (_i32, [_vp, _vp, _vp, _i64, _vp, _i32, _i32, _i32, _vp]),
 }
 
 _lib = None

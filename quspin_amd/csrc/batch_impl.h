@@ -23,16 +23,6 @@ struct WaveChunk {
   int64_t begin;  // first sector-local row
 };
 
-// Per-sector Taylor-loop parameters (device array, one per sector).
-struct ExpmSec {
-  double2 a;      // e^{aA}
-  double2 mu;     // tr(A)/N
-  double2 eta;    // e^{a mu / s}
-  double tol;
-  int s, m;       // partition (s stages of at most m terms)
-  int pad[2];
-};
-
 // One sector's CSR matrix (the CSR form of a batch).
 struct CsrSec {
   const void* indptr;       // int32 or int64 (idx64)
@@ -57,6 +47,9 @@ struct BatchArgs {
   void* y;
   double2 alpha;            // plain apply: y = alpha H x (+ y if beta)
   int beta;
+  const double* xdiag;      // optional extra diagonal over the stacked rows: H + xdscale * diag(xdiag)
+  double2 xdscale;          // its factor in a plain apply (the expm takes ExpmSec::dscale)
+  int* fail;                // tail check: per sector, set when a stage ended unconverged
   int stage, term;          // EPI: Taylor stage i and term j of this launch
   void* F;
   const ExpmSec* es;
@@ -159,8 +152,9 @@ template <> __device__ __forceinline__ double2 to_d2<double2>(double2 v) { retur
 // apply to the sector (outside its partition, finished, or truncated now);
 // otherwise the term's coefficient a/(s (j+1)) and shift -coef*mu.
 __device__ __forceinline__ bool taylor_term_setup(const BatchArgs& B, int k, int lane, double2& coef_s,
-                                                  double2& shift) {
+                                                  double2& shift, double2& dsc) {
   const ExpmSec E = B.es[k];
+  dsc = E.dscale;
   const int i = B.stage, j = B.term;
   if (i >= E.s || j >= E.m) return false;
   if (B.done[k]) return false;
@@ -236,10 +230,11 @@ __global__ void __launch_bounds__(256) k_hv_wave(BatchArgs B) {
   const bool col_ok = lane < B.ncol;
   const int lc = col_ok ? lane : 0;
 
-  double2 coef_s = B.alpha, shift = make_double2(0.0, 0.0);
+  double2 coef_s = B.alpha, shift = make_double2(0.0, 0.0), dsc = B.xdscale;
   if constexpr (EPI) {
-    if (!taylor_term_setup(B, k, lane, coef_s, shift)) return;
+    if (!taylor_term_setup(B, k, lane, coef_s, shift, dsc)) return;
   }
+  const double* __restrict__ xdiag = B.xdiag;
 
   const int64_t base = uniform64(B.row_off[k]);
   const VT* __restrict__ X = (const VT*)B.x + B.col0;
@@ -295,6 +290,12 @@ __global__ void __launch_bounds__(256) k_hv_wave(BatchArgs B) {
       }
       wave_lds_sync();
     }
+    if (xdiag) {
+      const double d = xdiag[row];
+      const double2 w = make_double2(dsc.x * d, dsc.y * d);
+      if constexpr (CPLX) acc = cadd(acc, cmul(w, xr));
+      else acc.x += w.x * xr.x;
+    }
     row_out<VT, EPI>(B, row, lane, col_ok, acc, xr, coef_s, shift, Y, FF, maxB, maxF);
   }
   if constexpr (EPI) {
@@ -340,10 +341,11 @@ __global__ void __launch_bounds__(256) k_csr_wave(BatchArgs B) {
   constexpr bool CPLX = VTr<VT>::cplx;
   constexpr int GB = sizeof(VT) >= 16 ? 8 : 16;
 
-  double2 coef_s = B.alpha, shift = make_double2(0.0, 0.0);
+  double2 coef_s = B.alpha, shift = make_double2(0.0, 0.0), dsc = B.xdscale;
   if constexpr (EPI) {
-    if (!taylor_term_setup(B, k, lane, coef_s, shift)) return;
+    if (!taylor_term_setup(B, k, lane, coef_s, shift, dsc)) return;
   }
+  const double* __restrict__ xdiag = B.xdiag;
   const int64_t base = uniform64(B.row_off[k]);
   const VT* __restrict__ X = (const VT*)B.x + B.col0;
   VT* __restrict__ Y = (VT*)B.y + B.col0;
@@ -402,6 +404,12 @@ __global__ void __launch_bounds__(256) k_csr_wave(BatchArgs B) {
         }
       }
     }
+    if (xdiag) {
+      const double d = xdiag[row];
+      const double2 w = make_double2(dsc.x * d, dsc.y * d);
+      if constexpr (CPLX) acc = cadd(acc, cmul(w, xr));
+      else acc.x += w.x * xr.x;
+    }
     row_out<VT, EPI>(B, row, lane, col_ok, acc, xr, coef_s, shift, Y, FF, maxB, maxF);
   }
   if constexpr (EPI) {
@@ -424,6 +432,7 @@ __global__ void __launch_bounds__(256) k_batch_stage(BatchArgs B) {
   const int lane = lane_id();
   const int i = B.stage;
   if (SCALE && i >= E.s) return;
+  const double2 eta = expm_eta(E);
   const bool col_ok = lane < B.ncol;
   VT* FF = (VT*)B.F + B.col0;
   VT* Bo = (VT*)B.y + B.col0;
@@ -435,8 +444,8 @@ __global__ void __launch_bounds__(256) k_batch_stage(BatchArgs B) {
     if (col_ok) {
       double2 v = VTr<VT>::ld(FF + row * B.ld + lane);
       if constexpr (SCALE) {
-        if constexpr (VTr<VT>::cplx) v = cmul(E.eta, v);
-        else v.x *= E.eta.x;
+        if constexpr (VTr<VT>::cplx) v = cmul(eta, v);
+        else v.x *= eta.x;
         VTr<VT>::st(FF + row * B.ld + lane, v);
         v = VTr<VT>::ld(FF + row * B.ld + lane);
       }

@@ -7,7 +7,9 @@
 // Al-Mohy & Higham Alg. 3.2 as in scipy ``_expm_multiply_simple_core``
 // (scipy/sparse/linalg/_expm_multiply.py:280-303).
 #include <cstring>
+#include <map>
 #include <string>
+#include <tuple>
 
 #include "sparse_impl.h"
 
@@ -345,17 +347,21 @@ __global__ void __launch_bounds__(256) k_csr_trace(const void* indptr, const voi
 
 template <typename MT>
 __global__ void __launch_bounds__(256) k_csr_colabs(const void* indptr, const void* indices, const MT* __restrict__ data,
-                                                    int idx64, int64_t Ns, double2 shift, double* __restrict__ colsum) {
+                                                    int idx64, int64_t Ns, double2 shift, const double* __restrict__ xdiag,
+                                                    double2 xs, double* __restrict__ colsum) {
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < Ns; r += (int64_t)gridDim.x * blockDim.x) {
     const int64_t k0 = ld_idx(indptr, idx64, r), k1 = ld_idx(indptr, idx64, r + 1);
+    // diagonal addition of this row: shift + xs * xdiag[r]
+    double2 dadd = shift;
+    if (xdiag) dadd = cadd(dadd, make_double2(xs.x * xdiag[r], xs.y * xdiag[r]));
     bool has_diag = false;
     for (int64_t k = k0; k < k1; ++k) {
       const int64_t c = ld_idx(indices, idx64, k);
       double2 v = MTr<MT>::ld(data + k);
-      if (c == r) { v = cadd(v, shift); has_diag = true; }
+      if (c == r) { v = cadd(v, dadd); has_diag = true; }
       atomicAdd(colsum + c, sqrt(v.x * v.x + v.y * v.y));
     }
-    if (!has_diag) atomicAdd(colsum + r, sqrt(shift.x * shift.x + shift.y * shift.y));
+    if (!has_diag) atomicAdd(colsum + r, sqrt(dadd.x * dadd.x + dadd.y * dadd.y));
   }
 }
 
@@ -394,9 +400,22 @@ extern "C" int qsx_csr_trace(const void* indptr, const void* indices, const void
 }
 
 // out (device) = max_j sum_i |A_ij + shift delta_ij| ; colsum: Ns device doubles (scratch)
+extern "C" int qsx_csr_1_norm_diag(const void* indptr, const void* indices, const void* data, int idx64,
+                                   int mat_dtype, int64_t Ns, double shift_re, double shift_im, const double* xdiag,
+                                   double xs_re, double xs_im, double* colsum, double* out, void* stream);
+
 extern "C" int qsx_csr_1_norm(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
                               int64_t Ns, double shift_re, double shift_im, double* colsum, double* out,
                               void* stream) {
+  return qsx_csr_1_norm_diag(indptr, indices, data, idx64, mat_dtype, Ns, shift_re, shift_im, nullptr, 0.0, 0.0,
+                             colsum, out, stream);
+}
+
+// ||A + shift I + xs diag(xdiag)||_1 (xdiag: Ns device doubles or null): the
+// 1-norm of a time-dependent H(t) = A + f(t) D whose dynamic part is diagonal.
+extern "C" int qsx_csr_1_norm_diag(const void* indptr, const void* indices, const void* data, int idx64,
+                                   int mat_dtype, int64_t Ns, double shift_re, double shift_im, const double* xdiag,
+                                   double xs_re, double xs_im, double* colsum, double* out, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   RedScratch R;
   int rc = get_red_scratch(R, st);
@@ -404,7 +423,7 @@ extern "C" int qsx_csr_1_norm(const void* indptr, const void* indices, const voi
   hipLaunchKernelGGL(k_fill_f64, dim3(red_grid(Ns)), dim3(256), 0, st, colsum, Ns, 0.0);
   QSX_MT_SWITCH(mat_dtype, hipLaunchKernelGGL(k_csr_colabs<MT>, dim3(red_grid(Ns) * 4), dim3(256), 0, st, indptr,
                                               indices, (const MT*)data, idx64, Ns, make_double2(shift_re, shift_im),
-                                              colsum));
+                                              xdiag, make_double2(xs_re, xs_im), colsum));
   hipLaunchKernelGGL(k_max_reduce, dim3(red_grid(Ns)), dim3(256), 0, st, colsum, Ns, R.partials, R.counter, out);
   QSX_LAUNCH_CHECK();
   return QSX_OK;
@@ -652,5 +671,163 @@ extern "C" int qsx_copy(const void* src, void* dst, int64_t bytes, void* stream)
   const int64_t n = bytes / 16;
   hipLaunchKernelGGL(k_copy16, dim3(2048), dim3(256), 0, (hipStream_t)stream, (const uint4*)src, (uint4*)dst, n);
   QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Time steps with device records: for t = 0..nsteps-1, v <- e^{a_t (A +
+// dscale_t D)} v, partition / a / mu / tol / dscale read from es[t]
+// ---------------------------------------------------------------------------
+// F *= eta(record) ; B = F ; ||F|| -> normB0 of the next stage ; done = 0 (stage i of the record)
+template <typename VT>
+__global__ void __launch_bounds__(256) k_expm_scale_rec(VT* __restrict__ F, VT* __restrict__ B, int64_t n,
+                                                        int64_t nvec, const ExpmSec* rec, int stage,
+                                                        double* __restrict__ normB0, ExpmCtl* ctl) {
+  const ExpmSec E = *rec;
+  if (stage >= E.s) return;
+  const double2 eta = expm_eta(E);
+  double mx = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double rs = 0.0;
+    for (int64_t j = 0; j < nvec; ++j) {
+      double2 v = VTr<VT>::ld(F + r * nvec + j);
+      if constexpr (VTr<VT>::cplx) v = cmul(eta, v); else v.x *= eta.x;
+      VTr<VT>::st(F + r * nvec + j, v);
+      VTr<VT>::st(B + r * nvec + j, v);
+      const double2 w = VTr<VT>::ld(F + r * nvec + j);
+      rs += VTr<VT>::cplx ? sqrt(w.x * w.x + w.y * w.y) : fabs(w.x);
+    }
+    mx = fmax(mx, rs);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = 0;
+  if (stage + 1 >= E.s) return;
+  __shared__ double sm[256];
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomic_max_nonneg(normB0, sm[0]);
+}
+
+// A stage that ran all m terms of its record without passing the truncation
+// test flags the step.
+__global__ void k_expm_tail(const ExpmSec* rec, int stage, const ExpmCtl* ctl, const double* nb, const double* nf,
+                            int* fail) {
+  const ExpmSec E = *rec;
+  if (stage >= E.s || E.m < 1 || ctl->done) return;
+  if (!(nb[E.m - 1] + nb[E.m] <= E.tol * nf[E.m])) *fail = 1;
+}
+
+// zero the norm slots and the done flag (graph-safe: no memset nodes)
+__global__ void k_expm_reset(ExpmCtl* ctl, double* slots, int64_t nslots) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += (int64_t)gridDim.x * blockDim.x)
+    slots[i] = 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctl->done = 0;
+}
+
+extern "C" int64_t qsx_evolve_csr_ctl_bytes(int s_max, int m_max) {
+  return (int64_t)sizeof(ExpmCtl) + (int64_t)sizeof(double) * 2 * (int64_t)s_max * (m_max + 1);
+}
+
+namespace {
+using StepKey = std::tuple<const void*, const void*, const void*, const void*, const void*, const void*, const void*,
+                           const void*, const void*, int64_t, int64_t, int, int, int, int, int, int>;
+std::map<StepKey, hipGraphExec_t> g_step_graphs;
+hipStream_t g_cap_stream = nullptr;
+}  // namespace
+
+// Drop every captured step sequence (call before freeing buffers that a
+// capture refers to if their addresses may be reused with other shapes).
+extern "C" void qsx_graph_cache_clear() {
+  for (auto& kv : g_step_graphs) (void)hipGraphExecDestroy(kv.second);
+  g_step_graphs.clear();
+}
+
+extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* data, int idx64,
+                                    int mat_dtype, int64_t Ns, int64_t nnz, const double* xdiag, int nsteps,
+                                    const void* es, int s_max, int m_max, void* v, void* work, int vec_dtype,
+                                    int64_t nvec, int* fail, void* ctl_work, int64_t ctl_bytes, int group,
+                                    int use_graph, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (Ns < 0 || nsteps < 1 || !es || s_max < 1 || m_max < 1 || !v || !work || !fail || !ctl_work || nvec < 1 ||
+      ctl_bytes < qsx_evolve_csr_ctl_bytes(s_max, m_max)) {
+    qsx_set_error("qsx_evolve_csr_steps: bad arguments");
+    return QSX_ERR_ARG;
+  }
+  if (Ns == 0) return QSX_OK;
+  ExpmCtl* ctl = (ExpmCtl*)ctl_work;
+  double* slots = (double*)((char*)ctl_work + sizeof(ExpmCtl));
+  const int64_t stride = (int64_t)(m_max + 1);
+  const int64_t nsl = 2 * (int64_t)s_max * stride;
+  double* normB = slots;
+  double* normF = slots + (int64_t)s_max * stride;
+  size_t esz = (vec_dtype == QSX_F32) ? 4 : (vec_dtype == QSX_F64 || vec_dtype == QSX_C64) ? 8 : 16;
+  void* Bb[2] = {work, (char*)work + esz * Ns * nvec};
+  const int G = group > 0 ? group : auto_group(Ns, nnz);
+  const unsigned rg = (unsigned)red_grid(Ns) * 4;
+  const ExpmSec* E = (const ExpmSec*)es;
+
+  auto record = [&](hipStream_t q) -> int {
+    for (int t = 0; t < nsteps; ++t) {
+      const ExpmSec* R = E + t;
+      hipLaunchKernelGGL(k_expm_reset, dim3(16), dim3(256), 0, q, ctl, slots, nsl);
+      QSX_LAUNCH_CHECK();
+      QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_start<VT>, dim3(rg), dim3(256), 0, q, (const VT*)v, (VT*)Bb[0],
+                                                  Ns, nvec, normB));
+      QSX_LAUNCH_CHECK();
+      for (int i = 0; i < s_max; ++i) {
+        for (int j = 0; j < m_max; ++j) {
+          CsrArgs A;
+          std::memset(&A, 0, sizeof(A));
+          A.indptr = indptr; A.indices = indices; A.data = data; A.idx64 = idx64; A.Ns = Ns; A.nvec = nvec;
+          A.x = Bb[j & 1]; A.y = Bb[(j + 1) & 1];
+          A.F = v;
+          A.ctl = ctl;
+          A.rec = R;
+          A.stage = i;
+          A.term = j;
+          A.xdiag = xdiag;
+          A.normB = normB + i * stride + j + 1;
+          A.normF = normF + i * stride + j + 1;
+          if (j >= 1) {
+            A.chk_c1 = normB + i * stride + j - 1;
+            A.chk_c2 = normB + i * stride + j;
+            A.chk_F = normF + i * stride + j;
+          }
+          int rc = launch_csr<true>(A, mat_dtype, vec_dtype, G, q);
+          if (rc) return rc;
+        }
+        hipLaunchKernelGGL(k_expm_tail, dim3(1), dim3(1), 0, q, R, i, ctl, normB + i * stride, normF + i * stride,
+                           fail + t);
+        QSX_LAUNCH_CHECK();
+        double* nb0 = (i + 1 < s_max) ? normB + (i + 1) * stride : normB;
+        QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_scale_rec<VT>, dim3(rg), dim3(256), 0, q, (VT*)v,
+                                                    (VT*)Bb[0], Ns, nvec, R, i, nb0, ctl));
+        QSX_LAUNCH_CHECK();
+      }
+    }
+    return QSX_OK;
+  };
+  if (!use_graph) return record(st);
+  const StepKey key{indptr, indices, data, xdiag, es, v, work, fail, ctl_work, Ns, nvec, nsteps, s_max, m_max, G,
+                    mat_dtype, vec_dtype};
+  auto it = g_step_graphs.find(key);
+  if (it == g_step_graphs.end()) {
+    if (!g_cap_stream) QSX_CHECK_HIP(hipStreamCreateWithFlags(&g_cap_stream, hipStreamNonBlocking));
+    QSX_CHECK_HIP(hipStreamBeginCapture(g_cap_stream, hipStreamCaptureModeThreadLocal));
+    int rc = record(g_cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(g_cap_stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) { qsx_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { qsx_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
+    it = g_step_graphs.emplace(key, ex).first;
+  }
+  QSX_CHECK_HIP(hipGraphLaunch(it->second, st));
   return QSX_OK;
 }

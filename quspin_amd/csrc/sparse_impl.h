@@ -24,6 +24,24 @@ struct ExpmCtl {
   // normB[i*(m+1)+j], normF[i*(m+1)+j] follow in device memory
 };
 
+// Per-step (and per-sector) Taylor-loop record, filled on the host or by the
+// partition predictor on the device.
+struct ExpmSec {
+  double2 a;       // e^{aA}
+  double2 mu;      // tr(A)/N
+  double2 dscale;  // factor of the extra diagonal (time-dependent term), if any
+  double tol;
+  int s, m;        // partition (s stages of at most m terms)
+  int pad[2];
+};
+
+// e^{a mu / s}
+__device__ __forceinline__ double2 expm_eta(const ExpmSec& E) {
+  const double er = (E.a.x * E.mu.x - E.a.y * E.mu.y) / E.s, ei = (E.a.x * E.mu.y + E.a.y * E.mu.x) / E.s;
+  const double mag = exp(er);
+  return make_double2(mag * cos(ei), mag * sin(ei));
+}
+
 struct CsrArgs {
   const void* indptr;   // int32 or int64
   const void* indices;  // int32 or int64
@@ -44,6 +62,12 @@ struct CsrArgs {
   const double* chk_c1; // previous test inputs: c1, c2, ||F|| (null: no test)
   const double* chk_c2;
   const double* chk_F;
+  // optional extra diagonal: A + xdscale * diag(xdiag) (xdscale from rec when set)
+  const double* xdiag;
+  double2 xdscale;
+  // device step record: partition, a, mu, tol, dscale read here (launch = term j of stage i)
+  const ExpmSec* rec;
+  int stage, term;
 };
 
 __device__ __forceinline__ int64_t ld_idx(const void* p, int idx64, int64_t i) {
@@ -53,12 +77,23 @@ __device__ __forceinline__ int64_t ld_idx(const void* p, int idx64, int64_t i) {
 // G lanes per row.  EPI: fused expm epilogue.
 template <typename MT, typename VT, int G, bool EPI>
 __global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
+  double2 a_t = A.a, mu_t = A.mu, xs = A.xdscale;
   if constexpr (EPI) {
+    double tol = A.ctl->tol;
+    if (A.rec) {
+      const ExpmSec E = *A.rec;
+      if (A.stage >= E.s || A.term >= E.m) return;
+      const double cf = 1.0 / ((double)E.s * (double)(A.term + 1));
+      a_t = make_double2(E.a.x * cf, E.a.y * cf);
+      mu_t = E.mu;
+      xs = E.dscale;
+      tol = E.tol;
+    }
     if (A.ctl->done) return;
     if (A.chk_c1) {
       // truncation test of the previous term (SCIPY:298-300)
       const double c1 = *A.chk_c1, c2 = *A.chk_c2, nf = *A.chk_F;
-      if (c1 + c2 <= A.ctl->tol * nf) {
+      if (c1 + c2 <= tol * nf) {
         if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl->done = 1;
         return;
       }
@@ -88,12 +123,18 @@ __global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
         const int64_t o = row * A.nvec + j;
         double2 acc = make_double2(sr, si);
         const double2 xr = VTr<VT>::ld(x + o);
+        double2 mu_r = mu_t;
+        if (A.xdiag) {
+          // the extra diagonal enters as a row-dependent shift
+          const double d = A.xdiag[row];
+          mu_r = make_double2(mu_r.x - xs.x * d, mu_r.y - xs.y * d);
+        }
         double2 res;
         if constexpr (CPLX) {
-          acc = cadd(acc, make_double2(-(A.mu.x * xr.x - A.mu.y * xr.y), -(A.mu.x * xr.y + A.mu.y * xr.x)));
-          res = cmul(A.a, acc);
+          acc = cadd(acc, make_double2(-(mu_r.x * xr.x - mu_r.y * xr.y), -(mu_r.x * xr.y + mu_r.y * xr.x)));
+          res = cmul(a_t, acc);
         } else {
-          res = make_double2(A.a.x * (acc.x - A.mu.x * xr.x), 0.0);
+          res = make_double2(a_t.x * (acc.x - mu_r.x * xr.x), 0.0);
         }
         if (A.beta) res = cadd(res, VTr<VT>::ld(y + o));
         VTr<VT>::st(y + o, res);

@@ -1,0 +1,168 @@
+"""Partition predictor (device MLP kernels) and time stepping under
+H(t) = H0 + f(t) D with device step records: gradients against the oracle,
+exact-partition stepping against dense exponentials, and what the trained
+network gains over the exact per-step selection."""
+
+import time
+
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from oracle import qsx_oracle as O
+from tests.models import hubbard_static
+
+pytestmark = pytest.mark.gpu
+
+
+def _normalised(X, Y, nrm):
+    xs, xsc, ys, ysc = nrm[:4], nrm[4:8], nrm[8:10], nrm[10:12]
+    return (X - xs) / xsc, (Y - ys) / ysc
+
+
+def test_mlp_forward_and_gradient_match_oracle():
+    from quspin_amd import _lib as L
+    from quspin_amd.tools.learned_step import PartitionMLP
+    rng = np.random.default_rng(0)
+    net = PartitionMLP(seed=3)
+    X = rng.normal(size=(37, 4)) * [3.0, 0.05, 2.0, 40.0] + [5.0, 0.02, 3.0, 60.0]
+    Y = rng.uniform(1, 30, size=(37, 2))
+    net.set_normalisation(X, Y)
+    nrm = L.to_host(net.norm)
+    params = net.host_params()
+    Xn, Yn = _normalised(X, Y, nrm)
+    y, cache = O.mlp_forward(params, Xn)
+    got = net.forward(X)
+    np.testing.assert_allclose(got, y * nrm[10:12] + nrm[8:10], rtol=1e-12, atol=1e-12)
+    g, loss = net.grad(X, Y)
+    dy = (y - Yn) / X.shape[0]
+    ref = np.concatenate([a.ravel() for a in O.mlp_backward(params, cache, dy)])
+    np.testing.assert_allclose(L.to_host(g), ref, rtol=1e-10, atol=1e-13)
+    assert abs(L.to_host(loss)[0] - 0.5 * np.sum((y - Yn) ** 2) / X.shape[0]) < 1e-12
+
+
+def test_mlp_adam_step_matches_numpy_and_training_converges():
+    from quspin_amd import _lib as L
+    from quspin_amd.tools.learned_step import PartitionMLP
+    rng = np.random.default_rng(1)
+    X = rng.uniform(0, 1, size=(200, 4))
+    Y = np.stack([10 + 5 * np.sin(3 * X[:, 0]) + 4 * X[:, 3], 1 + (X[:, 1] > 0.5)], 1)
+    net = PartitionMLP(seed=1)
+    p0 = L.to_host(net.params).copy()
+    net.fit(X, Y, epochs=1, lr=1e-2)
+    net2 = PartitionMLP(seed=1)
+    net2.set_normalisation(X, Y)
+    g0 = L.to_host(net2.grad(X, Y)[0])
+    m = 0.1 * g0
+    v = 0.001 * g0 ** 2
+    expect = p0 - 1e-2 * (m / 0.1) / (np.sqrt(v / 0.001) + 1e-8)
+    np.testing.assert_allclose(L.to_host(net.params), expect, rtol=1e-12, atol=1e-14)
+    hist = PartitionMLP(seed=2).fit(X, Y, epochs=1500, lr=3e-3)
+    assert hist[-1][1] < 0.05 * hist[0][1]
+
+
+def _hubbard(L_, U_amp=(4.0, 2.0, 3.0)):
+    from quspin_amd.basis import spinful_fermion_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    B = spinful_fermion_basis_general(L_, Nf=(L_ // 2, L_ // 2))
+    hop = quantum_LinearOperator(hubbard_static(L_, U=0.0), basis=B, dtype=np.complex128, check_symm=False,
+                                 check_herm=False, check_pcon=False)
+    D = quantum_LinearOperator([["n|n", [[1.0, i, i] for i in range(L_)]]], basis=B, dtype=np.float64,
+                               check_symm=False, check_herm=False, check_pcon=False)
+    c0, c1, w = U_amp
+    f = lambda t: c0 + c1 * np.cos(w * t)  # noqa: E731
+    phase = lambda t: (w * t) % (2 * np.pi)  # noqa: E731
+    return B, hop, D, f, phase
+
+
+def test_exact_stepping_matches_dense_midpoint_exponentials():
+    from quspin_amd import _lib as L
+    from quspin_amd.tools.learned_step import DrivenEvolution
+    B, hop, D, f, phase = _hubbard(6)
+    dt = 0.05
+    ev = DrivenEvolution(hop, D, f, dt, dtype=np.complex128, phase=phase)
+    Hd = ev.csr.to_scipy().toarray()
+    Dd = np.diag(L.to_host(ev.D))
+    rng = np.random.default_rng(5)
+    psi0 = rng.normal(size=B.Ns) + 1j * rng.normal(size=B.Ns)
+    psi0 /= np.linalg.norm(psi0)
+    psi = L.to_device(psi0.copy())
+    ev.run(psi, 0.3, 12, learned=False)
+    ref = psi0.copy()
+    for k in range(12):
+        t = 0.3 + (k + 0.5) * dt
+        ref = expm(-1j * dt * (Hd + f(t) * Dd)) @ ref
+    np.testing.assert_allclose(L.to_host(psi), ref, rtol=0, atol=1e-12)
+    assert ev.stats["exact_steps"] == 12 and ev.stats["norm_passes"] == 12
+
+
+def _trained(sizes=(6, 8, 10), dts=(0.01, 0.02, 0.05, 0.1), n_t=40, epochs=2500):
+    from quspin_amd.tools.learned_step import DrivenEvolution, PartitionMLP, training_set
+    evs = []
+    for L_ in sizes:
+        B, hop, D, f, phase = _hubbard(L_)
+        evs.append(DrivenEvolution(hop, D, f, 0.02, dtype=np.complex128, phase=phase))
+    X, Y = training_set(evs, t_max=4.0, dts=dts, n_t=n_t, seed=0)
+    net = PartitionMLP(seed=0)
+    net.fit(X, Y, epochs=epochs, lr=3e-3)
+    return net, X, Y
+
+
+def test_learned_evolution_equals_exact_and_skips_norm_passes():
+    """The trained network's partitions pass the tail check, give the exact
+    path's state, and remove every per-step norm pass and host round trip:
+    the learned run is faster than the exact selection on the same steps."""
+    from quspin_amd import _lib as L
+    from quspin_amd.tools.learned_step import DrivenEvolution
+    net, X, Y = _trained()
+    pred = net.forward(X)
+    assert np.mean(np.ceil(pred[:, 0]) + 2 >= Y[:, 0]) > 0.9
+    B, hop, D, f, phase = _hubbard(10)
+    dt, nsteps = 0.02, 60
+    rng = np.random.default_rng(9)
+    psi0 = rng.normal(size=B.Ns) + 1j * rng.normal(size=B.Ns)
+    psi0 /= np.linalg.norm(psi0)
+    ex = DrivenEvolution(hop, D, f, dt, dtype=np.complex128, phase=phase)
+    le = DrivenEvolution(hop, D, f, dt, dtype=np.complex128, phase=phase, predictor=net)
+    torch = L.torch()
+    # warm-up (graph capture, first launches)
+    le.run(L.to_device(psi0.copy()), 0.0, nsteps, learned=True, block=nsteps)
+    ex.run(L.to_device(psi0.copy()), 0.0, 2, learned=False)
+    le.stats = {k: 0 for k in le.stats}
+    ex.stats = {k: 0 for k in ex.stats}
+    pe = L.to_device(psi0.copy())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ex.run(pe, 0.0, nsteps, learned=False)
+    torch.cuda.synchronize()
+    t_exact = time.perf_counter() - t0
+    pl = L.to_device(psi0.copy())
+    t0 = time.perf_counter()
+    le.run(pl, 0.0, nsteps, learned=True, block=nsteps)
+    torch.cuda.synchronize()
+    t_learned = time.perf_counter() - t0
+    print(f"exact {1e3 * t_exact / nsteps:.3f} ms/step ({ex.stats}), learned {1e3 * t_learned / nsteps:.3f} ms/step "
+          f"({le.stats}), gain {t_exact / t_learned:.2f}x", flush=True)
+    np.testing.assert_allclose(L.to_host(pl), L.to_host(pe), rtol=0, atol=1e-11)
+    assert le.stats["learned_steps"] == nsteps and le.stats["norm_passes"] == 0
+    assert ex.stats["norm_passes"] == nsteps
+    assert t_learned < t_exact
+
+
+def test_tail_check_rejects_short_partitions():
+    """A predictor forced to too few terms fails the tail check; the block is
+    redone with the exact selection and the state stays exact."""
+    from quspin_amd import _lib as L
+    from quspin_amd.tools.learned_step import DrivenEvolution, PartitionMLP
+    B, hop, D, f, phase = _hubbard(6)
+    net = PartitionMLP(seed=0)
+    net.set_normalisation(np.zeros((2, 4)), np.array([[1.0, 1.0], [1.0, 1.0]]))  # predicts m ~ 1
+    bad = DrivenEvolution(hop, D, f, 0.1, dtype=np.complex128, phase=phase, predictor=net, m_margin=0, m_cap=2)
+    good = DrivenEvolution(hop, D, f, 0.1, dtype=np.complex128, phase=phase)
+    rng = np.random.default_rng(2)
+    psi0 = rng.normal(size=B.Ns) + 1j * rng.normal(size=B.Ns)
+    p1, p2 = L.to_device(psi0.copy()), L.to_device(psi0.copy())
+    bad.run(p1, 0.0, 10, learned=True, block=5)
+    good.run(p2, 0.0, 10, learned=False)
+    assert bad.stats["fallback_blocks"] == 2 and bad.stats["exact_steps"] == 10
+    np.testing.assert_allclose(L.to_host(p1), L.to_host(p2), rtol=0, atol=1e-12)

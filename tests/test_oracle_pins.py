@@ -192,3 +192,27 @@ def test_c_oracle_matches_numpy_oracle():
             v = np.random.default_rng(0).normal(size=len(b))
             for fused in (True, False):
                 np.testing.assert_allclose(C.hv_real(cg, cb, cn, terms, v, fused=fused), H @ v, atol=1e-12)
+
+
+def test_mlp_backward_finite_differences():
+    """Gradients of the oracle MLP against central differences of
+    sum(dy * y) (every parameter block)."""
+    rng = np.random.default_rng(4)
+    shapes = [(64, 4), (64,), (64, 64), (64,), (2, 64), (2,)]
+    params = [rng.normal(0, 0.3, size=s) for s in shapes]
+    X = rng.normal(size=(7, 4))
+    dy = rng.normal(size=(7, 2))
+    y, cache = O.mlp_forward(params, X)
+    grads = O.mlp_backward(params, cache, dy)
+    h = 1e-6
+    for bi, (P, G) in enumerate(zip(params, grads)):
+        flat = P.reshape(-1)
+        for idx in rng.choice(flat.size, size=min(6, flat.size), replace=False):
+            old = flat[idx]
+            flat[idx] = old + h
+            yp = O.mlp_forward(params, X)[0]
+            flat[idx] = old - h
+            ym = O.mlp_forward(params, X)[0]
+            flat[idx] = old
+            fd = np.sum(dy * (yp - ym)) / (2 * h)
+            assert abs(fd - G.reshape(-1)[idx]) < 1e-7 * max(1.0, abs(fd)), (bi, idx)
