@@ -50,6 +50,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (HBM counter calibration)")
+    ap.add_argument("--no-expm", action="store_true", help="skip the expm workloads (precision, batched, learned)")
+    ap.add_argument("--expm-only", default="", help="comma list of c3,c4,c5 to run (default all)")
+    ap.add_argument("--c3-steps", type=int, default=100)
+    ap.add_argument("--c5-steps", type=int, default=500)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU kernel "
@@ -144,6 +148,16 @@ def main():
 
     cfg = next(iter(op.config.values()), {})
     traffic = MEASURED_TRAFFIC.get((L, cfg.get("lookup"), cfg.get("schedule"), "float64"))
+    expm = {}
+    if not args.no_expm:
+        which = set(args.expm_only.split(",")) if args.expm_only else {"c3", "c4", "c5"}
+        if "c3" in which:
+            expm["precision_L28"] = run_precision(args.c3_steps)
+        if "c4" in which:
+            expm["batched_L26"] = run_batched()
+        if "c5" in which:
+            expm["learned_hubbard_L14"] = run_learned(args.c5_steps)
+
     line = {
         "metric": "H.v matvecs/s (symmetry-reduced matrix-free, fp64)",
         "value": round(value, 2),
@@ -164,9 +178,245 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
         "cpu_baseline": cpu,
         "lanczos": lz,
+        "expm": expm,
         "kernel_config": {str(k): v for k, v in op.config.items()},
     }
     print(json.dumps(line))
+
+
+def _ev_timer():
+    import torch
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    return st, e0, e1
+
+
+def run_precision(steps, L=28, dt=0.1):
+    """Both forms of the L=28 precision workload: matrix-free
+    (quantum_LinearOperator) and CSR (hamiltonian)."""
+    out = {"matrix_free": _run_precision(steps, "mf", L, dt)}
+    out["csr"] = _run_precision(steps, "csr", L, dt)
+    return out
+
+
+def _run_precision(steps, form, L=28, dt=0.1):
+    """XXZ chain L=28, Delta=1, h_i ~ U[-3, 3] (seed 0), Sz=0 without spatial
+    symmetry (40,116,600 states); Neel start; `steps` matrix-free expm steps
+    of dt in complex64 (FP64 accumulation inside every kernel) and in
+    complex128; state error and max |<Sz_i>| error after every step."""
+    import torch
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    from quspin_amd.operators._devop import vdot_dev
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    rng = np.random.default_rng(0)
+    h = rng.uniform(-3, 3, L)
+    static = chain_static(L) + [["z", [[float(h[i]), i] for i in range(L)]]]
+    B = spin_basis_general(L, Nup=L // 2, pauli=0)
+    kw = dict(basis=B, check_symm=False, check_herm=False, check_pcon=False)
+    Op = quantum_LinearOperator if form == "mf" else (lambda st, **k: hamiltonian(st, [], **k))
+    E = {np.complex64: expm_multiply_parallel(Op(static, dtype=np.float32, **kw), a=-1j * dt, dtype=np.complex64),
+         np.complex128: expm_multiply_parallel(Op(static, dtype=np.float64, **kw), a=-1j * dt, dtype=np.complex128)}
+    zops = [quantum_LinearOperator([["z", [[1.0, i]]]], dtype=np.float64, **kw)._qsx_operator() for i in range(L)]
+    i0 = B.index("10" * (L // 2))
+    psi = {}
+    for dt_ in E:
+        psi[dt_] = torch.zeros(B.Ns, dtype=torch.complex64 if dt_ == np.complex64 else torch.complex128, device="cuda")
+        psi[dt_][i0] = 1
+    work = {dt_: torch.empty(2 * B.Ns, dtype=psi[dt_].dtype, device="cuda") for dt_ in E}
+    tmp = torch.empty(B.Ns, dtype=torch.complex128, device="cuda")
+
+    def sz(p):
+        p = p if p.dtype == torch.complex128 else p.to(torch.complex128)
+        out = []
+        for zo in zops:
+            zo.apply(p, tmp)
+            out.append(vdot_dev(p, tmp).real)
+        return np.array(out)
+
+    ms = {dt_: 0.0 for dt_ in E}
+    errs, szerr = [], []
+    st, e0, e1 = _ev_timer()
+    for k in range(steps):
+        for dt_ in E:
+            e0.record(st)
+            psi[dt_] = E[dt_].dot(psi[dt_], work_array=work[dt_], overwrite_v=True)
+            e1.record(st)
+            torch.cuda.synchronize()
+            ms[dt_] += e0.elapsed_time(e1)
+        p32 = psi[np.complex64].to(torch.complex128)
+        p64 = psi[np.complex128]
+        d2 = vdot_dev(p32, p32).real + vdot_dev(p64, p64).real - 2 * vdot_dev(p32, p64).real
+        errs.append(float(np.sqrt(max(d2, 0.0))))
+        szerr.append(float(np.abs(sz(p32) - sz(p64)).max()))
+    sp32 = 1000.0 * steps / ms[np.complex64]
+    sp64 = 1000.0 * steps / ms[np.complex128]
+    return {"workload": f"XXZ L={L} Delta=1 h_i~U[-3,3] seed 0, Sz=0 ({B.Ns} states), Neel start, {steps} steps dt={dt}",
+            "path": ("matrix-free Taylor loop (quantum_LinearOperator)" if form == "mf" else
+                     "CSR Taylor loop (hamiltonian, real FP32/FP64 values)"),
+            "partition": [int(x) for x in E[np.complex64].partition],
+            "steps_per_s_fp32": round(sp32, 3), "steps_per_s_fp64": round(sp64, 3),
+            "speedup_fp32_vs_fp64": round(sp32 / sp64, 3),
+            "state_err_max": max(errs), "state_err_final": errs[-1], "state_err_every_10": errs[9::10],
+            "sz_err_max": max(szerr), "sz_err_every_10": szerr[9::10]}
+
+
+def run_batched(L=26, nv=64, t=2.0, plain_sectors=2):
+    """Heisenberg L=26, Sz=0, all 26 momentum sectors x 64 seeded states:
+    batched expm to t=2 (one launch per Taylor term for every sector, hipGraph
+    replay, autotuned form and schedule) in complex64 and complex128; the
+    plain FP64 path (one expm_multiply_parallel per sector) timed on
+    `plain_sectors` sectors and scaled to 26."""
+    import torch
+    from quspin_amd import _lib
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    from quspin_amd.tools.evolution import expm_multiply_parallel, expm_multiply_sectors
+    i = np.arange(L)
+    T = (i + 1) % L
+    static = chain_static(L)
+    bases = [spin_basis_general(L, Nup=L // 2, pauli=0, kblock=(T, k)) for k in range(L)]
+    kw = dict(check_symm=False, check_herm=False, check_pcon=False)
+    ops = {dt_: [quantum_LinearOperator(static, basis=b, dtype=dt_, **kw) for b in bases]
+           for dt_ in (np.complex64, np.complex128)}
+    rows = sum(b.Ns for b in bases)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    V = torch.randn(rows, nv, dtype=torch.complex128, device="cuda", generator=g)
+    out = {"workload": f"Heisenberg L={L}, Sz=0, k=0..{L - 1} ({rows} states, sectors "
+                       f"{min(b.Ns for b in bases)}..{max(b.Ns for b in bases)}) x {nv} vectors, expm to t={t}"}
+    F = {}
+    st, e0, e1 = _ev_timer()
+    for dt_, name in ((np.complex64, "fp32"), (np.complex128, "fp64")):
+        E = expm_multiply_sectors(ops[dt_], a=-1j * t, dtype=dt_, nvec=nv)
+        X = V.to(_lib.torch_dtype(dt_))
+        E.dot(X)
+        torch.cuda.synchronize()
+        e0.record(st)
+        F[name] = E.dot(X)
+        e1.record(st)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1)
+        # one batched H.V (the dominant kernel) for the roofline
+        Y = torch.empty_like(X)
+        E.batch.apply(X, Y)
+        torch.cuda.synchronize()
+        e0.record(st)
+        for _ in range(5):
+            E.batch.apply(X, Y)
+        e1.record(st)
+        torch.cuda.synchronize()
+        mv = e0.elapsed_time(e1) / 5
+        esz = np.dtype(dt_).itemsize
+        alg = rows * (4 + 1) + 2 * rows * nv * esz
+        out[name] = {"expm_s": round(ms / 1000.0, 4), "form": E.batch.form,
+                     "schedule": [E.batch.chunk_rows, E.batch.xcd_slices],
+                     "partitions": sorted({tuple(int(x) for x in p) for p in E.partitions}),
+                     "batched_hv_ms": round(mv, 4), "batched_hv_alg_GBps": round(alg / mv / 1e6, 1),
+                     "batched_hv_hbm_frac": round(alg / mv / 1e6 / HBM_PEAK_GBS, 4)}
+        del E, X, Y
+    f32 = F["fp32"].to(torch.complex128)
+    f64 = F["fp64"]
+    out["state_err_max_column"] = float(((f32 - f64).norm(dim=0) / V.norm(dim=0)).max().item())
+    del F, f32, f64
+    # plain FP64 GPU path: one expm_multiply_parallel per sector (64 vectors each)
+    tp = 0.0
+    off = 0
+    for k in range(plain_sectors):
+        Ek = expm_multiply_parallel(ops[np.complex128][k], a=-1j * t, dtype=np.complex128)
+        Vk = V[off:off + bases[k].Ns]
+        off += bases[k].Ns
+        torch.cuda.synchronize()
+        e0.record(st)
+        Ek.dot(Vk)
+        e1.record(st)
+        torch.cuda.synchronize()
+        tp += e0.elapsed_time(e1) / 1000.0
+    plain = tp * L / plain_sectors
+    out["plain_fp64_s_est"] = round(plain, 3)
+    out["plain_fp64_sample"] = f"{plain_sectors} of {L} sectors timed, scaled by {L}/{plain_sectors}"
+    out["speedup_fp32_batched_vs_plain_fp64"] = round(plain / out["fp32"]["expm_s"], 2)
+    return out
+
+
+def run_learned(steps, L=14, dt=0.02):
+    """1D spinful Fermi-Hubbard L=14 PBC, t_hop=1, U(t)=4+2cos(3t), (7,7)
+    (11,778,624 states): `steps` expm steps of dt in complex64 with the
+    partition predicted by the device MLP (trained here on reference
+    partitions of L=10 and L=12), against the exact per-step selection in
+    complex64 and the plain FP64 path (complex128, exact selection)."""
+    import time
+    import torch
+    from quspin_amd.basis import spinful_fermion_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    from quspin_amd.operators._devop import vdot_dev
+    from quspin_amd.tools.learned_step import DrivenEvolution, PartitionMLP, training_set
+    f = lambda t: 4.0 + 2.0 * np.cos(3.0 * t)  # noqa: E731
+    phase = lambda t: (3.0 * t) % (2 * np.pi)  # noqa: E731
+    kw = dict(check_symm=False, check_herm=False, check_pcon=False)
+
+    def hubbard(Lh, dtype):
+        Bh = spinful_fermion_basis_general(Lh, Nf=(Lh // 2, Lh // 2))
+        hop_r = [[-1.0, i, (i + 1) % Lh] for i in range(Lh)]
+        hop_l = [[+1.0, i, (i + 1) % Lh] for i in range(Lh)]
+        hop = quantum_LinearOperator([["+-|", hop_r], ["-+|", hop_l], ["|+-", hop_r], ["|-+", hop_l]], basis=Bh,
+                                     dtype=dtype, **kw)
+        D = quantum_LinearOperator([["n|n", [[1.0, i, i] for i in range(Lh)]]], basis=Bh, dtype=np.float64, **kw)
+        return Bh, hop, D
+
+    t0 = time.perf_counter()
+    evs = []
+    for Lh in (10, 12):
+        _, hop, D = hubbard(Lh, np.complex128)
+        evs.append(DrivenEvolution(hop, D, f, dt, dtype=np.complex128, phase=phase))
+    X, Y = training_set(evs, t_max=2 * np.pi / 3 * 2, dts=(0.005, 0.01, 0.02, 0.04, 0.08), n_t=40, seed=0)
+    t_data = time.perf_counter() - t0
+    net = PartitionMLP(seed=0)
+    t0 = time.perf_counter()
+    hist = net.fit(X, Y, epochs=3000, lr=3e-3)
+    torch.cuda.synchronize()
+    t_fit = time.perf_counter() - t0
+    del evs
+    B, hop32, D = hubbard(L, np.complex64)
+    _, hop64, _ = hubbard(L, np.complex128)
+    rng = np.random.default_rng(3)
+    i0 = int(rng.integers(B.Ns))
+    res = {}
+    # fp32: complex64 matrix, Taylor terms and SpMV with the state accumulated in complex128
+    runs = (("learned_fp32", np.complex64, hop32, net, True), ("exact_fp32", np.complex64, hop32, None, False),
+            ("exact_fp64", np.complex128, hop64, None, False))
+    psis = {}
+    for name, dt_, hop, pred, learned in runs:
+        ev = DrivenEvolution(hop, D, f, dt, dtype=dt_, phase=phase, predictor=pred, state_dtype=np.complex128)
+        psi = torch.zeros(B.Ns, dtype=torch.complex128, device="cuda")
+        psi[i0] = 1
+        ev.run(psi, 0.0, min(steps, 100), learned=learned, block=100)  # graph capture / first launches
+        psi.zero_()
+        psi[i0] = 1
+        ev.stats = {k: 0 for k in ev.stats}
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev.run(psi, 0.0, steps, learned=learned, block=100)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        res[name] = {"steps_per_s": round(steps / wall, 2), "ms_per_step": round(1000.0 * wall / steps, 4),
+                     "stats": dict(ev.stats)}
+        psis[name] = psi.to(torch.complex128)
+        del ev
+    p64 = psis["exact_fp64"]
+    for name in ("learned_fp32", "exact_fp32"):
+        p = psis[name]
+        d2 = vdot_dev(p, p).real + vdot_dev(p64, p64).real - 2 * vdot_dev(p, p64).real
+        res[name]["state_err_vs_fp64"] = float(np.sqrt(max(d2, 0.0)))
+    res["workload"] = (f"spinful Fermi-Hubbard L={L} PBC, t=1, U(t)=4+2cos(3t), Nf=({L // 2},{L // 2}) ({B.Ns} "
+                       f"states), {steps} steps dt={dt} from a seeded basis state")
+    res["training"] = {"samples": int(X.shape[0]), "sizes": [10, 12], "epochs": 3000,
+                       "loss_first_last": [hist[0][1], hist[-1][1]], "data_s": round(t_data, 2),
+                       "fit_s": round(t_fit, 2)}
+    res["speedup_learned_fp32_vs_exact_fp64"] = round(res["learned_fp32"]["steps_per_s"] /
+                                                      res["exact_fp64"]["steps_per_s"], 3)
+    res["gain_learned_vs_exact_fp32"] = round(res["learned_fp32"]["steps_per_s"] /
+                                              res["exact_fp32"]["steps_per_s"], 3)
+    return res
 
 
 if __name__ == "__main__":

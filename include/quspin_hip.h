@@ -265,13 +265,16 @@ int qsx_evolve_steps(qsx_hv_batch* b, int nsteps, void* es, int s_max, int m_max
  * with A a device CSR (as qsx_expm_multiply_csr) and the records es[t]
  * (ExpmSec layout above); the same tail check into fail[t].  ctl_work:
  * qsx_evolve_csr_ctl_bytes(s_max, m_max) bytes of device scratch owned by the
- * caller.  use_graph captures the whole sequence once per set of buffers
+ * caller.  vec_dtype is the dtype of the Taylor terms (work) and the SpMV;
+ * state_dtype that of v, the accumulated sum: the same, or its FP64
+ * counterpart (FP32 terms with FP64 accumulation), or -1 for the same.
+ * use_graph captures the whole sequence once per set of buffers
  * (qsx_graph_cache_clear drops the captures). */
 int64_t qsx_evolve_csr_ctl_bytes(int s_max, int m_max);
 int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
                          int64_t Ns, int64_t nnz, const double* xdiag, int nsteps, const void* es, int s_max,
-                         int m_max, void* v, void* work, int vec_dtype, int64_t nvec, int* fail, void* ctl_work,
-                         int64_t ctl_bytes, int group, int use_graph, void* stream);
+                         int m_max, void* v, void* work, int vec_dtype, int state_dtype, int64_t nvec, int* fail,
+                         void* ctl_work, int64_t ctl_bytes, int group, int use_graph, void* stream);
 void qsx_graph_cache_clear(void);
 
 /* ---- partition predictor (MLP 4 -> 64 -> 64 -> 2, tanh, FP64) ------------------ */

@@ -93,7 +93,7 @@ SIGNATURES = {
     "qsx_evolve_steps": (_i32, [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "qsx_evolve_csr_ctl_bytes": (_i64, [_i32, _i32]),
     "qsx_evolve_csr_steps": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _i32,
-                                    _i64, _vp, _vp, _i64, _i32, _i32, _vp]),
+                                    _i32, _i64, _vp, _vp, _i64, _i32, _i32, _vp]),
     "qsx_graph_cache_clear": (None, []),
     "qsx_mlp_param_count": (_i32, []),
     "qsx_mlp_forward": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp]),

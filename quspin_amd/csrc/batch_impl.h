@@ -197,10 +197,9 @@ __device__ __forceinline__ void row_out(const BatchArgs& B, int64_t row, int lan
     double rb = 0.0, rf = 0.0;
     if (col_ok) {
       VTr<VT>::st(Y + row * B.ld + lane, res);
-      const double2 rs = VTr<VT>::ld(Y + row * B.ld + lane);  // the stored (rounded) term enters F
-      double2 f = cadd(VTr<VT>::ld(FF + row * B.ld + lane), rs);
+      const double2 rs = VTr<VT>::rnd(res);  // the stored (rounded) term enters F
+      const double2 f = VTr<VT>::rnd(cadd(VTr<VT>::ld(FF + row * B.ld + lane), rs));
       VTr<VT>::st(FF + row * B.ld + lane, f);
-      f = VTr<VT>::ld(FF + row * B.ld + lane);
       rb = vabs<VT>(rs);
       rf = vabs<VT>(f);
     }
@@ -446,8 +445,8 @@ __global__ void __launch_bounds__(256) k_batch_stage(BatchArgs B) {
       if constexpr (SCALE) {
         if constexpr (VTr<VT>::cplx) v = cmul(eta, v);
         else v.x *= eta.x;
+        v = VTr<VT>::rnd(v);
         VTr<VT>::st(FF + row * B.ld + lane, v);
-        v = VTr<VT>::ld(FF + row * B.ld + lane);
       }
       VTr<VT>::st(Bo + row * B.ld + lane, v);
       a = vabs<VT>(v);

@@ -16,25 +16,32 @@
 #include "qsx_host.h"
 
 template <typename VT> struct VTr;
+// ld/st convert between the stored type and FP64; rnd gives the value that a
+// store followed by a load would return (rounding through the stored type),
+// without the memory round trip.
 template <> struct VTr<float> {
   static constexpr bool cplx = false;
   __device__ static double2 ld(const float* p) { return make_double2((double)*p, 0.0); }
   __device__ static void st(float* p, double2 v) { *p = (float)v.x; }
+  __device__ static double2 rnd(double2 v) { return make_double2((double)(float)v.x, 0.0); }
 };
 template <> struct VTr<double> {
   static constexpr bool cplx = false;
   __device__ static double2 ld(const double* p) { return make_double2(*p, 0.0); }
   __device__ static void st(double* p, double2 v) { *p = v.x; }
+  __device__ static double2 rnd(double2 v) { return make_double2(v.x, 0.0); }
 };
 template <> struct VTr<float2> {
   static constexpr bool cplx = true;
   __device__ static double2 ld(const float2* p) { float2 v = *p; return make_double2((double)v.x, (double)v.y); }
   __device__ static void st(float2* p, double2 v) { *p = make_float2((float)v.x, (float)v.y); }
+  __device__ static double2 rnd(double2 v) { return make_double2((double)(float)v.x, (double)(float)v.y); }
 };
 template <> struct VTr<double2> {
   static constexpr bool cplx = true;
   __device__ static double2 ld(const double2* p) { return *p; }
   __device__ static void st(double2* p, double2 v) { *p = v; }
+  __device__ static double2 rnd(double2 v) { return v; }
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {

@@ -16,40 +16,97 @@
 // ---------------------------------------------------------------------------
 // CSR SpMV dispatch
 // ---------------------------------------------------------------------------
-template <typename MT, typename VT, bool EPI>
-static void launch_csr_g(const CsrArgs& A, int G, hipStream_t st) {
-  const int64_t lanes = A.Ns * G;
-  const dim3 grid((unsigned)qsx_grid(lanes, 256)), blk(256);
-  switch (G) {
-    case 1: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 1, EPI>), grid, blk, 0, st, A); break;
-    case 2: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 2, EPI>), grid, blk, 0, st, A); break;
-    case 4: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 4, EPI>), grid, blk, 0, st, A); break;
-    case 8: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 8, EPI>), grid, blk, 0, st, A); break;
-    case 16: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 16, EPI>), grid, blk, 0, st, A); break;
-    case 32: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 32, EPI>), grid, blk, 0, st, A); break;
-    default: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 64, EPI>), grid, blk, 0, st, A); break;
+// Max over the per-block partials of one expm SpMV launch into the norm
+// slots (a few hundred atomics per launch instead of one per SpMV block).
+__global__ void __launch_bounds__(256) k_part_reduce(const double* __restrict__ part, int64_t nblocks,
+                                                     const int* done, double* normB, double* normF, int nsp) {
+  if (*done) return;
+  double mb = 0.0, mf = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nblocks; i += (int64_t)gridDim.x * blockDim.x) {
+    mb = fmax(mb, part[2 * i]);
+    mf = fmax(mf, part[2 * i + 1]);
+  }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mb = fmax(mb, __shfl_xor(mb, o, 64));
+    mf = fmax(mf, __shfl_xor(mf, o, 64));
+  }
+  __shared__ double sw[2][4];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sw[0][w] = mb; sw[1][w] = mf; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) { mb = fmax(mb, sw[0][i]); mf = fmax(mf, sw[1][i]); }
+    slot_put(normB, nsp, mb);
+    slot_put(normF, nsp, mf);
   }
 }
 
+// SpMV launches stride over rows with at most this many blocks (measured
+// faster than one block per 256/G rows on 10^7-row matrices; the per-block
+// norm partials of the expm form then fit a fixed buffer)
+constexpr int64_t EPI_MAX_BLOCKS = 65536;
+
+static int64_t csr_blocks(int64_t Ns, int G, bool /*epi*/) {
+  int64_t nb = qsx_grid(Ns * G, 256);
+  if (nb > EPI_MAX_BLOCKS) nb = EPI_MAX_BLOCKS;
+  return nb < 1 ? 1 : nb;
+}
+
+template <typename MT, typename VT, bool EPI, typename FT = VT>
+static void launch_csr_g(const CsrArgs& A, int G, hipStream_t st) {
+  const dim3 grid((unsigned)csr_blocks(A.Ns, G, EPI)), blk(256);
+  switch (G) {
+    case 1: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 1, EPI, FT>), grid, blk, 0, st, A); break;
+    case 2: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 2, EPI, FT>), grid, blk, 0, st, A); break;
+    case 4: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 4, EPI, FT>), grid, blk, 0, st, A); break;
+    case 8: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 8, EPI, FT>), grid, blk, 0, st, A); break;
+    case 16: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 16, EPI, FT>), grid, blk, 0, st, A); break;
+    case 32: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 32, EPI, FT>), grid, blk, 0, st, A); break;
+    default: hipLaunchKernelGGL((k_csr_spmv<MT, VT, 64, EPI, FT>), grid, blk, 0, st, A); break;
+  }
+}
+
+// fdt: dtype of the expm accumulator F (-1: the vector dtype); FP64 F with
+// FP32 vectors is the mixed-precision form.
 template <bool EPI>
-static int launch_csr(const CsrArgs& A, int mdt, int vdt, int G, hipStream_t st) {
+static int launch_csr(const CsrArgs& A, int mdt, int vdt, int G, hipStream_t st, int fdt = -1) {
+  if (fdt < 0) fdt = vdt;
   // matrix dtype must be the vector dtype or its real counterpart
-  if (vdt == QSX_F32 && mdt == QSX_F32) launch_csr_g<float, float, EPI>(A, G, st);
-  else if (vdt == QSX_F64 && mdt == QSX_F64) launch_csr_g<double, double, EPI>(A, G, st);
-  else if (vdt == QSX_C64 && mdt == QSX_F32) launch_csr_g<float, float2, EPI>(A, G, st);
-  else if (vdt == QSX_C64 && mdt == QSX_C64) launch_csr_g<float2, float2, EPI>(A, G, st);
-  else if (vdt == QSX_C128 && mdt == QSX_F64) launch_csr_g<double, double2, EPI>(A, G, st);
-  else if (vdt == QSX_C128 && mdt == QSX_C128) launch_csr_g<double2, double2, EPI>(A, G, st);
-  else { qsx_set_error("unsupported (matrix, vector) dtype pair"); return QSX_ERR_ARG; }
+  if (fdt == vdt) {
+    if (vdt == QSX_F32 && mdt == QSX_F32) launch_csr_g<float, float, EPI>(A, G, st);
+    else if (vdt == QSX_F64 && mdt == QSX_F64) launch_csr_g<double, double, EPI>(A, G, st);
+    else if (vdt == QSX_C64 && mdt == QSX_F32) launch_csr_g<float, float2, EPI>(A, G, st);
+    else if (vdt == QSX_C64 && mdt == QSX_C64) launch_csr_g<float2, float2, EPI>(A, G, st);
+    else if (vdt == QSX_C128 && mdt == QSX_F64) launch_csr_g<double, double2, EPI>(A, G, st);
+    else if (vdt == QSX_C128 && mdt == QSX_C128) launch_csr_g<double2, double2, EPI>(A, G, st);
+    else { qsx_set_error("unsupported (matrix, vector) dtype pair"); return QSX_ERR_ARG; }
+  } else if constexpr (EPI) {
+    if (vdt == QSX_F32 && fdt == QSX_F64 && mdt == QSX_F32) launch_csr_g<float, float, EPI, double>(A, G, st);
+    else if (vdt == QSX_C64 && fdt == QSX_C128 && mdt == QSX_F32) launch_csr_g<float, float2, EPI, double2>(A, G, st);
+    else if (vdt == QSX_C64 && fdt == QSX_C128 && mdt == QSX_C64) launch_csr_g<float2, float2, EPI, double2>(A, G, st);
+    else { qsx_set_error("unsupported mixed-precision (matrix, terms, accumulator) dtypes"); return QSX_ERR_ARG; }
+  } else {
+    qsx_set_error("accumulator dtype differs from the vector dtype outside the expm");
+    return QSX_ERR_ARG;
+  }
   QSX_LAUNCH_CHECK();
+  if constexpr (EPI) {
+    hipLaunchKernelGGL(k_part_reduce, dim3(64), dim3(256), 0, st, A.part, csr_blocks(A.Ns, G, true), &A.ctl->done,
+                       A.normB, A.normF, A.nsp);
+    QSX_LAUNCH_CHECK();
+  }
   return QSX_OK;
 }
 
+// Lanes per row from the mean row length: about a third to a sixth of it, at
+// least 2 (measured on MI355X: 16 entries per row run best with 4 lanes).
 static int auto_group(int64_t Ns, int64_t nnz) {
   const double avg = Ns > 0 ? (double)nnz / (double)Ns : 1.0;
-  int G = 1;
-  while (G < 64 && G < avg) G <<= 1;
-  return G > 1 ? G / 2 : 1;
+  if (avg <= 2.0) return 1;
+  int G = 2;
+  while (G < 64 && 3.0 * (2 * G) <= avg) G <<= 1;
+  return G;
 }
 
 // y = a*A*x (+ y when accumulate) ; x, y: (Ns, nvec) row-major.
@@ -436,7 +493,7 @@ extern "C" int qsx_csr_1_norm_diag(const void* indptr, const void* indices, cons
 // [1] tol, then normB[s*(m+1)], normF[s*(m+1)].
 template <typename VT>
 __global__ void __launch_bounds__(256) k_expm_start(const VT* __restrict__ F, VT* __restrict__ B, int64_t n,
-                                                    int64_t nvec, double* __restrict__ normB0) {
+                                                    int64_t nvec, double* __restrict__ normB0, int nsp) {
   double mx = 0.0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double rs = 0.0;
@@ -454,13 +511,14 @@ __global__ void __launch_bounds__(256) k_expm_start(const VT* __restrict__ F, VT
     if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomic_max_nonneg(normB0, sm[0]);
+  if (threadIdx.x == 0) slot_put(normB0, nsp, sm[0]);
 }
 
 // F *= eta ; B = F ; ||F|| -> normB0 of the next step ; done = 0
 template <typename VT>
 __global__ void __launch_bounds__(256) k_expm_scale(VT* __restrict__ F, VT* __restrict__ B, int64_t n, int64_t nvec,
-                                                    double2 eta, double* __restrict__ normB0, ExpmCtl* ctl) {
+                                                    double2 eta, double* __restrict__ normB0, ExpmCtl* ctl,
+                                                    int nsp) {
   double mx = 0.0;
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double rs = 0.0;
@@ -469,7 +527,7 @@ __global__ void __launch_bounds__(256) k_expm_scale(VT* __restrict__ F, VT* __re
       if constexpr (VTr<VT>::cplx) v = cmul(eta, v); else v.x *= eta.x;
       VTr<VT>::st(F + r * nvec + j, v);
       VTr<VT>::st(B + r * nvec + j, v);
-      const double2 w = VTr<VT>::ld(F + r * nvec + j);
+      const double2 w = VTr<VT>::rnd(v);
       rs += VTr<VT>::cplx ? sqrt(w.x * w.x + w.y * w.y) : fabs(w.x);
     }
     mx = fmax(mx, rs);
@@ -483,7 +541,7 @@ __global__ void __launch_bounds__(256) k_expm_scale(VT* __restrict__ F, VT* __re
     if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomic_max_nonneg(normB0, sm[0]);
+  if (threadIdx.x == 0) slot_put(normB0, nsp, sm[0]);
 }
 
 __global__ void k_expm_init_ctl(ExpmCtl* ctl, double tol, double* slots, int64_t nslots) {
@@ -507,9 +565,8 @@ __global__ void __launch_bounds__(256) k_expm_epi(const VT* __restrict__ B, VT* 
     double rb = 0.0, rf = 0.0;
     for (int64_t j = 0; j < nvec; ++j) {
       const double2 b = VTr<VT>::ld(B + r * nvec + j);
-      double2 f = cadd(VTr<VT>::ld(F + r * nvec + j), b);
+      const double2 f = VTr<VT>::rnd(cadd(VTr<VT>::ld(F + r * nvec + j), b));
       VTr<VT>::st(F + r * nvec + j, f);
-      f = VTr<VT>::ld(F + r * nvec + j);
       rb += VTr<VT>::cplx ? sqrt(b.x * b.x + b.y * b.y) : fabs(b.x);
       rf += VTr<VT>::cplx ? sqrt(f.x * f.x + f.y * f.y) : fabs(f.x);
     }
@@ -535,9 +592,14 @@ struct ExpmPlan {
   int vdt;
 };
 
-static int expm_ctl_alloc(const ExpmPlan& P, ExpmCtl** ctl, double** slots, hipStream_t st) {
-  const int64_t nsl = (int64_t)P.s * (P.m + 1) * 2;
-  QSX_CHECK_HIP(hipMallocAsync((void**)ctl, sizeof(ExpmCtl) + sizeof(double) * (nsl + 2), st));
+// doubles per norm slot in the CSR Taylor loops (their SpMV launches have one
+// block per 256/G rows; the matrix-free loop's epilogue grid is small: 1)
+constexpr int CSR_NSP = 8;
+
+static int expm_ctl_alloc(const ExpmPlan& P, ExpmCtl** ctl, double** slots, hipStream_t st, int nsp = 1) {
+  const int64_t nsl = (int64_t)P.s * (P.m + 1) * 2 * nsp;
+  // slots, then (CSR loops) 2 * EPI_MAX_BLOCKS per-block partials
+  QSX_CHECK_HIP(hipMallocAsync((void**)ctl, sizeof(ExpmCtl) + sizeof(double) * (nsl + 2 + 2 * EPI_MAX_BLOCKS), st));
   *slots = (double*)((char*)*ctl + sizeof(ExpmCtl));
   hipLaunchKernelGGL(k_expm_init_ctl, dim3(qsx_grid(nsl, 256) > 0 ? (unsigned)qsx_grid(nsl, 256) : 1u), dim3(256), 0, st,
                      *ctl, P.tol, *slots, nsl);
@@ -558,9 +620,10 @@ extern "C" int qsx_expm_multiply_csr(const void* indptr, const void* indices, co
   ExpmPlan P{s, m_star, tol, make_double2(a_re, a_im), make_double2(mu_re, mu_im), Ns, nvec, vec_dtype};
   ExpmCtl* ctl = nullptr;
   double* slots = nullptr;
-  int rc = expm_ctl_alloc(P, &ctl, &slots, st);
+  const int nsp = CSR_NSP;
+  int rc = expm_ctl_alloc(P, &ctl, &slots, st, nsp);
   if (rc) return rc;
-  const int64_t stride = (int64_t)(m_star + 1);
+  const int64_t stride = (int64_t)(m_star + 1) * nsp;  // slot (i, j) at normB + i * stride + j * nsp
   double* normB = slots;
   double* normF = slots + (int64_t)s * stride;
   size_t esz = (vec_dtype == QSX_F32) ? 4 : (vec_dtype == QSX_F64 || vec_dtype == QSX_C64) ? 8 : 16;
@@ -572,7 +635,7 @@ extern "C" int qsx_expm_multiply_csr(const void* indptr, const void* indices, co
   const double mag = std::exp(er / s);
   const double2 eta = make_double2(mag * std::cos(ei / s), mag * std::sin(ei / s));
   QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_start<VT>, dim3(rg), dim3(256), 0, st, (const VT*)v, (VT*)Bb[0], Ns,
-                                              nvec, normB));
+                                              nvec, normB, nsp));
   QSX_LAUNCH_CHECK();
   for (int i = 0; i < s; ++i) {
     for (int j = 0; j < m_star; ++j) {
@@ -586,19 +649,21 @@ extern "C" int qsx_expm_multiply_csr(const void* indptr, const void* indices, co
       A.beta = 0;
       A.F = v;
       A.ctl = ctl;
-      A.normB = normB + i * stride + j + 1;
-      A.normF = normF + i * stride + j + 1;
+      A.nsp = nsp;
+      A.part = slots + 2 * (int64_t)s * stride + 2;
+      A.normB = normB + i * stride + (j + 1) * nsp;
+      A.normF = normF + i * stride + (j + 1) * nsp;
       if (j >= 1) {
-        A.chk_c1 = normB + i * stride + j - 1;
-        A.chk_c2 = normB + i * stride + j;
-        A.chk_F = normF + i * stride + j;
+        A.chk_c1 = normB + i * stride + (j - 1) * nsp;
+        A.chk_c2 = normB + i * stride + j * nsp;
+        A.chk_F = normF + i * stride + j * nsp;
       }
       rc = launch_csr<true>(A, mat_dtype, vec_dtype, G, st);
       if (rc) return rc;
     }
     double* nb0 = (i + 1 < s) ? normB + (i + 1) * stride : nullptr;
     QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_scale<VT>, dim3(rg), dim3(256), 0, st, (VT*)v, (VT*)Bb[0], Ns,
-                                                nvec, eta, nb0, ctl));
+                                                nvec, eta, nb0, ctl, nsp));
     QSX_LAUNCH_CHECK();
   }
   (void)hipFreeAsync(ctl, st);
@@ -630,7 +695,7 @@ extern "C" int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star,
   const double mag = std::exp(er / s);
   const double2 eta = make_double2(mag * std::cos(ei / s), mag * std::sin(ei / s));
   QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_start<VT>, dim3(rg), dim3(256), 0, st, (const VT*)v, (VT*)Bb[0], Ns,
-                                              nvec, normB));
+                                              nvec, normB, 1));
   QSX_LAUNCH_CHECK();
   for (int i = 0; i < s; ++i) {
     for (int j = 0; j < m_star; ++j) {
@@ -651,7 +716,7 @@ extern "C" int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star,
     }
     double* nb0 = (i + 1 < s) ? normB + (i + 1) * stride : nullptr;
     QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_scale<VT>, dim3(rg), dim3(256), 0, st, (VT*)v, (VT*)Bb[0], Ns,
-                                                nvec, eta, nb0, ctl));
+                                                nvec, eta, nb0, ctl, 1));
     QSX_LAUNCH_CHECK();
   }
   (void)hipFreeAsync(ctl, st);
@@ -679,10 +744,10 @@ extern "C" int qsx_copy(const void* src, void* dst, int64_t bytes, void* stream)
 // dscale_t D)} v, partition / a / mu / tol / dscale read from es[t]
 // ---------------------------------------------------------------------------
 // F *= eta(record) ; B = F ; ||F|| -> normB0 of the next stage ; done = 0 (stage i of the record)
-template <typename VT>
-__global__ void __launch_bounds__(256) k_expm_scale_rec(VT* __restrict__ F, VT* __restrict__ B, int64_t n,
+template <typename FT, typename VT>
+__global__ void __launch_bounds__(256) k_expm_scale_rec(FT* __restrict__ F, VT* __restrict__ B, int64_t n,
                                                         int64_t nvec, const ExpmSec* rec, int stage,
-                                                        double* __restrict__ normB0, ExpmCtl* ctl) {
+                                                        double* __restrict__ normB0, ExpmCtl* ctl, int nsp) {
   const ExpmSec E = *rec;
   if (stage >= E.s) return;
   const double2 eta = expm_eta(E);
@@ -690,12 +755,12 @@ __global__ void __launch_bounds__(256) k_expm_scale_rec(VT* __restrict__ F, VT* 
   for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
     double rs = 0.0;
     for (int64_t j = 0; j < nvec; ++j) {
-      double2 v = VTr<VT>::ld(F + r * nvec + j);
-      if constexpr (VTr<VT>::cplx) v = cmul(eta, v); else v.x *= eta.x;
-      VTr<VT>::st(F + r * nvec + j, v);
+      double2 v = VTr<FT>::ld(F + r * nvec + j);
+      if constexpr (VTr<FT>::cplx) v = cmul(eta, v); else v.x *= eta.x;
+      VTr<FT>::st(F + r * nvec + j, v);
       VTr<VT>::st(B + r * nvec + j, v);
-      const double2 w = VTr<VT>::ld(F + r * nvec + j);
-      rs += VTr<VT>::cplx ? sqrt(w.x * w.x + w.y * w.y) : fabs(w.x);
+      const double2 w = VTr<FT>::rnd(v);
+      rs += VTr<FT>::cplx ? sqrt(w.x * w.x + w.y * w.y) : fabs(w.x);
     }
     mx = fmax(mx, rs);
   }
@@ -708,16 +773,41 @@ __global__ void __launch_bounds__(256) k_expm_scale_rec(VT* __restrict__ F, VT* 
     if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) atomic_max_nonneg(normB0, sm[0]);
+  if (threadIdx.x == 0) slot_put(normB0, nsp, sm[0]);
+}
+
+// B = F (rounded to the term dtype), ||F|| -> normB0
+template <typename FT, typename VT>
+__global__ void __launch_bounds__(256) k_expm_start_mixed(const FT* __restrict__ F, VT* __restrict__ B, int64_t n,
+                                                          int64_t nvec, double* __restrict__ normB0, int nsp) {
+  double mx = 0.0;
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+    double rs = 0.0;
+    for (int64_t j = 0; j < nvec; ++j) {
+      const double2 v = VTr<FT>::ld(F + r * nvec + j);
+      VTr<VT>::st(B + r * nvec + j, v);
+      rs += VTr<FT>::cplx ? sqrt(v.x * v.x + v.y * v.y) : fabs(v.x);
+    }
+    mx = fmax(mx, rs);
+  }
+  __shared__ double sm[256];
+  sm[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] = fmax(sm[threadIdx.x], sm[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) slot_put(normB0, nsp, sm[0]);
 }
 
 // A stage that ran all m terms of its record without passing the truncation
 // test flags the step.
 __global__ void k_expm_tail(const ExpmSec* rec, int stage, const ExpmCtl* ctl, const double* nb, const double* nf,
-                            int* fail) {
+                            int nsp, int* fail) {
   const ExpmSec E = *rec;
   if (stage >= E.s || E.m < 1 || ctl->done) return;
-  if (!(nb[E.m - 1] + nb[E.m] <= E.tol * nf[E.m])) *fail = 1;
+  const double c1 = slot_max(nb + (int64_t)(E.m - 1) * nsp, nsp), c2 = slot_max(nb + (int64_t)E.m * nsp, nsp);
+  if (!(c1 + c2 <= E.tol * slot_max(nf + (int64_t)E.m * nsp, nsp))) *fail = 1;
 }
 
 // zero the norm slots and the done flag (graph-safe: no memset nodes)
@@ -728,7 +818,8 @@ __global__ void k_expm_reset(ExpmCtl* ctl, double* slots, int64_t nslots) {
 }
 
 extern "C" int64_t qsx_evolve_csr_ctl_bytes(int s_max, int m_max) {
-  return (int64_t)sizeof(ExpmCtl) + (int64_t)sizeof(double) * 2 * (int64_t)s_max * (m_max + 1);
+  return (int64_t)sizeof(ExpmCtl) +
+         (int64_t)sizeof(double) * (2 * (int64_t)s_max * (m_max + 1) * CSR_NSP + 2 * EPI_MAX_BLOCKS);
 }
 
 namespace {
@@ -748,8 +839,8 @@ extern "C" void qsx_graph_cache_clear() {
 extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* data, int idx64,
                                     int mat_dtype, int64_t Ns, int64_t nnz, const double* xdiag, int nsteps,
                                     const void* es, int s_max, int m_max, void* v, void* work, int vec_dtype,
-                                    int64_t nvec, int* fail, void* ctl_work, int64_t ctl_bytes, int group,
-                                    int use_graph, void* stream) {
+                                    int state_dtype, int64_t nvec, int* fail, void* ctl_work, int64_t ctl_bytes,
+                                    int group, int use_graph, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (Ns < 0 || nsteps < 1 || !es || s_max < 1 || m_max < 1 || !v || !work || !fail || !ctl_work || nvec < 1 ||
       ctl_bytes < qsx_evolve_csr_ctl_bytes(s_max, m_max)) {
@@ -757,9 +848,16 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
     return QSX_ERR_ARG;
   }
   if (Ns == 0) return QSX_OK;
+  if (state_dtype < 0) state_dtype = vec_dtype;
+  const bool mixed = state_dtype != vec_dtype;
+  if (mixed && !((vec_dtype == QSX_C64 && state_dtype == QSX_C128) || (vec_dtype == QSX_F32 && state_dtype == QSX_F64))) {
+    qsx_set_error("qsx_evolve_csr_steps: the state dtype must be the term dtype or its FP64 counterpart");
+    return QSX_ERR_ARG;
+  }
   ExpmCtl* ctl = (ExpmCtl*)ctl_work;
   double* slots = (double*)((char*)ctl_work + sizeof(ExpmCtl));
-  const int64_t stride = (int64_t)(m_max + 1);
+  const int nsp = CSR_NSP;
+  const int64_t stride = (int64_t)(m_max + 1) * nsp;  // slot (i, j) at normB + i * stride + j * nsp
   const int64_t nsl = 2 * (int64_t)s_max * stride;
   double* normB = slots;
   double* normF = slots + (int64_t)s_max * stride;
@@ -774,8 +872,16 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
       const ExpmSec* R = E + t;
       hipLaunchKernelGGL(k_expm_reset, dim3(16), dim3(256), 0, q, ctl, slots, nsl);
       QSX_LAUNCH_CHECK();
-      QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_start<VT>, dim3(rg), dim3(256), 0, q, (const VT*)v, (VT*)Bb[0],
-                                                  Ns, nvec, normB));
+      if (!mixed) {
+        QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_start<VT>, dim3(rg), dim3(256), 0, q, (const VT*)v,
+                                                    (VT*)Bb[0], Ns, nvec, normB, nsp));
+      } else if (vec_dtype == QSX_C64) {
+        hipLaunchKernelGGL((k_expm_start_mixed<double2, float2>), dim3(rg), dim3(256), 0, q, (const double2*)v,
+                           (float2*)Bb[0], Ns, nvec, normB, nsp);
+      } else {
+        hipLaunchKernelGGL((k_expm_start_mixed<double, float>), dim3(rg), dim3(256), 0, q, (const double*)v,
+                           (float*)Bb[0], Ns, nvec, normB, nsp);
+      }
       QSX_LAUNCH_CHECK();
       for (int i = 0; i < s_max; ++i) {
         for (int j = 0; j < m_max; ++j) {
@@ -789,22 +895,32 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
           A.stage = i;
           A.term = j;
           A.xdiag = xdiag;
-          A.normB = normB + i * stride + j + 1;
-          A.normF = normF + i * stride + j + 1;
+          A.nsp = nsp;
+          A.part = slots + nsl;
+          A.normB = normB + i * stride + (j + 1) * nsp;
+          A.normF = normF + i * stride + (j + 1) * nsp;
           if (j >= 1) {
-            A.chk_c1 = normB + i * stride + j - 1;
-            A.chk_c2 = normB + i * stride + j;
-            A.chk_F = normF + i * stride + j;
+            A.chk_c1 = normB + i * stride + (j - 1) * nsp;
+            A.chk_c2 = normB + i * stride + j * nsp;
+            A.chk_F = normF + i * stride + j * nsp;
           }
-          int rc = launch_csr<true>(A, mat_dtype, vec_dtype, G, q);
+          int rc = launch_csr<true>(A, mat_dtype, vec_dtype, G, q, state_dtype);
           if (rc) return rc;
         }
         hipLaunchKernelGGL(k_expm_tail, dim3(1), dim3(1), 0, q, R, i, ctl, normB + i * stride, normF + i * stride,
-                           fail + t);
+                           nsp, fail + t);
         QSX_LAUNCH_CHECK();
         double* nb0 = (i + 1 < s_max) ? normB + (i + 1) * stride : normB;
-        QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_scale_rec<VT>, dim3(rg), dim3(256), 0, q, (VT*)v,
-                                                    (VT*)Bb[0], Ns, nvec, R, i, nb0, ctl));
+        if (!mixed) {
+          QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL((k_expm_scale_rec<VT, VT>), dim3(rg), dim3(256), 0, q, (VT*)v,
+                                                      (VT*)Bb[0], Ns, nvec, R, i, nb0, ctl, nsp));
+        } else if (vec_dtype == QSX_C64) {
+          hipLaunchKernelGGL((k_expm_scale_rec<double2, float2>), dim3(rg), dim3(256), 0, q, (double2*)v,
+                             (float2*)Bb[0], Ns, nvec, R, i, nb0, ctl, nsp);
+        } else {
+          hipLaunchKernelGGL((k_expm_scale_rec<double, float>), dim3(rg), dim3(256), 0, q, (double*)v, (float*)Bb[0],
+                             Ns, nvec, R, i, nb0, ctl, nsp);
+        }
         QSX_LAUNCH_CHECK();
       }
     }
@@ -812,7 +928,7 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
   };
   if (!use_graph) return record(st);
   const StepKey key{indptr, indices, data, xdiag, es, v, work, fail, ctl_work, Ns, nvec, nsteps, s_max, m_max, G,
-                    mat_dtype, vec_dtype};
+                    mat_dtype, vec_dtype * 8 + state_dtype};
   auto it = g_step_graphs.find(key);
   if (it == g_step_graphs.end()) {
     if (!g_cap_stream) QSX_CHECK_HIP(hipStreamCreateWithFlags(&g_cap_stream, hipStreamNonBlocking));

@@ -16,6 +16,16 @@ __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
 
 __device__ __forceinline__ double cabs2(double2 z) { return sqrt(z.x * z.x + z.y * z.y); }
 
+// A norm slot is a group of nsp doubles: writers take the atomic max into
+// entry blockIdx % nsp (spreading the same-address atomics of large grids),
+// readers take the max over the group.
+__device__ __forceinline__ double slot_max(const double* p, int nsp) {
+  double m = p[0];
+  for (int i = 1; i < nsp; ++i) m = fmax(m, p[i]);
+  return m;
+}
+__device__ __forceinline__ void slot_put(double* p, int nsp, double v) { atomic_max_nonneg(p + (blockIdx.x % nsp), v); }
+
 // Taylor-loop control block (one per expm call).
 struct ExpmCtl {
   int done;      // set when the truncation test of the current outer step passed
@@ -62,6 +72,8 @@ struct CsrArgs {
   const double* chk_c1; // previous test inputs: c1, c2, ||F|| (null: no test)
   const double* chk_c2;
   const double* chk_F;
+  int nsp;              // doubles per norm slot (>= 1)
+  double* part;         // expm: per-block (max |B| row sum, max |F| row sum), reduced by k_part_reduce
   // optional extra diagonal: A + xdscale * diag(xdiag) (xdscale from rec when set)
   const double* xdiag;
   double2 xdscale;
@@ -75,13 +87,20 @@ __device__ __forceinline__ int64_t ld_idx(const void* p, int idx64, int64_t i) {
 }
 
 // G lanes per row.  EPI: fused expm epilogue.
-template <typename MT, typename VT, int G, bool EPI>
+// FT: element type of F (the accumulated Taylor sum) in the expm epilogue;
+// FP64 F with FP32 terms is the mixed-precision Taylor loop.
+template <typename MT, typename VT, int G, bool EPI, typename FT = VT>
 __global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
   double2 a_t = A.a, mu_t = A.mu, xs = A.xdscale;
   if constexpr (EPI) {
+    // every control value is read in one round (no load waits on another)
+    const int done = A.ctl->done;
     double tol = A.ctl->tol;
+    ExpmSec E{};
+    if (A.rec) E = *A.rec;
+    double c1 = 0.0, c2 = 0.0, nf = 0.0;
+    if (A.chk_c1) { c1 = slot_max(A.chk_c1, A.nsp); c2 = slot_max(A.chk_c2, A.nsp); nf = slot_max(A.chk_F, A.nsp); }
     if (A.rec) {
-      const ExpmSec E = *A.rec;
       if (A.stage >= E.s || A.term >= E.m) return;
       const double cf = 1.0 / ((double)E.s * (double)(A.term + 1));
       a_t = make_double2(E.a.x * cf, E.a.y * cf);
@@ -89,25 +108,25 @@ __global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
       xs = E.dscale;
       tol = E.tol;
     }
-    if (A.ctl->done) return;
-    if (A.chk_c1) {
-      // truncation test of the previous term (SCIPY:298-300)
-      const double c1 = *A.chk_c1, c2 = *A.chk_c2, nf = *A.chk_F;
-      if (c1 + c2 <= tol * nf) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl->done = 1;
-        return;
-      }
+    if (done) return;
+    // truncation test of the previous term (SCIPY:298-300)
+    if (A.chk_c1 && c1 + c2 <= tol * nf) {
+      if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl->done = 1;
+      return;
     }
   }
   constexpr bool CPLX = VTr<VT>::cplx;
   const int lane = threadIdx.x % G;
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G;
-  double rowB = 0.0, rowF = 0.0;
-  if (row < A.Ns) {
+  double rowB = 0.0, rowF = 0.0;  // max over this group's rows of the row sums of |B|, |F|
+  // grid-stride over rows (the expm form runs a capped grid so that the norm
+  // atomics stay few); the G lanes of a row always iterate together
+  const int64_t stride = ((int64_t)gridDim.x * blockDim.x) / G;
+  for (int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; row < A.Ns; row += stride) {
     const int64_t k0 = ld_idx(A.indptr, A.idx64, row), k1 = ld_idx(A.indptr, A.idx64, row + 1);
     const MT* __restrict__ val = (const MT*)A.data;
     const VT* __restrict__ x = (const VT*)A.x;
     VT* __restrict__ y = (VT*)A.y;
+    double curB = 0.0, curF = 0.0;
     for (int64_t j = 0; j < A.nvec; ++j) {
       double sr = 0.0, si = 0.0;
       for (int64_t k = k0 + lane; k < k1; k += G) {
@@ -139,30 +158,37 @@ __global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
         if (A.beta) res = cadd(res, VTr<VT>::ld(y + o));
         VTr<VT>::st(y + o, res);
         if constexpr (EPI) {
-          VT* F = (VT*)A.F;
-          double2 f = VTr<VT>::ld(F + o);
-          // the stored (possibly rounded) B enters F, as in the reference
-          const double2 rs = VTr<VT>::ld(y + o);
-          f = cadd(f, rs);
-          VTr<VT>::st(F + o, f);
-          rowB += CPLX ? cabs2(rs) : fabs(rs.x);
-          const double2 fs = VTr<VT>::ld(F + o);
-          rowF += CPLX ? cabs2(fs) : fabs(fs.x);
+          FT* F = (FT*)A.F;
+          // the stored (rounded) B enters F, as in the reference
+          const double2 rs = VTr<VT>::rnd(res);
+          const double2 fs = VTr<FT>::rnd(cadd(VTr<FT>::ld(F + o), rs));
+          VTr<FT>::st(F + o, fs);
+          curB += CPLX ? cabs2(rs) : fabs(rs.x);
+          curF += CPLX ? cabs2(fs) : fabs(fs.x);
         }
       }
     }
+    rowB = fmax(rowB, curB);
+    rowF = fmax(rowF, curF);
   }
   if constexpr (EPI) {
-    // block max of the row sums, then one atomic per block
-    __shared__ double sB[256 / G], sF[256 / G];
-    const int gid = threadIdx.x / G;
-    if (lane == 0) { sB[gid] = rowB; sF[gid] = rowF; }
+    // block max of the row sums (wavefront shuffles, then the 4 wavefronts),
+    // one atomic per block and slot
+    double mb = rowB, mf = rowF;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      mb = fmax(mb, __shfl_xor(mb, o, 64));
+      mf = fmax(mf, __shfl_xor(mf, o, 64));
+    }
+    __shared__ double sw[2][4];
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sw[0][w] = mb; sw[1][w] = mf; }
     __syncthreads();
     if (threadIdx.x == 0) {
-      double mb = 0.0, mf = 0.0;
-      for (int i = 0; i < 256 / G; ++i) { mb = fmax(mb, sB[i]); mf = fmax(mf, sF[i]); }
-      atomic_max_nonneg(A.normB, mb);
-      atomic_max_nonneg(A.normF, mf);
+      for (int i = 1; i < (int)(blockDim.x >> 6); ++i) { mb = fmax(mb, sw[0][i]); mf = fmax(mf, sw[1][i]); }
+      A.part[2 * (int64_t)blockIdx.x] = mb;
+      A.part[2 * (int64_t)blockIdx.x + 1] = mf;
     }
   }
 }
+

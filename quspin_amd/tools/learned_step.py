@@ -170,10 +170,13 @@ class DrivenEvolution:
     or a static ``hamiltonian``); ``f``: the drive f(t); ``phase``: the drive
     phase as a function of t (an input of the network).  H0 is assembled once
     as a device CSR in the vector dtype; every step applies
-    H0 + f(t_mid) diag(D) through the CSR Taylor kernels."""
+    H0 + f(t_mid) diag(D) through the CSR Taylor kernels.  ``dtype`` is the
+    dtype of the Taylor terms and the SpMV; ``state_dtype`` (default: the
+    same) that of the state, in which the terms are accumulated: complex128
+    with complex64 terms is the FP32 SpMV / FP64 accumulation form."""
 
     def __init__(self, H0, D_op, f, dt, dtype=np.complex64, phase=None, predictor=None, m_margin=2, m_cap=60,
-                 s_cap=8, group=0):
+                 s_cap=8, group=0, state_dtype=None):
         _check_record()
         op = as_device_operator(H0)
         dop = as_device_operator(D_op)
@@ -183,11 +186,12 @@ class DrivenEvolution:
         self.f, self.dt = f, float(dt)
         self.phase = phase if phase is not None else (lambda t: 0.0)
         self.vdt = np.dtype(dtype)
+        # the accumulated Taylor sum (the state): the term dtype or its FP64 counterpart
+        self.sdt = np.dtype(state_dtype) if state_dtype is not None else self.vdt
         self.tol = float(np.finfo(self.vdt).eps / 2)
         self.predictor = predictor
         self.m_margin, self.m_cap, self.s_cap, self.group = m_margin, m_cap, s_cap, group
-        self.csr = op.to_csr(self.vdt) if isinstance(op, MatrixFreeOp) else DeviceCSR.from_scipy(op.to_scipy(),
-                                                                                                 self.vdt)
+        self.csr = self._assemble(op)
         self.N = self.csr.shape[0]
         self.trH0 = complex(self.csr.trace())
         ones = L.torch().ones(self.N, dtype=L.torch().float64, device="cuda")
@@ -197,7 +201,25 @@ class DrivenEvolution:
         self._colsum = L.empty(self.N, np.float64)
         self._nout = L.empty(1, np.float64)
         self._ctl = {}
+        if not group:
+            self.group = self._tune_group()
         self.stats = {"norm_passes": 0, "learned_steps": 0, "exact_steps": 0, "fallback_blocks": 0}
+
+    def _assemble(self, op):
+        """H0 as a device CSR: real values (half the bytes) when H0 is real,
+        else values of the term dtype."""
+        real = np.dtype(np.float32 if self.vdt == np.complex64 else np.float64)
+        for dt_ in ((real, self.vdt) if self.vdt.kind == "c" else (self.vdt,)):
+            try:
+                if isinstance(op, MatrixFreeOp):
+                    return op.to_csr(dt_)
+                M = op.to_scipy()
+                if np.dtype(dt_).kind == "f" and np.iscomplexobj(M.data) and np.any(M.data.imag):
+                    continue
+                return DeviceCSR.from_scipy(M.real if np.dtype(dt_).kind == "f" else M, dt_)
+            except TypeError:
+                continue
+        raise TypeError("H0 cannot be assembled in {}".format(self.vdt))
 
     # --------------------------------------------------------------- per t
     def mu(self, t):
@@ -245,44 +267,87 @@ class DrivenEvolution:
         ctl = self._ctl_buf(s_max, m_max)
         L.check(L.load().qsx_evolve_csr_steps(
             ptr(c.indptr), ptr(c.indices), ptr(c.data), c.idx64, L.dtype_code(c.dtype), self.N, c.nnz, ptr(self.D),
-            int(nsteps), ptr(rec), int(s_max), int(m_max), ptr(psi), ptr(work), L.dtype_code(self.vdt), 1, ptr(fail),
+            int(nsteps), ptr(rec), int(s_max), int(m_max), ptr(psi), ptr(work), L.dtype_code(self.vdt),
+            L.dtype_code(self.sdt), 1, ptr(fail),
             ptr(ctl), int(ctl.numel() * 8), int(self.group), int(bool(graph)), stream_handle()),
             "qsx_evolve_csr_steps")
 
+    def _tune_group(self, candidates=(2, 4, 8)):
+        """Lanes per CSR row of the Taylor-term kernel: one 8-term step timed
+        per candidate on the operator itself (small operators keep 0 = the
+        library's rule)."""
+        if self.N < 65536:
+            return 0
+        t = L.torch()
+        psi = L.zeros(self.N, self.sdt)
+        psi[0] = 1
+        work = L.empty(2 * self.N, self.vdt)
+        rec = L.to_device(self._records([0.0], [(8, 1)]).view(np.uint8))
+        fail = L.zeros(1, np.int32)
+        st = t.cuda.current_stream()
+        best = None
+        for g in candidates:
+            self.group = g
+            self._steps(psi, work, rec, 1, 1, 8, fail, False)
+            e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+            e0.record(st)
+            self._steps(psi, work, rec, 1, 1, 8, fail, False)
+            e1.record(st)
+            t.cuda.synchronize()
+            ms = e0.elapsed_time(e1)
+            if best is None or ms < best[0]:
+                best = (ms, g)
+        return best[1]
+
     # ----------------------------------------------------------------- run
+    def _buffers(self, nb):
+        """Device buffers of a block of nb steps, kept so that a captured
+        step sequence is replayed (same addresses) block after block."""
+        b = getattr(self, "_buf", None)
+        if b is None or b["nb"] != nb:
+            b = {"nb": nb, "rec": L.empty(nb * RECORD.itemsize, np.uint8), "X": L.empty((nb, 4), np.float64),
+                 "fail": L.zeros(nb, np.int32), "work": L.empty(2 * self.N, self.vdt),
+                 "backup": L.empty(self.N, self.sdt), "rec1": L.empty(RECORD.itemsize, np.uint8),
+                 "fail1": L.zeros(1, np.int32)}
+            self._buf = b
+        return b
+
     def run(self, psi, t0, nsteps, learned=True, block=50, graph=True):
         """Evolve the device vector ``psi`` (overwritten) by ``nsteps`` steps
         from ``t0``; returns psi."""
-        if psi.numel() != self.N or L.np_dtype(psi.dtype) != self.vdt or not psi.is_contiguous():
-            raise ValueError("psi must be a contiguous device vector of {} {}".format(self.N, self.vdt))
-        work = L.empty(2 * self.N, self.vdt)
+        t = L.torch()
+        if psi.numel() != self.N or L.np_dtype(psi.dtype) != self.sdt or not psi.is_contiguous():
+            raise ValueError("psi must be a contiguous device vector of {} {}".format(self.N, self.sdt))
         k = 0
         while k < nsteps:
             nb = min(block, nsteps - k)
             ts = [t0 + (k + i + 0.5) * self.dt for i in range(nb)]
+            buf = self._buffers(block)
             done = False
             if learned and self.predictor is not None:
-                backup = psi.clone()
-                rec = L.to_device(self._records(ts).view(np.uint8))
-                X = L.to_device(np.array([self.inputs(t) for t in ts], np.float64))
+                buf["backup"].copy_(psi)
+                rec = buf["rec"][:nb * RECORD.itemsize]
+                rec.copy_(t.from_numpy(self._records(ts).view(np.uint8)))
+                X = buf["X"][:nb]
+                X.copy_(t.from_numpy(np.array([self.inputs(tt) for tt in ts], np.float64)))
                 self.predictor.write_partitions(X, rec, self.m_margin, self.m_cap, self.s_cap)
                 parts = L.to_host(rec).view(RECORD)
                 s_b = int(parts["s"].max())
-                m_b = int(-(-int(parts["m"].max()) // 4) * 4)  # launch bound rounded up to a multiple of 4
-                fail = L.zeros(nb, np.int32)
-                self._steps(psi, work, rec, nb, s_b, m_b, fail, graph)
+                m_b = int(-(-int(parts["m"].max()) // 8) * 8)  # launch bound rounded up to a multiple of 8
+                fail = buf["fail"][:nb]
+                fail.zero_()
+                self._steps(psi, buf["work"], rec, nb, s_b, m_b, fail, graph)
                 if int(L.to_host(fail).sum()) == 0:
                     self.stats["learned_steps"] += nb
                     done = True
                 else:
                     self.stats["fallback_blocks"] += 1
-                    psi.copy_(backup)
+                    psi.copy_(buf["backup"])
             if not done:
-                for t in ts:
-                    m, s = self.exact(t)
-                    rec = L.to_device(self._records([t], [(m, s)]).view(np.uint8))
-                    fail = L.zeros(1, np.int32)
-                    self._steps(psi, work, rec, 1, max(s, 1), max(m, 1), fail, False)
+                for tt in ts:
+                    m, s = self.exact(tt)
+                    buf["rec1"].copy_(t.from_numpy(self._records([tt], [(m, s)]).view(np.uint8)))
+                    self._steps(psi, buf["work"], buf["rec1"], 1, max(s, 1), max(m, 1), buf["fail1"], False)
                     self.stats["exact_steps"] += 1
             k += nb
         return psi

@@ -166,3 +166,26 @@ def test_tail_check_rejects_short_partitions():
     good.run(p2, 0.0, 10, learned=False)
     assert bad.stats["fallback_blocks"] == 2 and bad.stats["exact_steps"] == 10
     np.testing.assert_allclose(L.to_host(p1), L.to_host(p2), rtol=0, atol=1e-12)
+
+
+def test_fp32_terms_with_fp64_state_accumulation():
+    """complex64 matrix, Taylor terms and SpMV with the state accumulated in
+    complex128: closer to the FP64 path than the all-complex64 loop."""
+    from quspin_amd import _lib as L
+    from quspin_amd.tools.learned_step import DrivenEvolution
+    B, hop, D, f, phase = _hubbard(8)
+    dt, n = 0.05, 60
+    rng = np.random.default_rng(4)
+    psi0 = rng.normal(size=B.Ns) + 1j * rng.normal(size=B.Ns)
+    psi0 /= np.linalg.norm(psi0)
+    out = {}
+    for name, dt_, sdt in (("fp64", np.complex128, None), ("fp32", np.complex64, None),
+                           ("mixed", np.complex64, np.complex128)):
+        ev = DrivenEvolution(hop, D, f, dt, dtype=dt_, phase=phase, state_dtype=sdt)
+        p = L.to_device(psi0.astype(sdt or dt_))
+        ev.run(p, 0.0, n, learned=False)
+        out[name] = L.to_host(p).astype(np.complex128)
+    e32 = np.linalg.norm(out["fp32"] - out["fp64"])
+    emx = np.linalg.norm(out["mixed"] - out["fp64"])
+    print(f"state error after {n} steps: complex64 {e32:.3e}, complex64 terms + complex128 state {emx:.3e}")
+    assert emx < 1e-6 and emx < 0.7 * e32
