@@ -193,6 +193,15 @@ int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star, double tol
 int qsx_lanczos_update(int dtype, const void* v, const double* b_prev, const void* q, void* r, int64_t n,
                        double* a_out, double* b_out, void* stream);
 int qsx_lanczos_div(int dtype, const void* r, const double* b, void* q, int64_t n, void* stream);
+/* The whole m-step recurrence of lanczos_full / lanczos_iter
+ * (_lanczos_utils.py:158-227) on the device: op_kind 0 = H.v context
+ * (hv_ctx), 1 = CSR (indptr/indices/data).  Q holds q0 in row 0 and receives
+ * all m vectors when store_Q, else it is a ring of 3 vectors; ab = m x
+ * (a re/im, b re/im) doubles.  use_graph captures the sequence once per set
+ * of buffers and replays it. */
+int qsx_lanczos_run(int op_kind, void* hv_ctx, const void* indptr, const void* indices, const void* data, int idx64,
+                    int mat_dtype, int64_t nnz, int group, int dtype, int64_t n, int m, void* Q, int store_Q, void* r,
+                    double* ab, int use_graph, void* stream);
 int qsx_vdot(int dtype, const void* x, const void* y, int64_t n, double* out, void* stream);
 int qsx_nrm2(int dtype, const void* x, int64_t n, double* out, void* stream);
 int qsx_axpy(int dtype, const void* x, void* y, int64_t n, double a_re, double a_im, const double* sdev, double sgn,

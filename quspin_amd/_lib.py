@@ -78,6 +78,8 @@ SIGNATURES = {
     "qsx_axpy": (_i32, [_i32, _vp, _vp, _i64, _dbl, _dbl, _vp, _dbl, _i32, _vp]),
     "qsx_lanczos_update": (_i32, [_i32, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "qsx_lanczos_div": (_i32, [_i32, _vp, _vp, _vp, _i64, _vp]),
+    "qsx_lanczos_run": (_i32, [_i32, _vp, _vp, _vp, _vp, _i32, _i32, _i64, _i32, _i32, _i64, _i32, _vp, _i32, _vp, _vp,
+                               _i32, _vp]),
     "qsx_lincomb": (_i32, [_i32, _vp, _i64, _i64, _vp, _vp, _vp]),
     "qsx_copy": (_i32, [_vp, _vp, _i64, _vp]),
     "qsx_sector_rank": (_i32, [_vp, _vp, _i64, _vp, _vp, _vp]),

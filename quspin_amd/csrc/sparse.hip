@@ -152,18 +152,34 @@ __device__ __forceinline__ void block_finish(double vr, double vi, double* parti
     last = (t == gridDim.x - 1);
   }
   __syncthreads();
-  if (last && threadIdx.x == 0) {
+  if (last) {
+    // the last block reduces every block's partial: threads read a fixed
+    // strided share (device-coherent loads, past this CU's cache), then the
+    // same fixed tree as above, so the result does not depend on timing
     __threadfence();
     double a = 0.0, b = 0.0;
-    for (unsigned int k = 0; k < gridDim.x; ++k) {
-      const double pr = ((volatile double*)partials)[2 * k];
-      const double pi = ((volatile double*)partials)[2 * k + 1];
+    for (unsigned int k = threadIdx.x; k < gridDim.x; k += blockDim.x) {
+      const double pr = __hip_atomic_load(partials + 2 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      const double pi = __hip_atomic_load(partials + 2 * k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       if (OP == 0) { a += pr; b += pi; } else a = fmax(a, pr);
     }
-    if (post_scale_sqrt != 0.0) a = sqrt(a);
-    out[0] = a;
-    out[1] = b;
-    *counter = 0u;
+    sr[threadIdx.x] = a;
+    si[threadIdx.x] = b;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+      if (threadIdx.x < o) {
+        if (OP == 0) { sr[threadIdx.x] += sr[threadIdx.x + o]; si[threadIdx.x] += si[threadIdx.x + o]; }
+        else sr[threadIdx.x] = fmax(sr[threadIdx.x], sr[threadIdx.x + o]);
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      a = sr[0];
+      if (post_scale_sqrt != 0.0) a = sqrt(a);
+      out[0] = a;
+      out[1] = si[0];
+      *counter = 0u;
+    }
   }
 }
 
@@ -826,6 +842,9 @@ namespace {
 using StepKey = std::tuple<const void*, const void*, const void*, const void*, const void*, const void*, const void*,
                            const void*, const void*, int64_t, int64_t, int, int, int, int, int, int>;
 std::map<StepKey, hipGraphExec_t> g_step_graphs;
+using LzKey = std::tuple<const void*, const void*, const void*, const void*, const void*, const void*, int64_t, int, int,
+                         int, int>;
+std::map<LzKey, hipGraphExec_t> g_lz_graphs;
 hipStream_t g_cap_stream = nullptr;
 }  // namespace
 
@@ -834,6 +853,8 @@ hipStream_t g_cap_stream = nullptr;
 extern "C" void qsx_graph_cache_clear() {
   for (auto& kv : g_step_graphs) (void)hipGraphExecDestroy(kv.second);
   g_step_graphs.clear();
+  for (auto& kv : g_lz_graphs) (void)hipGraphExecDestroy(kv.second);
+  g_lz_graphs.clear();
 }
 
 extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* data, int idx64,
@@ -943,6 +964,70 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) { qsx_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
     it = g_step_graphs.emplace(key, ex).first;
+  }
+  QSX_CHECK_HIP(hipGraphLaunch(it->second, st));
+  return QSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Whole Lanczos recurrence on the device (_lanczos_utils.py:158-227): m steps
+// of div / H.v / fused updates with alpha, beta kept in device memory, as one
+// launch sequence that can be captured once and replayed.
+// ---------------------------------------------------------------------------
+extern "C" int qsx_hv_ctx_apply(void* X, const void* x, void* y, double a_off_re, double a_off_im,
+                                double a_diag_re, double a_diag_im, int beta, void* stream);
+
+extern "C" int qsx_lanczos_run(int op_kind, void* hv_ctx, const void* indptr, const void* indices, const void* data,
+                               int idx64, int mat_dtype, int64_t nnz, int group, int dtype, int64_t n, int m, void* Q,
+                               int store_Q, void* r, double* ab, int use_graph, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (n < 1 || m < 1 || !Q || !r || !ab || (op_kind == 0 && !hv_ctx) || (op_kind == 1 && (!indptr || !data))) {
+    qsx_set_error("qsx_lanczos_run: bad arguments");
+    return QSX_ERR_ARG;
+  }
+  RedScratch R;
+  int rc = get_red_scratch(R, st);  // allocated before any capture
+  if (rc) return rc;
+  const size_t esz = (dtype == QSX_F32) ? 4 : (dtype == QSX_F64 || dtype == QSX_C64) ? 8 : 16;
+  auto qi = [&](int i) -> void* {
+    return (char*)Q + esz * (size_t)n * (size_t)(store_Q ? i : (i % 3));
+  };
+  auto apply = [&](const void* x, hipStream_t q) -> int {
+    if (op_kind == 0) return qsx_hv_ctx_apply(hv_ctx, x, r, 1.0, 0.0, 1.0, 0.0, 0, q);
+    return qsx_csr_matvec(indptr, indices, data, idx64, mat_dtype, n, nnz, x, r, dtype, 1, 1.0, 0.0, 0, group, q);
+  };
+  auto record = [&](hipStream_t q) -> int {
+    int e = apply(qi(0), q);
+    if (e) return e;
+    e = qsx_lanczos_update(dtype, nullptr, nullptr, qi(0), r, n, ab, ab + 2, q);
+    if (e) return e;
+    for (int i = 1; i < m; ++i) {
+      double* bp = ab + 4 * (int64_t)(i - 1) + 2;
+      e = qsx_lanczos_div(dtype, r, bp, qi(i), n, q);
+      if (e) return e;
+      e = apply(qi(i), q);
+      if (e) return e;
+      e = qsx_lanczos_update(dtype, qi(i - 1), bp, qi(i), r, n, ab + 4 * (int64_t)i, ab + 4 * (int64_t)i + 2, q);
+      if (e) return e;
+    }
+    return QSX_OK;
+  };
+  if (!use_graph) return record(st);
+  const LzKey key{hv_ctx, indptr, data, Q, r, ab, n, m, store_Q, dtype, op_kind};
+  auto it = g_lz_graphs.find(key);
+  if (it == g_lz_graphs.end()) {
+    if (!g_cap_stream) QSX_CHECK_HIP(hipStreamCreateWithFlags(&g_cap_stream, hipStreamNonBlocking));
+    QSX_CHECK_HIP(hipStreamBeginCapture(g_cap_stream, hipStreamCaptureModeThreadLocal));
+    rc = record(g_cap_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(g_cap_stream, &g);
+    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+    if (e != hipSuccess) { qsx_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) { qsx_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
+    it = g_lz_graphs.emplace(key, ex).first;
   }
   QSX_CHECK_HIP(hipGraphLaunch(it->second, st));
   return QSX_OK;

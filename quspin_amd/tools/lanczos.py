@@ -52,6 +52,25 @@ def _device_loop(op, v0, m, dtype, eps, store_Q, Q_out=None):
         qbuf = None
     r = L.empty(n, dtype)
     ab = L.zeros((m, 2, 2), np.float64)  # [i, 0] = a_i, [i, 1] = b_i (2 doubles each)
+    kind = _run_kind(op, dtype)
+    if kind is not None and m >= 8:
+        # the whole recurrence as one device launch sequence, captured once per
+        # operator and shape and replayed (the workspace keeps its addresses)
+        ws = _workspace(op, dtype, n, m, store_Q)
+        Qbuf, r, ab = ws["Q"], ws["r"], ws["ab"]
+        Qbuf[0].copy_(q0)
+        ctx, csr = kind
+        if csr is None:
+            args = (0, ctx.h, None, None, None, 0, 0, 0, 0)
+        else:
+            d = csr._data_for(dtype)
+            args = (1, None, ptr(csr.indptr), ptr(csr.indices), ptr(d), csr.idx64,
+                    L.dtype_code(L.np_dtype(d.dtype)), csr.nnz, 0)
+        rc = lib.qsx_lanczos_run(*args, code, n, m, ptr(Qbuf), int(bool(store_Q)), ptr(r), ptr(ab), 1, st)
+        L.check(rc, "qsx_lanczos_run")
+        if store_Q:
+            Q.copy_(Qbuf)
+        return _finish(ab, m, eps, Q)
 
     def qi(i):
         return qbuf[i] if store_Q else ring[i % 3]
@@ -66,6 +85,12 @@ def _device_loop(op, v0, m, dtype, eps, store_Q, Q_out=None):
         rc = lib.qsx_lanczos_update(code, ptr(qi(i - 1)), ptr(ab[i - 1, 1]), ptr(qi(i)), ptr(r), n,
                                     ptr(ab[i, 0]), ptr(ab[i, 1]), st)
         L.check(rc, "qsx_lanczos_update")
+    return _finish(ab, m, eps, Q)
+
+
+def _finish(ab, m, eps, Q):
+    """(a, b, effective m, Q) from the device alpha/beta array: the
+    recurrence stops at the first b_i < eps (_lanczos_utils.py:191-201)."""
     h = L.to_host(ab)
     a = h[:, 0, 0].copy()
     b = h[:, 1, 0].copy()
@@ -75,6 +100,29 @@ def _device_loop(op, v0, m, dtype, eps, store_Q, Q_out=None):
             m_eff = i
             break
     return a, b, m_eff, Q
+
+
+def _workspace(op, dtype, n, m, store_Q):
+    """Per-operator device buffers of the captured recurrence."""
+    cache = op.__dict__.setdefault("_lanczos_ws", {})
+    key = (np.dtype(dtype).str, n, m, bool(store_Q))
+    if key not in cache:
+        if len(cache) >= 4:
+            cache.clear()
+        cache[key] = {"Q": L.empty((m if store_Q else 3, n), dtype), "r": L.empty(n, dtype),
+                      "ab": L.zeros((m, 2, 2), np.float64)}
+    return cache[key]
+
+
+def _run_kind(op, dtype):
+    """(H.v context, None) or (None, csr) for the one-call device recurrence,
+    or None."""
+    from ..operators._devop import DeviceCSR, MatrixFreeOp
+    if isinstance(op, MatrixFreeOp):
+        return op._context(np.dtype(dtype), 1), None
+    if isinstance(op, DeviceCSR):
+        return None, op
+    return None
 
 
 def _host_loop(A, v0, m, dtype, eps):
