@@ -389,11 +389,18 @@ def run_learned(steps, L=14, dt=0.02):
     i0 = int(rng.integers(B.Ns))
     res = {}
     # fp32: complex64 matrix, Taylor terms and SpMV with the state accumulated in complex128
-    runs = (("learned_fp32", np.complex64, hop32, net, True), ("exact_fp32", np.complex64, hop32, None, False),
-            ("exact_fp64", np.complex128, hop64, None, False))
+    # "csr": H0 streamed as one CSR per Taylor term; "kron": H0 = H_up (x) 1 + 1 (x) H_down, psi read once per term
+    runs = (("learned_fp32", np.complex64, hop32, net, True, "csr"),
+            ("exact_fp32", np.complex64, hop32, None, False, "csr"),
+            ("exact_fp64", np.complex128, hop64, None, False, "csr"),
+            ("learned_fp32_kron", np.complex64, hop32, net, True, "kron"),
+            ("exact_fp32_kron", np.complex64, hop32, None, False, "kron"),
+            ("learned_fp64_kron", np.complex128, hop64, net, True, "kron"),
+            ("exact_fp64_kron", np.complex128, hop64, None, False, "kron"))
     psis = {}
-    for name, dt_, hop, pred, learned in runs:
-        ev = DrivenEvolution(hop, D, f, dt, dtype=dt_, phase=phase, predictor=pred, state_dtype=np.complex128)
+    for name, dt_, hop, pred, learned, form in runs:
+        ev = DrivenEvolution(hop, D, f, dt, dtype=dt_, phase=phase, predictor=pred, state_dtype=np.complex128,
+                             form=form)
         psi = torch.zeros(B.Ns, dtype=torch.complex128, device="cuda")
         psi[i0] = 1
         ev.run(psi, 0.0, min(steps, 100), learned=learned, block=100)  # graph capture / first launches
@@ -410,7 +417,7 @@ def run_learned(steps, L=14, dt=0.02):
         psis[name] = psi.to(torch.complex128)
         del ev
     p64 = psis["exact_fp64"]
-    for name in ("learned_fp32", "exact_fp32"):
+    for name in [r[0] for r in runs if r[0] != "exact_fp64"]:
         p = psis[name]
         d2 = vdot_dev(p, p).real + vdot_dev(p64, p64).real - 2 * vdot_dev(p, p64).real
         res[name]["state_err_vs_fp64"] = float(np.sqrt(max(d2, 0.0)))
@@ -423,6 +430,10 @@ def run_learned(steps, L=14, dt=0.02):
                                                       res["exact_fp64"]["steps_per_s"], 3)
     res["gain_learned_vs_exact_fp32"] = round(res["learned_fp32"]["steps_per_s"] /
                                               res["exact_fp32"]["steps_per_s"], 3)
+    res["speedup_learned_fp32_kron_vs_exact_fp64"] = round(res["learned_fp32_kron"]["steps_per_s"] /
+                                                           res["exact_fp64"]["steps_per_s"], 3)
+    res["gain_kron_vs_csr_learned_fp32"] = round(res["learned_fp32_kron"]["steps_per_s"] /
+                                                 res["learned_fp32"]["steps_per_s"], 3)
     return res
 
 

@@ -286,6 +286,41 @@ int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* da
                          void* ctl_work, int64_t ctl_bytes, int group, int use_graph, void* stream);
 void qsx_graph_cache_clear(void);
 
+/* ---- species-factorised operator (spinful fermions, no spatial symmetry) ---- */
+/* In the (N_up, N_down) sector of spinful_fermion_basis_general without
+ * symmetry maps the state index is u * Nd + d (basis_general/fermion.py:466-474 sizes the sector as the
+ * product C(N, N_up) C(N, N_down); up occupations are the high bits), and
+ * an operator whose off-diagonal terms each act on one species ('|' strings of
+ * hamiltonian's static list, hamiltonian_core.py:292-356) is
+ *     H = H_up (x) 1 + 1 (x) H_down + diag(sdiag) + xs diag(xdiag).
+ * The factors are device CSR matrices of the spinless sectors (the layout of
+ * qsx_csr_matvec, int32 column indices, shared dtype mat_dtype); the arrays
+ * stay owned by the caller.  The product is the same as hamiltonian.dot on
+ * the CSR of H (hamiltonian_core.py:615-789) with one read and one write of
+ * psi per row instead of the CSR stream. */
+typedef struct qsx_kron qsx_kron;
+int qsx_kron_create(const void* ip_u, const void* ind_u, const void* dat_u, int idx64_u, int64_t Nu, int64_t nnz_u,
+                    const void* ip_d, const void* ind_d, const void* dat_d, int idx64_d, int64_t Nd, int64_t nnz_d,
+                    const double* sdiag, int mat_dtype, qsx_kron** out);
+/* also drops every captured step sequence (qsx_graph_cache_clear) */
+void qsx_kron_destroy(qsx_kron* K);
+/* y = a (H_up (x) 1 + 1 (x) H_down + diag(sdiag) + xs diag(xdiag)) x
+ * (+ y if accumulate); sdiag (given at create: the static diagonal of the
+ * terms that act on both species) and xdiag: Nu*Nd device doubles or NULL. */
+int qsx_kron_matvec(const qsx_kron* K, const double* xdiag, double xs_re, double xs_im, const void* x, void* y,
+                    int vec_dtype, double a_re, double a_im, int accumulate, void* stream);
+/* ||H + diag(sdiag) - mu I + xs diag(xdiag)||_1 into out[0] (the norm LazyOperatorNormInfo
+ * needs, EMP.html:399-466): offu / offd are the factors' off-diagonal column
+ * 1-norms (device doubles), diag_u / diag_d their diagonals as complex
+ * (re, im) device doubles; work: 1024 device doubles. */
+int qsx_kron_1_norm(const qsx_kron* K, const double* offu, const double* diag_u, const double* offd,
+                    const double* diag_d, const double* xdiag, double xs_re, double xs_im, double mu_re, double mu_im,
+                    double* work, double* out, void* stream);
+/* qsx_evolve_csr_steps on the factorised operator (one vector). */
+int qsx_evolve_kron_steps(const qsx_kron* K, const double* xdiag, int nsteps, const void* es, int s_max, int m_max,
+                          void* v, void* work, int vec_dtype, int state_dtype, int* fail, void* ctl_work,
+                          int64_t ctl_bytes, int use_graph, void* stream);
+
 /* ---- partition predictor (MLP 4 -> 64 -> 64 -> 2, tanh, FP64) ------------------ */
 
 /* Replaces the per-step 1-norm estimate and _fragment_3_1 choice of

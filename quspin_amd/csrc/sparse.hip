@@ -42,6 +42,15 @@ __global__ void __launch_bounds__(256) k_part_reduce(const double* __restrict__ 
   }
 }
 
+// Fold the per-block norm partials of one expm launch (nblocks blocks) into
+// its norm slots; used by every expm term kernel.
+int qsx_part_reduce_launch(const CsrArgs& A, int64_t nblocks, hipStream_t st) {
+  hipLaunchKernelGGL(k_part_reduce, dim3(64), dim3(256), 0, st, A.part, nblocks, &A.ctl->done, A.normB, A.normF,
+                     A.nsp);
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
 // SpMV launches stride over rows with at most this many blocks (measured
 // faster than one block per 256/G rows on 10^7-row matrices; the per-block
 // norm partials of the expm form then fit a fixed buffer)
@@ -91,11 +100,7 @@ static int launch_csr(const CsrArgs& A, int mdt, int vdt, int G, hipStream_t st,
     return QSX_ERR_ARG;
   }
   QSX_LAUNCH_CHECK();
-  if constexpr (EPI) {
-    hipLaunchKernelGGL(k_part_reduce, dim3(64), dim3(256), 0, st, A.part, csr_blocks(A.Ns, G, true), &A.ctl->done,
-                       A.normB, A.normF, A.nsp);
-    QSX_LAUNCH_CHECK();
-  }
+  if constexpr (EPI) return qsx_part_reduce_launch(A, csr_blocks(A.Ns, G, true), st);
   return QSX_OK;
 }
 
@@ -857,22 +862,25 @@ extern "C" void qsx_graph_cache_clear() {
   g_lz_graphs.clear();
 }
 
-extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* data, int idx64,
-                                    int mat_dtype, int64_t Ns, int64_t nnz, const double* xdiag, int nsteps,
-                                    const void* es, int s_max, int m_max, void* v, void* work, int vec_dtype,
-                                    int state_dtype, int64_t nvec, int* fail, void* ctl_work, int64_t ctl_bytes,
-                                    int group, int use_graph, void* stream) {
-  hipStream_t st = (hipStream_t)stream;
+using TermFn = int (*)(const CsrArgs&, const void*, int, int, hipStream_t);
+
+// The time-step loop for any expm term kernel: for t = 0..nsteps-1, stage i,
+// term j: term(A) with A carrying vectors, record, slots and partials; the
+// term kernel writes its per-block norm partials (part_blocks blocks at most).
+int qsx_evolve_steps_with(int64_t Ns, int64_t nvec, const double* xdiag, int nsteps, const void* es, int s_max,
+                          int m_max, void* v, void* work, int vec_dtype, int state_dtype, int* fail, void* ctl_work,
+                          int64_t ctl_bytes, int64_t part_blocks, TermFn term, const void* term_ctx,
+                          const void* const key_ptr[3], int key_int, int use_graph, hipStream_t st) {
   if (Ns < 0 || nsteps < 1 || !es || s_max < 1 || m_max < 1 || !v || !work || !fail || !ctl_work || nvec < 1 ||
-      ctl_bytes < qsx_evolve_csr_ctl_bytes(s_max, m_max)) {
-    qsx_set_error("qsx_evolve_csr_steps: bad arguments");
+      ctl_bytes < qsx_evolve_csr_ctl_bytes(s_max, m_max) || part_blocks > EPI_MAX_BLOCKS) {
+    qsx_set_error("qsx_evolve_steps: bad arguments");
     return QSX_ERR_ARG;
   }
   if (Ns == 0) return QSX_OK;
   if (state_dtype < 0) state_dtype = vec_dtype;
   const bool mixed = state_dtype != vec_dtype;
   if (mixed && !((vec_dtype == QSX_C64 && state_dtype == QSX_C128) || (vec_dtype == QSX_F32 && state_dtype == QSX_F64))) {
-    qsx_set_error("qsx_evolve_csr_steps: the state dtype must be the term dtype or its FP64 counterpart");
+    qsx_set_error("qsx_evolve_steps: the state dtype must be the term dtype or its FP64 counterpart");
     return QSX_ERR_ARG;
   }
   ExpmCtl* ctl = (ExpmCtl*)ctl_work;
@@ -884,7 +892,6 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
   double* normF = slots + (int64_t)s_max * stride;
   size_t esz = (vec_dtype == QSX_F32) ? 4 : (vec_dtype == QSX_F64 || vec_dtype == QSX_C64) ? 8 : 16;
   void* Bb[2] = {work, (char*)work + esz * Ns * nvec};
-  const int G = group > 0 ? group : auto_group(Ns, nnz);
   const unsigned rg = (unsigned)red_grid(Ns) * 4;
   const ExpmSec* E = (const ExpmSec*)es;
 
@@ -908,7 +915,8 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
         for (int j = 0; j < m_max; ++j) {
           CsrArgs A;
           std::memset(&A, 0, sizeof(A));
-          A.indptr = indptr; A.indices = indices; A.data = data; A.idx64 = idx64; A.Ns = Ns; A.nvec = nvec;
+          A.Ns = Ns;
+          A.nvec = nvec;
           A.x = Bb[j & 1]; A.y = Bb[(j + 1) & 1];
           A.F = v;
           A.ctl = ctl;
@@ -925,7 +933,7 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
             A.chk_c2 = normB + i * stride + j * nsp;
             A.chk_F = normF + i * stride + j * nsp;
           }
-          int rc = launch_csr<true>(A, mat_dtype, vec_dtype, G, q, state_dtype);
+          int rc = term(A, term_ctx, vec_dtype, state_dtype, q);
           if (rc) return rc;
         }
         hipLaunchKernelGGL(k_expm_tail, dim3(1), dim3(1), 0, q, R, i, ctl, normB + i * stride, normF + i * stride,
@@ -948,8 +956,10 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
     return QSX_OK;
   };
   if (!use_graph) return record(st);
-  const StepKey key{indptr, indices, data, xdiag, es, v, work, fail, ctl_work, Ns, nvec, nsteps, s_max, m_max, G,
-                    mat_dtype, vec_dtype * 8 + state_dtype};
+  // key_ptr names the operator's arrays (the cache is keyed on addresses; it
+  // is cleared with qsx_graph_cache_clear when those are freed).
+  const StepKey key{key_ptr[0], key_ptr[1], key_ptr[2], xdiag, es, v, work, fail, ctl_work, Ns, nvec, nsteps, s_max,
+                    m_max, key_int, 0, vec_dtype * 8 + state_dtype};
   auto it = g_step_graphs.find(key);
   if (it == g_step_graphs.end()) {
     if (!g_cap_stream) QSX_CHECK_HIP(hipStreamCreateWithFlags(&g_cap_stream, hipStreamNonBlocking));
@@ -967,6 +977,39 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
   }
   QSX_CHECK_HIP(hipGraphLaunch(it->second, st));
   return QSX_OK;
+}
+
+namespace {
+struct CsrTermCtx {
+  const void* indptr;
+  const void* indices;
+  const void* data;
+  int idx64, mat_dtype, G;
+  int64_t nnz;
+};
+
+int csr_term(const CsrArgs& A0, const void* ctx, int vdt, int fdt, hipStream_t q) {
+  const CsrTermCtx* c = (const CsrTermCtx*)ctx;
+  CsrArgs A = A0;
+  A.indptr = c->indptr;
+  A.indices = c->indices;
+  A.data = c->data;
+  A.idx64 = c->idx64;
+  return launch_csr<true>(A, c->mat_dtype, vdt, c->G, q, fdt);
+}
+}  // namespace
+
+extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* data, int idx64,
+                                    int mat_dtype, int64_t Ns, int64_t nnz, const double* xdiag, int nsteps,
+                                    const void* es, int s_max, int m_max, void* v, void* work, int vec_dtype,
+                                    int state_dtype, int64_t nvec, int* fail, void* ctl_work, int64_t ctl_bytes,
+                                    int group, int use_graph, void* stream) {
+  if (Ns > 0 && (!indptr || !indices || !data)) { qsx_set_error("qsx_evolve_csr_steps: bad arguments"); return QSX_ERR_ARG; }
+  CsrTermCtx c{indptr, indices, data, idx64, mat_dtype, group > 0 ? group : auto_group(Ns, nnz), nnz};
+  const void* key[3] = {indptr, indices, data};
+  return qsx_evolve_steps_with(Ns, nvec, xdiag, nsteps, es, s_max, m_max, v, work, vec_dtype, state_dtype, fail,
+                               ctl_work, ctl_bytes, csr_blocks(Ns, c.G, true), csr_term, &c, key,
+                               c.G * 16 + mat_dtype, use_graph, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -89,31 +89,93 @@ __device__ __forceinline__ int64_t ld_idx(const void* p, int idx64, int64_t i) {
 // G lanes per row.  EPI: fused expm epilogue.
 // FT: element type of F (the accumulated Taylor sum) in the expm epilogue;
 // FP64 F with FP32 terms is the mixed-precision Taylor loop.
+// Expm launch prologue shared by the SpMV kernels: every control value is
+// read in one round (no load waits on another).  Returns false when the launch
+// has nothing to do (term outside the record's partition, series finished,
+// or the truncation test of the previous term passes now); otherwise sets
+// the term's coefficient a_t, shift mu_t and extra-diagonal factor xs.
+__device__ __forceinline__ bool expm_term_prologue(const CsrArgs& A, double2& a_t, double2& mu_t, double2& xs) {
+  const int done = A.ctl->done;
+  double tol = A.ctl->tol;
+  ExpmSec E{};
+  if (A.rec) E = *A.rec;
+  double c1 = 0.0, c2 = 0.0, nf = 0.0;
+  if (A.chk_c1) { c1 = slot_max(A.chk_c1, A.nsp); c2 = slot_max(A.chk_c2, A.nsp); nf = slot_max(A.chk_F, A.nsp); }
+  if (A.rec) {
+    if (A.stage >= E.s || A.term >= E.m) return false;
+    const double cf = 1.0 / ((double)E.s * (double)(A.term + 1));
+    a_t = make_double2(E.a.x * cf, E.a.y * cf);
+    mu_t = E.mu;
+    xs = E.dscale;
+    tol = E.tol;
+  }
+  if (done) return false;
+  // truncation test of the previous term (SCIPY:298-300)
+  if (A.chk_c1 && c1 + c2 <= tol * nf) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl->done = 1;
+    return false;
+  }
+  return true;
+}
+
+// Output of one element of an expm term or a plain product:
+// res = a_t (acc - (mu - xs d) x_r) (+ y);  with EPI F += rounded res and
+// the row sums of |B|, |F| accumulate into curB, curF.
+template <typename VT, bool EPI, typename FT>
+__device__ __forceinline__ void spmv_out(const CsrArgs& A, int64_t o, int64_t row, double2 acc, double2 xr,
+                                         double2 a_t, double2 mu_t, double2 xs, double& curB, double& curF) {
+  constexpr bool CPLX = VTr<VT>::cplx;
+  VT* __restrict__ y = (VT*)A.y;
+  double2 mu_r = mu_t;
+  if (A.xdiag) {
+    // the extra diagonal enters as a row-dependent shift
+    const double d = A.xdiag[row];
+    mu_r = make_double2(mu_r.x - xs.x * d, mu_r.y - xs.y * d);
+  }
+  double2 res;
+  if constexpr (CPLX) {
+    acc = cadd(acc, make_double2(-(mu_r.x * xr.x - mu_r.y * xr.y), -(mu_r.x * xr.y + mu_r.y * xr.x)));
+    res = cmul(a_t, acc);
+  } else {
+    res = make_double2(a_t.x * (acc.x - mu_r.x * xr.x), 0.0);
+  }
+  if (A.beta) res = cadd(res, VTr<VT>::ld(y + o));
+  VTr<VT>::st(y + o, res);
+  if constexpr (EPI) {
+    FT* F = (FT*)A.F;
+    // the stored (rounded) B enters F, as in the reference
+    const double2 rs = VTr<VT>::rnd(res);
+    const double2 fs = VTr<FT>::rnd(cadd(VTr<FT>::ld(F + o), rs));
+    VTr<FT>::st(F + o, fs);
+    curB += CPLX ? cabs2(rs) : fabs(rs.x);
+    curF += CPLX ? cabs2(fs) : fabs(fs.x);
+  }
+}
+
+// Block max of the per-thread row-sum maxima into part[blockIdx] (two
+// doubles); k_part_reduce folds the partials of a launch into the slots.
+__device__ __forceinline__ void block_norm_partials(const CsrArgs& A, double mb, double mf) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    mb = fmax(mb, __shfl_xor(mb, o, 64));
+    mf = fmax(mf, __shfl_xor(mf, o, 64));
+  }
+  __shared__ double sw[2][16];
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sw[0][w] = mb; sw[1][w] = mf; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) { mb = fmax(mb, sw[0][i]); mf = fmax(mf, sw[1][i]); }
+    A.part[2 * (int64_t)blockIdx.x] = mb;
+    A.part[2 * (int64_t)blockIdx.x + 1] = mf;
+  }
+}
+
 template <typename MT, typename VT, int G, bool EPI, typename FT = VT>
 __global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
   double2 a_t = A.a, mu_t = A.mu, xs = A.xdscale;
   if constexpr (EPI) {
-    // every control value is read in one round (no load waits on another)
-    const int done = A.ctl->done;
-    double tol = A.ctl->tol;
-    ExpmSec E{};
-    if (A.rec) E = *A.rec;
-    double c1 = 0.0, c2 = 0.0, nf = 0.0;
-    if (A.chk_c1) { c1 = slot_max(A.chk_c1, A.nsp); c2 = slot_max(A.chk_c2, A.nsp); nf = slot_max(A.chk_F, A.nsp); }
-    if (A.rec) {
-      if (A.stage >= E.s || A.term >= E.m) return;
-      const double cf = 1.0 / ((double)E.s * (double)(A.term + 1));
-      a_t = make_double2(E.a.x * cf, E.a.y * cf);
-      mu_t = E.mu;
-      xs = E.dscale;
-      tol = E.tol;
-    }
-    if (done) return;
-    // truncation test of the previous term (SCIPY:298-300)
-    if (A.chk_c1 && c1 + c2 <= tol * nf) {
-      if (blockIdx.x == 0 && threadIdx.x == 0) A.ctl->done = 1;
-      return;
-    }
+    if (!expm_term_prologue(A, a_t, mu_t, xs)) return;
   }
   constexpr bool CPLX = VTr<VT>::cplx;
   const int lane = threadIdx.x % G;
@@ -125,7 +187,6 @@ __global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
     const int64_t k0 = ld_idx(A.indptr, A.idx64, row), k1 = ld_idx(A.indptr, A.idx64, row + 1);
     const MT* __restrict__ val = (const MT*)A.data;
     const VT* __restrict__ x = (const VT*)A.x;
-    VT* __restrict__ y = (VT*)A.y;
     double curB = 0.0, curF = 0.0;
     for (int64_t j = 0; j < A.nvec; ++j) {
       double sr = 0.0, si = 0.0;
@@ -140,55 +201,11 @@ __global__ void __launch_bounds__(256) k_csr_spmv(CsrArgs A) {
       if constexpr (CPLX) si = grp_sum<G>(si);
       if (lane == 0) {
         const int64_t o = row * A.nvec + j;
-        double2 acc = make_double2(sr, si);
-        const double2 xr = VTr<VT>::ld(x + o);
-        double2 mu_r = mu_t;
-        if (A.xdiag) {
-          // the extra diagonal enters as a row-dependent shift
-          const double d = A.xdiag[row];
-          mu_r = make_double2(mu_r.x - xs.x * d, mu_r.y - xs.y * d);
-        }
-        double2 res;
-        if constexpr (CPLX) {
-          acc = cadd(acc, make_double2(-(mu_r.x * xr.x - mu_r.y * xr.y), -(mu_r.x * xr.y + mu_r.y * xr.x)));
-          res = cmul(a_t, acc);
-        } else {
-          res = make_double2(a_t.x * (acc.x - mu_r.x * xr.x), 0.0);
-        }
-        if (A.beta) res = cadd(res, VTr<VT>::ld(y + o));
-        VTr<VT>::st(y + o, res);
-        if constexpr (EPI) {
-          FT* F = (FT*)A.F;
-          // the stored (rounded) B enters F, as in the reference
-          const double2 rs = VTr<VT>::rnd(res);
-          const double2 fs = VTr<FT>::rnd(cadd(VTr<FT>::ld(F + o), rs));
-          VTr<FT>::st(F + o, fs);
-          curB += CPLX ? cabs2(rs) : fabs(rs.x);
-          curF += CPLX ? cabs2(fs) : fabs(fs.x);
-        }
+        spmv_out<VT, EPI, FT>(A, o, row, make_double2(sr, si), VTr<VT>::ld(x + o), a_t, mu_t, xs, curB, curF);
       }
     }
     rowB = fmax(rowB, curB);
     rowF = fmax(rowF, curF);
   }
-  if constexpr (EPI) {
-    // block max of the row sums (wavefront shuffles, then the 4 wavefronts),
-    // one atomic per block and slot
-    double mb = rowB, mf = rowF;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      mb = fmax(mb, __shfl_xor(mb, o, 64));
-      mf = fmax(mf, __shfl_xor(mf, o, 64));
-    }
-    __shared__ double sw[2][4];
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) { sw[0][w] = mb; sw[1][w] = mf; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int i = 1; i < (int)(blockDim.x >> 6); ++i) { mb = fmax(mb, sw[0][i]); mf = fmax(mf, sw[1][i]); }
-      A.part[2 * (int64_t)blockIdx.x] = mb;
-      A.part[2 * (int64_t)blockIdx.x + 1] = mf;
-    }
-  }
+  if constexpr (EPI) block_norm_partials(A, rowB, rowF);
 }
-

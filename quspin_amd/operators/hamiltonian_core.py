@@ -74,6 +74,8 @@ class hamiltonian:
             M = sp.csr_matrix(M, dtype=self._dtype)
             host_static = M if host_static is None else host_static + M
         self._static = self._assemble(_consolidate_static(static_opstr), host_static)
+        # the term list of an operator-string-only static part (the species-factorised form reads it)
+        self._static_list = _consolidate_static(static_opstr) if not static_other else None
         # dynamic parts: one CSR per (f, f_args)
         self._dynamic = {}
         groups = _consolidate_dynamic(dyn_opstr)
@@ -112,6 +114,7 @@ class hamiltonian:
         H._shape = tuple(shape)
         H._basis = basis
         H._basis_for_build = basis
+        H._static_list = None
         return H
 
     # ------------------------------------------------------------ properties

@@ -173,12 +173,18 @@ class DrivenEvolution:
     H0 + f(t_mid) diag(D) through the CSR Taylor kernels.  ``dtype`` is the
     dtype of the Taylor terms and the SpMV; ``state_dtype`` (default: the
     same) that of the state, in which the terms are accumulated: complex128
-    with complex64 terms is the FP32 SpMV / FP64 accumulation form."""
+    with complex64 terms is the FP32 SpMV / FP64 accumulation form.
+    ``form``: "csr" (H0 assembled as one CSR) or "kron" (H0 on a spinful
+    fermion sector without symmetry maps as H_up (x) 1 + 1 (x) H_down plus a
+    static diagonal: the factorised product reads psi once per term instead
+    of streaming the CSR of H0)."""
 
     def __init__(self, H0, D_op, f, dt, dtype=np.complex64, phase=None, predictor=None, m_margin=2, m_cap=60,
-                 s_cap=8, group=0, state_dtype=None):
+                 s_cap=8, group=0, state_dtype=None, form="csr"):
         _check_record()
-        op = as_device_operator(H0)
+        if form not in ("csr", "kron"):
+            raise ValueError("form must be 'csr' or 'kron'")
+        self.form = form
         dop = as_device_operator(D_op)
         if not isinstance(dop, MatrixFreeOp):
             raise TypeError("D_op must be a matrix-free operator (quantum_LinearOperator)")
@@ -191,17 +197,25 @@ class DrivenEvolution:
         self.tol = float(np.finfo(self.vdt).eps / 2)
         self.predictor = predictor
         self.m_margin, self.m_cap, self.s_cap, self.group = m_margin, m_cap, s_cap, group
-        self.csr = self._assemble(op)
-        self.N = self.csr.shape[0]
-        self.trH0 = complex(self.csr.trace())
+        if form == "kron":
+            self.csr, self.kron = None, self._assemble_kron(H0)
+            self.N = self.kron.N
+            self.trH0 = complex(self.kron.trace())
+            self.normH0 = self.kron.one_norm(0.0)
+        else:
+            self.kron, self.csr = None, self._assemble(as_device_operator(H0))
+            self.N = self.csr.shape[0]
+            self.trH0 = complex(self.csr.trace())
+            self.normH0 = self.csr.one_norm(0.0)
+        if self.D.numel() != self.N:
+            raise ValueError("D_op and H0 act on different spaces")
         ones = L.torch().ones(self.N, dtype=L.torch().float64, device="cuda")
         self.trD = vdot_dev(ones, self.D).real
         self.maxD = float(np.abs(L.to_host(self.D)).max()) if self.N else 0.0
-        self.normH0 = self.csr.one_norm(0.0)
         self._colsum = L.empty(self.N, np.float64)
         self._nout = L.empty(1, np.float64)
         self._ctl = {}
-        if not group:
+        if not group and form == "csr":
             self.group = self._tune_group()
         self.stats = {"norm_passes": 0, "learned_steps": 0, "exact_steps": 0, "fallback_blocks": 0}
 
@@ -221,6 +235,18 @@ class DrivenEvolution:
                 continue
         raise TypeError("H0 cannot be assembled in {}".format(self.vdt))
 
+    def _assemble_kron(self, H0):
+        """H0 as species factors: real factors when H0 is real."""
+        from ..operators._kron import KronOp
+        real = np.dtype(np.float32 if self.vdt == np.complex64 else np.float64)
+        if self.vdt.kind == "c":
+            try:
+                return KronOp(H0, real)
+            except TypeError as e:
+                if "real" not in str(e):
+                    raise
+        return KronOp(H0, self.vdt)
+
     # --------------------------------------------------------------- per t
     def mu(self, t):
         return (self.trH0 + self.f(t) * self.trD) / self.N
@@ -232,12 +258,14 @@ class DrivenEvolution:
     def norm_1(self, t):
         """||H(t) - mu(t) I||_1 (one device pass over the CSR of H0)."""
         mu = self.mu(t)
+        self.stats["norm_passes"] += 1
+        if self.kron is not None:
+            return self.kron.one_norm(-mu, self.D, float(self.f(t)))
         c = self.csr
         L.check(L.load().qsx_csr_1_norm_diag(ptr(c.indptr), ptr(c.indices), ptr(c.data), c.idx64,
                                              L.dtype_code(c.dtype), self.N, -mu.real, -mu.imag, ptr(self.D),
                                              float(self.f(t)), 0.0, ptr(self._colsum), ptr(self._nout),
                                              stream_handle()), "qsx_csr_1_norm_diag")
-        self.stats["norm_passes"] += 1
         return float(L.to_host(self._nout)[0])
 
     def exact(self, t):
@@ -263,8 +291,11 @@ class DrivenEvolution:
         return self._ctl[key]
 
     def _steps(self, psi, work, rec, nsteps, s_max, m_max, fail, graph):
-        c = self.csr
         ctl = self._ctl_buf(s_max, m_max)
+        if self.kron is not None:
+            self.kron.evolve_steps(self.D, rec, nsteps, s_max, m_max, psi, work, self.vdt, self.sdt, fail, ctl, graph)
+            return
+        c = self.csr
         L.check(L.load().qsx_evolve_csr_steps(
             ptr(c.indptr), ptr(c.indices), ptr(c.data), c.idx64, L.dtype_code(c.dtype), self.N, c.nnz, ptr(self.D),
             int(nsteps), ptr(rec), int(s_max), int(m_max), ptr(psi), ptr(work), L.dtype_code(self.vdt),
