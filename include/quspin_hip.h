@@ -304,6 +304,11 @@ int qsx_kron_create(const void* ip_u, const void* ind_u, const void* dat_u, int 
                     const double* sdiag, int mat_dtype, qsx_kron** out);
 /* also drops every captured step sequence (qsx_graph_cache_clear) */
 void qsx_kron_destroy(qsx_kron* K);
+/* Kernel form: 0 by size (default: one pass while psi fits in L2, else a
+ * row pass over H_down and an L2-resident column-strip pass over H_up), 1 one
+ * pass, 2 two passes.  The two-pass form allocates an N_u*N_d-element buffer
+ * on its first product or step sequence (outside any capture). */
+int qsx_kron_set_mode(qsx_kron* K, int mode);
 /* y = a (H_up (x) 1 + 1 (x) H_down + diag(sdiag) + xs diag(xdiag)) x
  * (+ y if accumulate); sdiag (given at create: the static diagonal of the
  * terms that act on both species) and xdiag: Nu*Nd device doubles or NULL. */

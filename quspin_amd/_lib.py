@@ -100,6 +100,7 @@ SIGNATURES = {
     "qsx_kron_create": (_i32, [_vp, _vp, _vp, _i32, _i64, _i64, _vp, _vp, _vp, _i32, _i64, _i64, _vp, _i32,
                                C.POINTER(_vp)]),
     "qsx_kron_destroy": (None, [_vp]),
+    "qsx_kron_set_mode": (_i32, [_vp, _i32]),
     "qsx_kron_matvec": (_i32, [_vp, _vp, _dbl, _dbl, _vp, _vp, _i32, _dbl, _dbl, _i32, _vp]),
     "qsx_kron_1_norm": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _vp]),
     "qsx_evolve_kron_steps": (_i32, [_vp, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _i32, _i32, _vp, _vp, _i64, _i32,

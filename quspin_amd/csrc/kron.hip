@@ -7,17 +7,29 @@
 //     H = H_up (x) 1 + 1 (x) H_down + diag,
 // with H_up / H_down the CSR matrices of the same terms in the spinless
 // sectors (the Jordan-Wigner string of a same-species pair of operators only
-// crosses modes of that species).  One block takes one up row u of psi
-// (viewed as an N_u x N_d matrix): the row is staged in LDS once and serves
-// every H_down product of the row; the H_up product reads the partner rows
-// u' coalesced along d.  Same products as the CSR of H
-// (hamiltonian.dot, hamiltonian_core.py:615-789), with per-row HBM traffic
-// of one read and one write instead of the CSR stream.
+// crosses modes of that species).  Viewing psi as an N_u x N_d matrix X,
+// H psi = H_up X + X H_down^T + diag: the down part combines entries of one
+// row, the up part entries of one column.  Same products as the CSR of H
+// (hamiltonian.dot, hamiltonian_core.py:615-789).
+//
+// Two forms:
+//  * one pass (small sectors): a block per up row stages the row in LDS for
+//    the down part and reads the partner rows for the up part; the partner
+//    rows come back from L2 only while the whole vector fits there;
+//  * two passes (large sectors): a row pass Z = X H_down^T (row staged in
+//    LDS, H_down in a sliced-ELL layout with coalesced entries), then a
+//    column-strip pass Y = Z + H_up X + diag over strips of 64 columns, the
+//    blocks of one XCD walking a contiguous run of strips so that the strip
+//    (N_u x 64 elements) stays in that XCD's L2 while its partner rows are
+//    read: each element of X comes from HBM about twice per product instead
+//    of once per partner row.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
 #include <string>
 #include <tuple>
+#include <vector>
 
 #include "sparse_impl.h"
 
@@ -26,6 +38,15 @@ struct qsx_kron {
   const void* ip_d; const int32_t* ind_d; const void* dat_d; int idx64_d; int64_t Nd; int64_t nnz_d;
   const double* sdiag;  // static diagonal (terms acting on both species), or null
   int mat_dtype;
+  // H_down in sliced ELL: chunks of 64 rows; entry j of row d at
+  // sell_off[d / 64] + 64 j + d % 64 (padding: column 0, value 0)
+  int64_t* sell_off = nullptr;
+  int32_t* sell_idx = nullptr;
+  void* sell_val = nullptr;
+  // row-pass output of the two-pass form (allocated on first use)
+  mutable void* z = nullptr;
+  mutable size_t z_bytes = 0;
+  int mode = 0;  // 0: by size, 1: one pass, 2: two passes
 };
 
 struct KronArgs {
@@ -33,10 +54,36 @@ struct KronArgs {
   const void* ip_u; const int32_t* ind_u; const void* dat_u; int idx64_u; int64_t Nu;
   const void* ip_d; const int32_t* ind_d; const void* dat_d; int idx64_d; int64_t Nd;
   const double* sdiag;
+  const int64_t* sell_off; const int32_t* sell_idx; const void* sell_val;
+  void* z;
 };
 
 constexpr int64_t KRON_MAX_BLOCKS = 65536;
 constexpr int64_t KRON_STAGE_BYTES = 64 * 1024;
+
+__device__ __forceinline__ int64_t kr_uniform64(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffffll));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+// lane t's double (t wave-uniform)
+__device__ __forceinline__ double rl_f64(double x, int t) {
+  const int64_t b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), t);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), t);
+  return __longlong_as_double((int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+// acc += m v for a matrix entry m and a vector element v (a real matrix
+// entry multiplies both parts of a complex element; complex entries only
+// meet complex vectors)
+template <typename MT, typename VT>
+__device__ __forceinline__ void kr_mac(double& sr, double& si, double2 m, double2 v) {
+  if constexpr (MTr<MT>::cplx) { sr += m.x * v.x - m.y * v.y; si += m.x * v.y + m.y * v.x; }
+  else if constexpr (VTr<VT>::cplx) { sr += m.x * v.x; si += m.x * v.y; }
+  else sr += m.x * v.x;
+}
 
 // STAGE: the up row of psi is staged in LDS (N_d * sizeof(VT) bytes).
 template <typename MT, typename VT, bool EPI, typename FT, bool STAGE>
@@ -48,7 +95,6 @@ __global__ void __launch_bounds__(256) k_kron(KronArgs K) {
   if constexpr (EPI) {
     if (!expm_term_prologue(A, a_t, mu_t, xs)) return;
   }
-  constexpr bool CPLX = VTr<VT>::cplx;
   const VT* __restrict__ X = (const VT*)A.x;
   const MT* __restrict__ du = (const MT*)K.dat_u;
   const MT* __restrict__ dd = (const MT*)K.dat_d;
@@ -69,16 +115,14 @@ __global__ void __launch_bounds__(256) k_kron(KronArgs K) {
         const int32_t c = K.ind_d[k];
         const double2 m = MTr<MT>::ld(dd + k);
         const double2 v = STAGE ? VTr<VT>::ld(srow + c) : VTr<VT>::ld(xu + c);
-        if constexpr (CPLX) { sr += m.x * v.x - m.y * v.y; si += m.x * v.y + m.y * v.x; }
-        else sr += m.x * v.x;
+        kr_mac<MT, VT>(sr, si, m, v);
       }
       // H_up (x) 1: partner rows u', same column d (coalesced along d)
       for (int64_t k = ku0; k < ku1; ++k) {
         const int64_t c = (int64_t)K.ind_u[k];
         const double2 m = MTr<MT>::ld(du + k);
         const double2 v = VTr<VT>::ld(X + c * Nd + d);
-        if constexpr (CPLX) { sr += m.x * v.x - m.y * v.y; si += m.x * v.y + m.y * v.x; }
-        else sr += m.x * v.x;
+        kr_mac<MT, VT>(sr, si, m, v);
       }
       const int64_t row = u * Nd + d;
       const double2 xr = STAGE ? VTr<VT>::ld(srow + d) : VTr<VT>::ld(xu + d);
@@ -97,19 +141,265 @@ __global__ void __launch_bounds__(256) k_kron(KronArgs K) {
   if constexpr (EPI) block_norm_partials(A, maxB, maxF);
 }
 
-static int64_t kron_blocks(const qsx_kron* K) {
-  const int64_t nb = K->Nu < KRON_MAX_BLOCKS ? K->Nu : KRON_MAX_BLOCKS;
+// Row pass of the two-pass form: Z[u, :] = X[u, :] H_down^T.  A block stages
+// R up rows in LDS (grid-stride over row groups); each wave takes whole
+// 64-row chunks of H_down, so the chunk's width and offset are wave-uniform,
+// the sliced-ELL entries load coalesced, and each entry serves the R rows.
+// With EPI the launch runs the term's prologue, so that a finished series
+// skips both passes.
+template <typename MT, typename VT, bool EPI, bool STAGE, int R>
+__global__ void __launch_bounds__(256) k_kron_down(KronArgs K) {
+  static_assert(STAGE || R == 1, "direct reads take one row at a time");
+  extern __shared__ unsigned char kron_smem[];
+  VT* srow = (VT*)kron_smem;
+  if constexpr (EPI) {
+    double2 a_t, mu_t, xs;
+    if (!expm_term_prologue(K.A, a_t, mu_t, xs)) return;
+  }
+  const VT* __restrict__ X = (const VT*)K.A.x;
+  VT* __restrict__ Z = (VT*)K.z;
+  const MT* __restrict__ sv = (const MT*)K.sell_val;
+  const int32_t* __restrict__ sx = K.sell_idx;
+  const int64_t Nd = K.Nd, Nu = K.Nu;
+  const int lane = threadIdx.x & 63;
+  const int64_t w0 = (int64_t)(threadIdx.x >> 6) * 64;
+  for (int64_t u0 = (int64_t)blockIdx.x * R; u0 < Nu; u0 += (int64_t)gridDim.x * R) {
+    const int nr = (int)(Nu - u0 < R ? Nu - u0 : R);
+    const VT* __restrict__ xu = X + u0 * Nd;
+    if constexpr (STAGE) {
+      for (int64_t i = threadIdx.x; i < nr * Nd; i += blockDim.x) srow[i] = xu[i];
+      __syncthreads();
+    }
+    for (int64_t d0 = w0; d0 < Nd; d0 += blockDim.x) {
+      const int64_t ch = d0 >> 6;
+      const int64_t off = kr_uniform64(K.sell_off[ch]);
+      const int64_t wid = (kr_uniform64(K.sell_off[ch + 1]) - off) >> 6;
+      const int32_t* __restrict__ ci = sx + off + lane;
+      const MT* __restrict__ cv = sv + off + lane;
+      double sr[R], si[R];
+#pragma unroll
+      for (int r = 0; r < R; ++r) { sr[r] = 0.0; si[r] = 0.0; }
+      // entries in batches of four: the batch's index and value loads are
+      // issued before its gathers (the width is a multiple of nothing, so a
+      // batch past the width reads entry 0 with weight 0)
+      for (int64_t j = 0; j < wid; j += 4) {
+        int32_t c[4];
+        double2 m[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int64_t jq = j + q < wid ? j + q : 0;
+          c[q] = ci[jq * 64];
+          m[q] = MTr<MT>::ld(cv + jq * 64);
+          if (j + q >= wid) m[q] = make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            // rows past nr read a staged row of this group (discarded)
+            const int rr = r < nr ? r : 0;
+            const double2 v = STAGE ? VTr<VT>::ld(srow + rr * Nd + c[q]) : VTr<VT>::ld(xu + c[q]);
+            kr_mac<MT, VT>(sr[r], si[r], m[q], v);
+          }
+        }
+      }
+      const int64_t d = d0 + lane;
+      if (d < Nd) {
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (r < nr) VTr<VT>::st(Z + (u0 + r) * Nd + d, make_double2(sr[r], si[r]));
+      }
+    }
+    if constexpr (STAGE) __syncthreads();
+  }
+}
+
+// Block index with the blocks of each XCD (workgroups are dispatched
+// round-robin over the 8 XCDs) renumbered into one contiguous run.
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t nb) {
+  const int64_t x = b & 7, pos = b >> 3, q = nb >> 3, r = nb & 7;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + pos;
+}
+
+constexpr int KRON_UP_ROWS = 4;  // up rows per wave in the column-strip pass
+
+// Column-strip pass: Y = a_t (Z + H_up X - (mu - diag) X) with the Taylor
+// epilogue.  Work unit = (strip s of 64 columns, group g of 16 up rows), in
+// strip-major order: wave w takes rows u0 .. u0 + 3 (u0 = 16 g + 4 w; the
+// H_up rows are wave-uniform, read through the scalar cache) and lane l
+// column d = 64 s + l, so every partner-row read is one coalesced segment of
+// the strip; the partner reads of the four rows are issued together.
+template <typename MT, typename VT, bool EPI, typename FT>
+__global__ void __launch_bounds__(256) k_kron_up(KronArgs K, int64_t nunits, int64_t groups,
+                                                 const int32_t* __restrict__ iu, const MT* __restrict__ du) {
+  constexpr int RW = KRON_UP_ROWS;
+  const CsrArgs& A = K.A;
+  double2 a_t = A.a, mu_t = A.mu, xs = A.xdscale;
+  if constexpr (EPI) {
+    if (!expm_term_prologue(A, a_t, mu_t, xs)) return;
+  }
+  const VT* __restrict__ X = (const VT*)A.x;
+  const VT* __restrict__ Z = (const VT*)K.z;
+  const int64_t Nd = K.Nd, Nu = K.Nu;
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double maxB = 0.0, maxF = 0.0;
+  for (int64_t unit = xcd_contiguous(blockIdx.x, gridDim.x); unit < nunits; unit += gridDim.x) {
+    const int64_t s = unit / groups;
+    const int64_t u0 = kr_uniform64(((unit - s * groups) * 4 + w) * RW);
+    if (u0 >= Nu) continue;
+    const int nr = (int)(Nu - u0 < RW ? Nu - u0 : RW);
+    const int64_t d = s * 64 + lane;
+    const bool dok = d < Nd;
+    const int64_t ds = dok ? d : Nd - 1;  // lanes past the last column read in bounds, store nothing
+    int64_t kb[RW + 1];
+#pragma unroll
+    for (int r = 0; r <= RW; ++r) kb[r] = kr_uniform64(ld_idx(K.ip_u, K.idx64_u, u0 + (r < nr ? r : nr)));
+    int64_t maxdeg = 0;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) maxdeg = kb[r + 1] - kb[r] > maxdeg ? kb[r + 1] - kb[r] : maxdeg;
+    double sr[RW], si[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const double2 z = r < nr ? VTr<VT>::ld(Z + (u0 + r) * Nd + ds) : make_double2(0.0, 0.0);
+      sr[r] = z.x;
+      si[r] = z.y;
+    }
+    const VT* __restrict__ xd = X + ds;
+    const int64_t ne = kb[RW] - kb[0];  // the rows' entries are contiguous
+    if (ne <= 64) {
+      // one lane-parallel load of all entries, broadcast with readlane, so the
+      // only round trips of the loop are the partner reads (issued eight at a
+      // time: two steps of the four rows, branch-free; a row past its degree
+      // reads entry 0 with weight 0)
+      const int64_t kl = kb[0] + (lane < ne ? lane : 0);
+      const int ci = iu[kl];
+      const double2 ml = MTr<MT>::ld(du + kl);
+      for (int64_t j = 0; j < maxdeg; j += 2) {
+        int64_t c[2][RW];
+        double2 m[2][RW];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+          for (int r = 0; r < RW; ++r) {
+            const int64_t k = kb[r] + j + h;
+            const bool on = k < kb[r + 1];  // wave-uniform
+            const int t = on ? (int)(k - kb[0]) : 0;
+            c[h][r] = (int64_t)__builtin_amdgcn_readlane(ci, t);
+            m[h][r] = make_double2(on ? rl_f64(ml.x, t) : 0.0, on ? rl_f64(ml.y, t) : 0.0);
+          }
+        }
+        double2 v[2][RW];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int r = 0; r < RW; ++r) v[h][r] = VTr<VT>::ld(xd + c[h][r] * Nd);
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+          for (int r = 0; r < RW; ++r) kr_mac<MT, VT>(sr[r], si[r], m[h][r], v[h][r]);
+      }
+    } else {
+      for (int64_t j = 0; j < maxdeg; ++j) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          const int64_t k = kb[r] + j;
+          if (k < kb[r + 1]) {  // wave-uniform
+            const double2 v = VTr<VT>::ld(xd + (int64_t)iu[k] * Nd);
+            kr_mac<MT, VT>(sr[r], si[r], MTr<MT>::ld(du + k), v);
+          }
+        }
+      }
+    }
+    if (!dok) continue;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (r >= nr) break;
+      const int64_t row = (u0 + r) * Nd + d;
+      const double2 xr = VTr<VT>::ld(X + row);
+      if (K.sdiag) {
+        const double g = K.sdiag[row];
+        sr[r] += g * xr.x;
+        si[r] += g * xr.y;
+      }
+      double curB = 0.0, curF = 0.0;
+      spmv_out<VT, EPI, FT>(A, row, row, make_double2(sr[r], si[r]), xr, a_t, mu_t, xs, curB, curF);
+      maxB = fmax(maxB, curB);
+      maxF = fmax(maxF, curF);
+    }
+  }
+  if constexpr (EPI) block_norm_partials(A, maxB, maxF);
+}
+
+static bool kron_two_pass(const qsx_kron* K, int64_t vsize) {
+  if (K->mode == 1) return false;
+  if (K->mode == 2) return true;
+  // one pass while the vector stays L2-resident (8 XCDs x 4 MB)
+  return K->Nu * K->Nd * vsize > ((int64_t)16 << 20);
+}
+
+static int64_t kron_up_groups(const qsx_kron* K) { return (K->Nu + 4 * KRON_UP_ROWS - 1) / (4 * KRON_UP_ROWS); }
+static int64_t kron_up_units(const qsx_kron* K) { return ((K->Nd + 63) / 64) * kron_up_groups(K); }
+
+// blocks of the launch that writes the norm partials
+static int64_t kron_blocks(const qsx_kron* K, int64_t vsize) {
+  const int64_t n = kron_two_pass(K, vsize) ? kron_up_units(K) : K->Nu;
+  const int64_t nb = n < KRON_MAX_BLOCKS ? n : KRON_MAX_BLOCKS;
   return nb < 1 ? 1 : nb;
 }
 
 template <typename MT, typename VT, bool EPI, typename FT>
 static void launch_kron_t(const KronArgs& KA, const qsx_kron* K, hipStream_t st) {
-  const dim3 grid((unsigned)kron_blocks(K)), blk(256);
   const int64_t stage = K->Nd * (int64_t)sizeof(VT);
-  if (stage <= KRON_STAGE_BYTES)
+  const bool staged = stage <= KRON_STAGE_BYTES;
+  const dim3 blk(256);
+  if (kron_two_pass(K, sizeof(VT))) {
+    // rows staged per row-pass block: as many as fit the LDS budget, at most 4
+    // rows staged per row-pass block: up to 32 KB of LDS per block (five
+    // blocks per CU); more rows per block at lower occupancy measured slower
+    const int R = !staged ? 1 : (4 * stage <= 32 * 1024) ? 4 : (2 * stage <= 32 * 1024) ? 2 : 1;
+    const int64_t ng = (K->Nu + R - 1) / R;
+    const dim3 gd((unsigned)(ng < KRON_MAX_BLOCKS ? ng : KRON_MAX_BLOCKS));
+    if (!staged)
+      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, false, 1>), gd, blk, 0, st, KA);
+    else if (R == 4)
+      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, true, 4>), gd, blk, (size_t)(4 * stage), st, KA);
+    else if (R == 2)
+      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, true, 2>), gd, blk, (size_t)(2 * stage), st, KA);
+    else
+      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, true, 1>), gd, blk, (size_t)stage, st, KA);
+    hipLaunchKernelGGL((k_kron_up<MT, VT, EPI, FT>), dim3((unsigned)kron_blocks(K, sizeof(VT))), blk, 0, st, KA,
+                       kron_up_units(K), kron_up_groups(K), K->ind_u, (const MT*)K->dat_u);
+    return;
+  }
+  const dim3 grid((unsigned)kron_blocks(K, sizeof(VT)));
+  if (staged)
     hipLaunchKernelGGL((k_kron<MT, VT, EPI, FT, true>), grid, blk, (size_t)stage, st, KA);
   else
     hipLaunchKernelGGL((k_kron<MT, VT, EPI, FT, false>), grid, blk, 0, st, KA);
+}
+
+static int64_t vt_size(int vdt) { return (vdt == QSX_F32) ? 4 : (vdt == QSX_F64 || vdt == QSX_C64) ? 8 : 16; }
+
+extern "C" void qsx_graph_cache_clear();
+
+// The row-pass buffer of the two-pass form, allocated outside any capture
+// (the entry points call this before launching or recording).  A larger
+// buffer replaces the old one and drops captured step sequences, which hold
+// its address.
+static int kron_prepare(const qsx_kron* K, int vdt) {
+  const int64_t vs = vt_size(vdt);
+  if (!kron_two_pass(K, vs)) return QSX_OK;
+  const size_t need = (size_t)(K->Nu * K->Nd * vs);
+  if (K->z_bytes >= need) return QSX_OK;
+  qsx_graph_cache_clear();
+  if (K->z) {
+    QSX_CHECK_HIP(hipFree(K->z));
+    K->z = nullptr;
+    K->z_bytes = 0;
+  }
+  QSX_CHECK_HIP(hipMalloc(&K->z, need));
+  K->z_bytes = need;
+  return QSX_OK;
 }
 
 template <bool EPI>
@@ -120,6 +410,8 @@ static int launch_kron(const CsrArgs& A, const qsx_kron* K, int vdt, int fdt, hi
   KA.ip_u = K->ip_u; KA.ind_u = K->ind_u; KA.dat_u = K->dat_u; KA.idx64_u = K->idx64_u; KA.Nu = K->Nu;
   KA.ip_d = K->ip_d; KA.ind_d = K->ind_d; KA.dat_d = K->dat_d; KA.idx64_d = K->idx64_d; KA.Nd = K->Nd;
   KA.sdiag = K->sdiag;
+  KA.sell_off = K->sell_off; KA.sell_idx = K->sell_idx; KA.sell_val = K->sell_val;
+  KA.z = K->z;
   const int mdt = K->mat_dtype;
   if (fdt < 0) fdt = vdt;
   if (fdt == vdt) {
@@ -151,21 +443,84 @@ extern "C" int qsx_kron_create(const void* ip_u, const void* ind_u, const void* 
     qsx_set_error("qsx_kron_create: bad arguments");
     return QSX_ERR_ARG;
   }
+  // H_down in sliced ELL, built on the host from the device CSR
+  const size_t msz = (mat_dtype == QSX_F32) ? 4 : (mat_dtype == QSX_F64 || mat_dtype == QSX_C64) ? 8 : 16;
+  std::vector<int64_t> ip((size_t)Nd + 1);
+  if (idx64_d) {
+    QSX_CHECK_HIP(hipMemcpy(ip.data(), ip_d, sizeof(int64_t) * (Nd + 1), hipMemcpyDeviceToHost));
+  } else {
+    std::vector<int32_t> ip32((size_t)Nd + 1);
+    QSX_CHECK_HIP(hipMemcpy(ip32.data(), ip_d, sizeof(int32_t) * (Nd + 1), hipMemcpyDeviceToHost));
+    for (int64_t i = 0; i <= Nd; ++i) ip[i] = ip32[i];
+  }
+  if (ip[0] != 0 || ip[Nd] != nnz_d) { qsx_set_error("qsx_kron_create: H_down indptr does not span nnz_d"); return QSX_ERR_ARG; }
+  std::vector<int32_t> ind((size_t)nnz_d);
+  std::vector<unsigned char> val((size_t)nnz_d * msz);
+  if (nnz_d) {
+    QSX_CHECK_HIP(hipMemcpy(ind.data(), ind_d, sizeof(int32_t) * nnz_d, hipMemcpyDeviceToHost));
+    QSX_CHECK_HIP(hipMemcpy(val.data(), dat_d, msz * nnz_d, hipMemcpyDeviceToHost));
+  }
+  const int64_t nch = (Nd + 63) / 64;
+  std::vector<int64_t> off((size_t)nch + 1, 0);
+  for (int64_t c = 0; c < nch; ++c) {
+    int64_t w = 0;
+    for (int64_t d = c * 64; d < Nd && d < (c + 1) * 64; ++d) w = std::max(w, ip[d + 1] - ip[d]);
+    off[c + 1] = off[c] + 64 * w;
+  }
+  const int64_t ne = off[nch] > 0 ? off[nch] : 1;
+  std::vector<int32_t> sidx((size_t)ne, 0);
+  std::vector<unsigned char> sval((size_t)ne * msz, 0);
+  for (int64_t d = 0; d < Nd; ++d) {
+    const int64_t c = d / 64, l = d % 64;
+    for (int64_t k = ip[d]; k < ip[d + 1]; ++k) {
+      const int32_t col = ind[k];
+      if (col < 0 || col >= Nd) { qsx_set_error("qsx_kron_create: H_down column out of range"); return QSX_ERR_ARG; }
+      const int64_t e = off[c] + 64 * (k - ip[d]) + l;
+      sidx[e] = col;
+      std::memcpy(&sval[e * msz], &val[k * msz], msz);
+    }
+  }
   auto* K = new qsx_kron();
   K->ip_u = ip_u; K->ind_u = (const int32_t*)ind_u; K->dat_u = dat_u; K->idx64_u = idx64_u; K->Nu = Nu; K->nnz_u = nnz_u;
   K->ip_d = ip_d; K->ind_d = (const int32_t*)ind_d; K->dat_d = dat_d; K->idx64_d = idx64_d; K->Nd = Nd; K->nnz_d = nnz_d;
   K->sdiag = sdiag;
   K->mat_dtype = mat_dtype;
+  hipError_t e1 = hipMalloc(&K->sell_off, sizeof(int64_t) * (nch + 1));
+  hipError_t e2 = hipMalloc(&K->sell_idx, sizeof(int32_t) * ne);
+  hipError_t e3 = hipMalloc(&K->sell_val, msz * ne);
+  if (e1 == hipSuccess && e2 == hipSuccess && e3 == hipSuccess) {
+    e1 = hipMemcpy(K->sell_off, off.data(), sizeof(int64_t) * (nch + 1), hipMemcpyHostToDevice);
+    e2 = hipMemcpy(K->sell_idx, sidx.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice);
+    e3 = hipMemcpy(K->sell_val, sval.data(), msz * ne, hipMemcpyHostToDevice);
+  }
+  if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+    if (K->sell_off) (void)hipFree(K->sell_off);
+    if (K->sell_idx) (void)hipFree(K->sell_idx);
+    if (K->sell_val) (void)hipFree(K->sell_val);
+    delete K;
+    qsx_set_error("qsx_kron_create: device allocation or copy failed");
+    return QSX_ERR_HIP;
+  }
   *out = K;
   return QSX_OK;
 }
 
-extern "C" void qsx_graph_cache_clear();
+// 0: one or two passes by size (default), 1: one pass, 2: two passes
+extern "C" int qsx_kron_set_mode(qsx_kron* K, int mode) {
+  if (!K || mode < 0 || mode > 2) { qsx_set_error("qsx_kron_set_mode: bad arguments"); return QSX_ERR_ARG; }
+  K->mode = mode;
+  return QSX_OK;
+}
 
 // Captured step sequences hold the factors' addresses: drop them with the
 // operator so that a later operator at recycled addresses is captured anew.
 extern "C" void qsx_kron_destroy(qsx_kron* K) {
+  if (!K) return;
   qsx_graph_cache_clear();
+  if (K->sell_off) (void)hipFree(K->sell_off);
+  if (K->sell_idx) (void)hipFree(K->sell_idx);
+  if (K->sell_val) (void)hipFree(K->sell_val);
+  if (K->z) (void)hipFree(K->z);
   delete K;
 }
 
@@ -181,6 +536,8 @@ extern "C" int qsx_kron_matvec(const qsx_kron* K, const double* xdiag, double xs
   std::memset(&A, 0, sizeof(A));
   A.Ns = K->Nu * K->Nd; A.nvec = 1; A.x = x; A.y = y; A.a = make_double2(a_re, a_im); A.beta = accumulate ? 1 : 0;
   A.xdiag = xdiag; A.xdscale = make_double2(xs_re, xs_im);
+  const int rc = kron_prepare(K, vec_dtype);
+  if (rc) return rc;
   return launch_kron<false>(A, K, vec_dtype, -1, (hipStream_t)stream);
 }
 
@@ -194,14 +551,16 @@ __global__ void __launch_bounds__(256) k_kron_norm(const double* offu, const dou
                                                    const double* sdiag, const double* xdiag, double2 xs, double2 mu,
                                                    double* part) {
   double mx = 0.0;
-  const int64_t n = Nu * Nd;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t u = i / Nd, d = i - u * Nd;
-    double2 g = cadd(diu[u], did[d]);
-    g = make_double2(g.x - mu.x, g.y - mu.y);
-    if (sdiag) g.x += sdiag[i];
-    if (xdiag) g = make_double2(g.x + xs.x * xdiag[i], g.y + xs.y * xdiag[i]);
-    mx = fmax(mx, offu[u] + offd[d] + sqrt(g.x * g.x + g.y * g.y));
+  for (int64_t u = blockIdx.x; u < Nu; u += gridDim.x) {
+    const double ou = offu[u];
+    const double2 gu = make_double2(diu[u].x - mu.x, diu[u].y - mu.y);
+    for (int64_t d = threadIdx.x; d < Nd; d += blockDim.x) {
+      const int64_t i = u * Nd + d;
+      double2 g = cadd(gu, did[d]);
+      if (sdiag) g.x += sdiag[i];
+      if (xdiag) g = make_double2(g.x + xs.x * xdiag[i], g.y + xs.y * xdiag[i]);
+      mx = fmax(mx, ou + offd[d] + sqrt(g.x * g.x + g.y * g.y));
+    }
   }
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
@@ -214,11 +573,18 @@ __global__ void __launch_bounds__(256) k_kron_norm(const double* offu, const dou
   }
 }
 
-__global__ void k_kron_norm_fin(const double* part, int nb, double* out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+__global__ void __launch_bounds__(1024) k_kron_norm_fin(const double* part, int nb, double* out) {
   double mx = 0.0;
-  for (int i = 0; i < nb; ++i) mx = fmax(mx, part[i]);
-  out[0] = mx;
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) mx = fmax(mx, part[i]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  __shared__ double sw[16];
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) mx = fmax(mx, sw[i]);
+    out[0] = mx;
+  }
 }
 
 // work: at least 1024 doubles of device scratch (block maxima)
@@ -232,7 +598,7 @@ extern "C" int qsx_kron_1_norm(const qsx_kron* K, const double* offu, const doub
                      (const double2*)diag_d, K->Nd, K->sdiag, xdiag, make_double2(xs_re, xs_im), make_double2(mu_re, mu_im),
                      work);
   QSX_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_kron_norm_fin, dim3(1), dim3(64), 0, st, work, nb, out);
+  hipLaunchKernelGGL(k_kron_norm_fin, dim3(1), dim3(1024), 0, st, work, nb, out);
   QSX_LAUNCH_CHECK();
   return QSX_OK;
 }
@@ -250,15 +616,17 @@ static int kron_term(const CsrArgs& A, const void* ctx, int vdt, int fdt, hipStr
   const qsx_kron* K = (const qsx_kron*)ctx;
   const int rc = launch_kron<true>(A, K, vdt, fdt, st);
   if (rc) return rc;
-  return qsx_part_reduce_launch(A, kron_blocks(K), st);
+  return qsx_part_reduce_launch(A, kron_blocks(K, vt_size(vdt)), st);
 }
 
 extern "C" int qsx_evolve_kron_steps(const qsx_kron* K, const double* xdiag, int nsteps, const void* es, int s_max,
                                      int m_max, void* v, void* work, int vec_dtype, int state_dtype, int* fail,
                                      void* ctl_work, int64_t ctl_bytes, int use_graph, void* stream) {
   if (!K) { qsx_set_error("qsx_evolve_kron_steps: null operator"); return QSX_ERR_ARG; }
-  const void* key[3] = {K->dat_u, K->dat_d, K};
+  const int rc = kron_prepare(K, vec_dtype);
+  if (rc) return rc;
+  const void* key[3] = {K->dat_u, K->z, K};
   return qsx_evolve_steps_with(K->Nu * K->Nd, 1, xdiag, nsteps, es, s_max, m_max, v, work, vec_dtype, state_dtype,
-                               fail, ctl_work, ctl_bytes, kron_blocks(K), kron_term, K, key, 1024 + K->mat_dtype,
-                               use_graph, (hipStream_t)stream);
+                               fail, ctl_work, ctl_bytes, kron_blocks(K, vt_size(vec_dtype)), kron_term, K, key,
+                               1024 + 16 * K->mode + K->mat_dtype, use_graph, (hipStream_t)stream);
 }

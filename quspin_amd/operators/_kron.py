@@ -175,6 +175,11 @@ class KronOp:
                 pass
             self.h = None
 
+    def set_mode(self, mode):
+        """Kernel form: 0 by size, 1 one pass, 2 two passes (row pass over
+        H_down, L2-resident column-strip pass over H_up)."""
+        L.check(L.load().qsx_kron_set_mode(self.h, int(mode)), "qsx_kron_set_mode")
+
     @property
     def shape(self):
         return (self.N, self.N)

@@ -51,9 +51,11 @@ def _vec(n, dtype, seed):
     return v.astype(dtype)
 
 
+@pytest.mark.parametrize("mode", [1, 2])
 @pytest.mark.parametrize("dtype,cplx,tol", [(np.float64, False, 1e-12), (np.complex128, True, 1e-12),
                                             (np.float32, False, 3e-6), (np.complex64, True, 3e-6)])
-def test_kron_product_matches_oracle(dtype, cplx, tol):
+def test_kron_product_matches_oracle(dtype, cplx, tol, mode):
+    """Both kernel forms (1: one pass, 2: row pass + column-strip pass)."""
     from quspin_amd import _lib as L
     from quspin_amd.basis import spinful_fermion_basis_general
     from quspin_amd.operators import quantum_LinearOperator
@@ -64,7 +66,8 @@ def test_kron_product_matches_oracle(dtype, cplx, tol):
     ref = _oracle_H(B, st, Lh)
     Q = quantum_LinearOperator(st, basis=B, dtype=np.complex128 if cplx else np.float64, **_KW)
     K = KronOp(Q, dtype)
-    print(f"case: L={Lh} Nf=(3,2) dtype={np.dtype(dtype).name} Nu={K.Nu} Nd={K.Nd}", flush=True)
+    K.set_mode(mode)
+    print(f"case: L={Lh} Nf=(3,2) dtype={np.dtype(dtype).name} mode={mode} Nu={K.Nu} Nd={K.Nd}", flush=True)
     assert (K.Nu, K.Nd) == (20, 15) and K.sdiag is not None
     np.testing.assert_allclose(K.to_scipy().toarray(), ref.toarray(), atol=1e-6 if tol > 1e-9 else 1e-13)
     x = _vec(B.Ns, dtype, 1)
@@ -85,9 +88,10 @@ def test_kron_product_matches_oracle(dtype, cplx, tol):
                                atol=tol * np.abs(ref @ x).max())
 
 
-def test_kron_unstaged_rows_match_oracle():
+@pytest.mark.parametrize("mode", [1, 2])
+def test_kron_unstaged_rows_match_oracle(mode):
     """A down sector wider than the LDS row stage (C(15,7) complex128 rows:
-    103 KB) takes the direct-read kernel."""
+    103 KB) takes the direct-read kernels."""
     from quspin_amd import _lib as L
     from quspin_amd.basis import spinful_fermion_basis_general
     from quspin_amd.operators import quantum_LinearOperator
@@ -96,7 +100,8 @@ def test_kron_unstaged_rows_match_oracle():
     B = spinful_fermion_basis_general(Lh, Nf=(1, 7))
     st = _model(Lh, complex_hop=True)
     K = KronOp(quantum_LinearOperator(st, basis=B, dtype=np.complex128, **_KW), np.complex128)
-    print(f"case: L={Lh} Nf=(1,7) Ns={B.Ns}", flush=True)
+    K.set_mode(mode)
+    print(f"case: L={Lh} Nf=(1,7) Ns={B.Ns} mode={mode}", flush=True)
     assert K.Nd * 16 > 64 * 1024
     ref = _oracle_H(B, st, Lh)
     x = _vec(B.Ns, np.complex128, 4)
@@ -190,10 +195,14 @@ def test_kron_graph_replay_equals_csr_form():
     psi0 = rng.normal(size=B.Ns) + 1j * rng.normal(size=B.Ns)
     psi0 /= np.linalg.norm(psi0)
     out = {}
-    for form, dt_, sdt in (("csr", np.complex128, None), ("kron", np.complex128, None),
-                           ("kron", np.complex64, np.complex128)):
+    for form, dt_, sdt, mode in (("csr", np.complex128, None, 0), ("kron", np.complex128, None, 1),
+                                 ("kron", np.complex128, None, 2), ("kron", np.complex64, np.complex128, 1),
+                                 ("kron", np.complex64, np.complex128, 2)):
+        print(f"case: form={form} dtype={np.dtype(dt_).name} mode={mode}", flush=True)
         ev = DrivenEvolution(hop, D, f, 0.02, dtype=dt_, phase=phase, predictor=net, m_margin=0, state_dtype=sdt,
                              form=form)
+        if mode:
+            ev.kron.set_mode(mode)
         reps = []
         for rep in range(3):
             p = L.to_device(psi0.astype(sdt or dt_))
@@ -203,8 +212,10 @@ def test_kron_graph_replay_equals_csr_form():
             assert ev.stats["fallback_blocks"] == 0
         for r in reps[1:]:
             np.testing.assert_array_equal(r, reps[0])
-        out[(form, np.dtype(dt_).name)] = reps[0]
-    np.testing.assert_allclose(out[("kron", "complex128")], out[("csr", "complex128")], rtol=0, atol=1e-13)
-    err = np.linalg.norm(out[("kron", "complex64")] - out[("csr", "complex128")])
-    print(f"kron complex64 terms + complex128 state vs FP64: {err:.3e}", flush=True)
-    assert err < 1e-5
+        out[(form, np.dtype(dt_).name, mode)] = reps[0]
+    for mode in (1, 2):
+        np.testing.assert_allclose(out[("kron", "complex128", mode)], out[("csr", "complex128", 0)], rtol=0,
+                                   atol=1e-13)
+        err = np.linalg.norm(out[("kron", "complex64", mode)] - out[("csr", "complex128", 0)])
+        print(f"kron mode {mode} complex64 terms + complex128 state vs FP64: {err:.3e}", flush=True)
+        assert err < 1e-5
