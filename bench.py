@@ -55,6 +55,9 @@ def main():
     ap.add_argument("--expm-only", default="", help="comma list of c3,c4,c5 to run (default all)")
     ap.add_argument("--c3-steps", type=int, default=100)
     ap.add_argument("--c5-steps", type=int, default=500)
+    ap.add_argument("--c5-runs", default="", help="comma list of the driven-Hubbard runs to time (default all)")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="driven-Hubbard steps launched directly, without capturing step sequences")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build measures the single-GPU kernel "
@@ -163,7 +166,7 @@ def main():
         if "c4" in which:
             expm["batched_L26"] = run_batched()
         if "c5" in which:
-            expm["learned_hubbard_L14"] = run_learned(args.c5_steps)
+            expm["learned_hubbard_L14"] = run_learned(args.c5_steps, runs=args.c5_runs, graph=not args.no_graph)
 
     line = {
         "metric": "H.v matvecs/s (symmetry-reduced matrix-free, fp64)",
@@ -345,12 +348,16 @@ def run_batched(L=26, nv=64, t=2.0, plain_sectors=2):
     return out
 
 
-def run_learned(steps, L=14, dt=0.02):
+def run_learned(steps, L=14, dt=0.02, runs="", graph=True):
     """1D spinful Fermi-Hubbard L=14 PBC, t_hop=1, U(t)=4+2cos(3t), (7,7)
     (11,778,624 states): `steps` expm steps of dt in complex64 with the
     partition predicted by the device MLP (trained here on reference
     partitions of L=10 and L=12), against the exact per-step selection in
-    complex64 and the plain FP64 path (complex128, exact selection)."""
+    complex64 and the plain FP64 path (complex128, exact selection); each
+    with H0 as one CSR and as the species-factorised operator.  `runs`
+    selects run names (the FP64 CSR reference always runs); `graph=False`
+    launches every step directly instead of replaying captured blocks."""
+    runs_sel = set(r for r in runs.split(",") if r)
     import time
     import torch
     from quspin_amd.basis import spinful_fermion_basis_general
@@ -397,19 +404,21 @@ def run_learned(steps, L=14, dt=0.02):
             ("exact_fp32_kron", np.complex64, hop32, None, False, "kron"),
             ("learned_fp64_kron", np.complex128, hop64, net, True, "kron"),
             ("exact_fp64_kron", np.complex128, hop64, None, False, "kron"))
+    if runs_sel:
+        runs = tuple(r for r in runs if r[0] in runs_sel or r[0] == "exact_fp64")
     psis = {}
     for name, dt_, hop, pred, learned, form in runs:
         ev = DrivenEvolution(hop, D, f, dt, dtype=dt_, phase=phase, predictor=pred, state_dtype=np.complex128,
                              form=form)
         psi = torch.zeros(B.Ns, dtype=torch.complex128, device="cuda")
         psi[i0] = 1
-        ev.run(psi, 0.0, min(steps, 100), learned=learned, block=100)  # graph capture / first launches
+        ev.run(psi, 0.0, min(steps, 100), learned=learned, block=100, graph=graph)  # capture / first launches
         psi.zero_()
         psi[i0] = 1
         ev.stats = {k: 0 for k in ev.stats}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ev.run(psi, 0.0, steps, learned=learned, block=100)
+        ev.run(psi, 0.0, steps, learned=learned, block=100, graph=graph)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         res[name] = {"steps_per_s": round(steps / wall, 2), "ms_per_step": round(1000.0 * wall / steps, 4),
@@ -426,14 +435,14 @@ def run_learned(steps, L=14, dt=0.02):
     res["training"] = {"samples": int(X.shape[0]), "sizes": [10, 12], "epochs": 3000,
                        "loss_first_last": [hist[0][1], hist[-1][1]], "data_s": round(t_data, 2),
                        "fit_s": round(t_fit, 2)}
-    res["speedup_learned_fp32_vs_exact_fp64"] = round(res["learned_fp32"]["steps_per_s"] /
-                                                      res["exact_fp64"]["steps_per_s"], 3)
-    res["gain_learned_vs_exact_fp32"] = round(res["learned_fp32"]["steps_per_s"] /
-                                              res["exact_fp32"]["steps_per_s"], 3)
-    res["speedup_learned_fp32_kron_vs_exact_fp64"] = round(res["learned_fp32_kron"]["steps_per_s"] /
-                                                           res["exact_fp64"]["steps_per_s"], 3)
-    res["gain_kron_vs_csr_learned_fp32"] = round(res["learned_fp32_kron"]["steps_per_s"] /
-                                                 res["learned_fp32"]["steps_per_s"], 3)
+    def ratio(a, b):
+        if a in res and b in res:
+            return round(res[a]["steps_per_s"] / res[b]["steps_per_s"], 3)
+        return None
+    res["speedup_learned_fp32_vs_exact_fp64"] = ratio("learned_fp32", "exact_fp64")
+    res["gain_learned_vs_exact_fp32"] = ratio("learned_fp32", "exact_fp32")
+    res["speedup_learned_fp32_kron_vs_exact_fp64"] = ratio("learned_fp32_kron", "exact_fp64")
+    res["gain_kron_vs_csr_learned_fp32"] = ratio("learned_fp32_kron", "learned_fp32")
     return res
 
 

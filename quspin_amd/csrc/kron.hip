@@ -353,7 +353,6 @@ static void launch_kron_t(const KronArgs& KA, const qsx_kron* K, hipStream_t st)
   const bool staged = stage <= KRON_STAGE_BYTES;
   const dim3 blk(256);
   if (kron_two_pass(K, sizeof(VT))) {
-    // rows staged per row-pass block: as many as fit the LDS budget, at most 4
     // rows staged per row-pass block: up to 32 KB of LDS per block (five
     // blocks per CU); more rows per block at lower occupancy measured slower
     const int R = !staged ? 1 : (4 * stage <= 32 * 1024) ? 4 : (2 * stage <= 32 * 1024) ? 2 : 1;
