@@ -84,10 +84,16 @@ int qsx_sector_rank(qsx_core* core, const void* states, int64_t n, int64_t* out,
 
 /* Lookup table of the sector (state rank -> basis index | element code):
  * size query and build.  Trades HBM for the orbit search of the H.v; no
- * counterpart in the reference. */
-int qsx_table_info(const qsx_core* core, int64_t Ns, int64_t* entries, int* entry_bytes, int* idx_bits);
+ * counterpart in the reference.  nrm / n_bytes: the basis n array (device).
+ * When lut_len > 0 the code field is an id into a coefficient table of
+ * lut_len double2 (chi(g) * sqrt(n) per group element and distinct n of the
+ * basis), filled by qsx_table_build, so the H.v needs no read of the
+ * target's n; otherwise it is the group element. */
+int qsx_table_info(const qsx_core* core, int64_t Ns, const void* nrm, int n_bytes, int64_t* entries, int* entry_bytes,
+                   int* idx_bits, int64_t* lut_len);
 int qsx_table_build(qsx_core* core, const void* basis, int64_t Ns, const int64_t* begin, const int64_t* end, int N_p,
-                    void* table, int entry_bytes, int idx_bits, void* stream);
+                    const void* nrm, int n_bytes, void* table, int entry_bytes, int idx_bits, void* lut,
+                    int64_t lut_len, void* stream);
 
 /* ---- operators ----------------------------------------------------------- */
 
@@ -132,7 +138,8 @@ int qsx_hv_ctx_create(qsx_core* core, qsx_oplist* l, const void* basis, const vo
                       const int64_t* begin, const int64_t* end, int N_p, int dtype, int64_t nvec, int transposed,
                       int conjugated, int diag_conj, const void* diag_arr, int diag_arr_cplx, qsx_hv_ctx** out);
 void qsx_hv_ctx_destroy(qsx_hv_ctx* ctx);
-int qsx_hv_ctx_set_table(qsx_hv_ctx* ctx, const void* table, int entry_bytes, int idx_bits);
+/* lut: the coefficient table of qsx_table_build (NULL when lut_len was 0) */
+int qsx_hv_ctx_set_table(qsx_hv_ctx* ctx, const void* table, int entry_bytes, int idx_bits, const void* lut);
 int qsx_hv_ctx_set_schedule(qsx_hv_ctx* ctx, int sched);
 int qsx_hv_ctx_apply(qsx_hv_ctx* ctx, const void* x, void* y, double a_off_re, double a_off_im, double a_diag_re,
                      double a_diag_im, int beta, void* stream);

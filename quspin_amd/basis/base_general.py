@@ -342,8 +342,10 @@ class basis_general:
         if "table" in self._dev:
             return self._dev["table"]
         lib = L.load()
-        ent, eb, ib = L._i64(), L._i32(), L._i32()
-        rc = lib.qsx_table_info(self._core.h, self._Ns, L.C.byref(ent), L.C.byref(eb), L.C.byref(ib))
+        ent, eb, ib, nl = L._i64(), L._i32(), L._i32(), L._i64()
+        d = self._dev
+        rc = lib.qsx_table_info(self._core.h, self._Ns, ptr(d["n"]), d["n_bytes"], L.C.byref(ent), L.C.byref(eb),
+                                L.C.byref(ib), L.C.byref(nl))
         if rc != 0:
             self._dev["table"] = None
             return None
@@ -351,11 +353,13 @@ class basis_general:
         if nbytes > (max_bytes if max_bytes is not None else 40 * 2 ** 30):
             return None
         tab = L.empty(max(ent.value, 1), np.int32 if eb.value == 4 else np.int64)
-        rc = lib.qsx_table_build(self._core.h, ptr(self._dev["basis"]), self._Ns, ptr(self._dev.get("begin")),
-                                 ptr(self._dev.get("end")), self._N_p, ptr(tab), eb.value, ib.value,
-                                 stream_handle())
+        # coefficient table chi(g) * sqrt(n) (complex128 pairs), when the entries carry its ids
+        lut = L.empty(nl.value, np.complex128) if nl.value > 0 else None
+        rc = lib.qsx_table_build(self._core.h, ptr(d["basis"]), self._Ns, ptr(d.get("begin")), ptr(d.get("end")),
+                                 self._N_p, ptr(d["n"]), d["n_bytes"], ptr(tab), eb.value, ib.value,
+                                 ptr(lut) if lut is not None else None, nl.value, stream_handle())
         L.check(rc, "qsx_table_build")
-        self._dev["table"] = (tab, eb.value, ib.value)
+        self._dev["table"] = (tab, eb.value, ib.value, lut)
         return self._dev["table"]
 
     def _dev_args(self):

@@ -668,11 +668,22 @@ __device__ __forceinline__ I sector_next(const RankDev* __restrict__ R, I s, int
   return s + 1;
 }
 
+// Coefficient classes of the table: the distinct normalisations n > 0 of
+// the basis (sorted), at most QSX_TABLE_NCLS of them.
+constexpr int QSX_TABLE_NCLS = 64;
+
+__device__ __forceinline__ uint32_t table_n(const void* p, int nb, int64_t i) {
+  return nb == 1 ? ((const uint8_t*)p)[i] : nb == 2 ? ((const uint16_t*)p)[i] : ((const uint32_t*)p)[i];
+}
+
+// ncls > 0: the code field holds code + G * class(n_c) (an id into the
+// coefficient table chi(code) * sqrt(n_c)); ncls = 0: the group element code.
 template <typename I, int M, bool FERM>
 __global__ void __launch_bounds__(256) k_table(const CoreDev* __restrict__ C, const RankDev* __restrict__ R,
                                                const I* __restrict__ basis, int64_t Ns, const int64_t* __restrict__ begin,
                                                const int64_t* __restrict__ end, int N_p, int64_t total, int64_t chunk,
-                                               void* __restrict__ table, int u64, int idx_bits) {
+                                               void* __restrict__ table, int u64, int idx_bits, const void* nrm,
+                                               int n_bytes, const uint32_t* __restrict__ nvals, int ncls) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t a = t * chunk;
   if (a >= total) return;
@@ -688,34 +699,87 @@ __global__ void __launch_bounds__(256) k_table(const CoreDev* __restrict__ C, co
       hi = end[p];
     }
     const int64_t idx = bsearch_desc<I>(basis, lo, hi, rep);
+    uint32_t code = (uint32_t)O.code;
+    if (ncls > 0 && idx >= 0) {
+      const uint32_t n = table_n(nrm, n_bytes, idx);
+      int k = 0;
+      while (k + 1 < ncls && nvals[k] != n) ++k;
+      code += (uint32_t)C->G * (uint32_t)k;
+    }
     if (u64) {
       uint64_t e = ~0ull;
-      if (idx >= 0) e = (uint64_t)idx | ((uint64_t)O.code << 40) | (O.sign < 0 ? (1ull << 63) : 0ull);
+      if (idx >= 0) e = (uint64_t)idx | ((uint64_t)code << 40) | (O.sign < 0 ? (1ull << 63) : 0ull);
       ((uint64_t*)table)[rk] = e;
     } else {
       uint32_t e = 0xFFFFFFFFu;
-      if (idx >= 0) e = (uint32_t)idx | ((uint32_t)O.code << idx_bits);
+      if (idx >= 0) e = (uint32_t)idx | (code << idx_bits);
       ((uint32_t*)table)[rk] = e;
     }
     if (rk + 1 < b) s = sector_next<I>(R, s, rk + 1);
   }
 }
 
+// Distinct normalisations n > 0 of the basis (sorted); false when there are
+// more than QSX_TABLE_NCLS.
+static int table_classes(const void* nrm, int n_bytes, int64_t Ns, std::vector<uint32_t>& vals, bool& fits) {
+  fits = false;
+  vals.clear();
+  if (!nrm || Ns <= 0) return QSX_OK;
+  std::vector<unsigned char> h((size_t)Ns * n_bytes);
+  QSX_CHECK_HIP(hipMemcpy(h.data(), nrm, h.size(), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> v;
+  for (int64_t i = 0; i < Ns; ++i) {
+    uint32_t n = n_bytes == 1 ? h[i] : n_bytes == 2 ? ((const uint16_t*)h.data())[i] : ((const uint32_t*)h.data())[i];
+    if (n == 0) continue;
+    if (std::find(v.begin(), v.end(), n) == v.end()) {
+      v.push_back(n);
+      if ((int)v.size() > QSX_TABLE_NCLS) return QSX_OK;
+    }
+  }
+  std::sort(v.begin(), v.end());
+  vals = v;
+  fits = true;
+  return QSX_OK;
+}
+
 // Size and layout of the lookup table for the sector of the last make_basis:
-// entries, bytes per entry (4 or 8) and index bits of the 4-byte layout.
+// entries, bytes per entry (4 or 8), index bits of the 4-byte layout, and the
+// length of the coefficient table (0: entries hold the group element; the
+// kernels then read n_c of the target).  nrm / n_bytes: the basis n array.
 // Returns QSX_ERR_UNSUPPORTED when the basis was not built from one sector.
-extern "C" int qsx_table_info(const qsx_core* c, int64_t Ns, int64_t* entries, int* entry_bytes, int* idx_bits) {
+extern "C" int qsx_table_info(const qsx_core* c, int64_t Ns, const void* nrm, int n_bytes, int64_t* entries,
+                              int* entry_bytes, int* idx_bits, int64_t* lut_len) {
   if (!c || !c->sector_ok || c->h.nmaps == 0) {
     qsx_set_error("lookup table needs a symmetric basis made from one particle sector");
     return QSX_ERR_UNSUPPORTED;
+  }
+  if (!entries || !entry_bytes || !idx_bits || !lut_len || (nrm && n_bytes != 1 && n_bytes != 2 && n_bytes != 4)) {
+    qsx_set_error("qsx_table_info: bad arguments");
+    return QSX_ERR_ARG;
   }
   int ib = 1;
   while (ib < 40 && ((int64_t)1 << ib) <= Ns) ++ib;
   int cb = 0;
   while ((1 << cb) < c->h.G) ++cb;
+  std::vector<uint32_t> vals;
+  bool fits = false;
+  const int rc = table_classes(nrm, n_bytes, Ns, vals, fits);
+  if (rc) return rc;
+  const int64_t nl = fits ? (int64_t)c->h.G * (int64_t)vals.size() : 0;
+  int lb = 0;
+  while (nl > 0 && ((int64_t)1 << lb) < nl) ++lb;
   *entries = c->sector_total;
   *idx_bits = ib;
-  *entry_bytes = (ib + cb <= 31 && !c->h.fermion) ? 4 : 8;
+  if (!c->h.fermion && nl > 0 && ib + lb <= 31) {
+    *entry_bytes = 4;
+    *lut_len = nl;
+  } else if (!c->h.fermion && ib + cb <= 31) {
+    *entry_bytes = 4;
+    *lut_len = 0;
+  } else {
+    *entry_bytes = 8;
+    *lut_len = (nl > 0 && nl <= (1 << 23)) ? nl : 0;
+  }
   if (*entry_bytes == 8 && (Ns >= (1ll << 40) || c->h.G > (1 << 23))) {
     qsx_set_error("lookup table: basis or group too large");
     return QSX_ERR_UNSUPPORTED;
@@ -723,13 +787,41 @@ extern "C" int qsx_table_info(const qsx_core* c, int64_t Ns, int64_t* entries, i
   return QSX_OK;
 }
 
+// Fills table (entries as described by qsx_table_info) and, when lut_len > 0,
+// the coefficient table lut[code + G * class] = chi(code) * sqrt(n_class)
+// (lut_len double2 of device memory).  Synchronous.
 extern "C" int qsx_table_build(qsx_core* c, const void* basis, int64_t Ns, const int64_t* begin, const int64_t* end,
-                               int N_p, void* table, int entry_bytes, int idx_bits, void* stream) {
-  if (!c || !c->sector_ok || (entry_bytes != 4 && entry_bytes != 8)) {
+                               int N_p, const void* nrm, int n_bytes, void* table, int entry_bytes, int idx_bits,
+                               void* lut, int64_t lut_len, void* stream) {
+  if (!c || !c->sector_ok || (entry_bytes != 4 && entry_bytes != 8) || (lut_len > 0 && (!lut || !nrm))) {
     qsx_set_error("qsx_table_build: bad arguments");
     return QSX_ERR_ARG;
   }
   hipStream_t st = (hipStream_t)stream;
+  std::vector<uint32_t> vals;
+  uint32_t* nv_d = nullptr;
+  int ncls = 0;
+  if (lut_len > 0) {
+    bool fits = false;
+    const int rc = table_classes(nrm, n_bytes, Ns, vals, fits);
+    if (rc) return rc;
+    if (!fits || (int64_t)c->h.G * (int64_t)vals.size() != lut_len) {
+      qsx_set_error("qsx_table_build: coefficient table length does not match the basis");
+      return QSX_ERR_ARG;
+    }
+    ncls = (int)vals.size();
+    std::vector<double2> lh((size_t)lut_len);
+    for (int k = 0; k < ncls; ++k) {
+      const double r = std::sqrt((double)vals[k]);
+      for (int g = 0; g < c->h.G; ++g) {
+        const double2 p = c->phase_h[g];
+        lh[(size_t)g + (size_t)c->h.G * k] = make_double2(r * p.x, r * p.y);
+      }
+    }
+    QSX_CHECK_HIP(hipMemcpy(lut, lh.data(), sizeof(double2) * lut_len, hipMemcpyHostToDevice));
+    QSX_CHECK_HIP(hipMalloc(&nv_d, sizeof(uint32_t) * ncls));
+    QSX_CHECK_HIP(hipMemcpy(nv_d, vals.data(), sizeof(uint32_t) * ncls, hipMemcpyHostToDevice));
+  }
   const int64_t total = c->sector_total;
   const int64_t chunk = 16;
   const int64_t nth = (total + chunk - 1) / chunk;
@@ -737,12 +829,20 @@ extern "C" int qsx_table_build(qsx_core* c, const void* basis, int64_t Ns, const
   const int u64 = entry_bytes == 8;
   const bool ferm = c->h.fermion != 0;
   if (c->h.N <= 32) {
-    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits)); }
-    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits)); }
+    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls)); }
+    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls)); }
   } else {
-    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits)); }
-    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits)); }
+    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls)); }
+    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls)); }
   }
-  QSX_LAUNCH_CHECK();
+  const hipError_t le = hipGetLastError();
+  if (nv_d) {
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(nv_d);
+  }
+  if (le != hipSuccess) {
+    qsx_set_error(std::string("kernel launch: ") + hipGetErrorString(le));
+    return QSX_ERR_HIP;
+  }
   return QSX_OK;
 }

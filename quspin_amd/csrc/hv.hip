@@ -587,11 +587,12 @@ struct qsx_hv_ctx {
 };
 
 // Use a sector lookup table (qsx_table_build) instead of the orbit search.
-extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_bytes, int idx_bits) {
+extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_bytes, int idx_bits, const void* lut) {
   if (!X || (table && entry_bytes != 4 && entry_bytes != 8)) { qsx_set_error("qsx_hv_ctx_set_table: bad arguments"); return QSX_ERR_ARG; }
   X->A.table = table;
   X->A.table_u64 = entry_bytes == 8;
   X->A.idx_bits = idx_bits;
+  X->A.tlut = table ? (const double2*)lut : nullptr;
   return QSX_OK;
 }
 

@@ -111,6 +111,9 @@ struct HvArgs {
   const void* table;
   int table_u64;
   int idx_bits;
+  // with a coefficient table the entry's code field is an id into tlut
+  // (sign-free chi(g) * sqrt(n_c) of the target), so no n_c read is needed
+  const double2* tlut;
   int sched;          // 0: one lane per row, 1: compacted queue (one vector)
 };
 
@@ -248,11 +251,20 @@ __device__ __forceinline__ int64_t sector_rank_inc(const RankDev* __restrict__ R
   return res;
 }
 
-// Representative of s2, its basis index, element code, stabiliser size and
-// fermion sign.  Returns false when the representative is not in the basis.
+// Element factor of a target reached through group element `code` with
+// stabiliser count cnt and fermion sign: sign * chi(g) * sqrt(n_c).
+__device__ __forceinline__ double2 elem_fac(const HvArgs& A, int code, int cnt, int sign) {
+  const double w = (double)sign * sqrt_small(cnt);
+  const double2 p = A.ph[code];
+  return make_double2(w * p.x, w * p.y);
+}
+
+// Basis index of the representative of the sector state of rank rk, and its
+// element factor.  Returns false when the representative is not in the basis.
 template <typename I>
-__device__ __forceinline__ bool table_entry(const HvArgs& A, int64_t rk, int64_t& c, int& code, int& cnt, int& sign) {
+__device__ __forceinline__ bool table_entry(const HvArgs& A, int64_t rk, int64_t& c, double2& fac) {
   if (rk < 0) return false;
+  int code, sign = 1;
   if (A.table_u64) {
     const uint64_t e = ((const uint64_t*)A.table)[rk];
     if (e == ~0ull) return false;
@@ -265,24 +277,28 @@ __device__ __forceinline__ bool table_entry(const HvArgs& A, int64_t rk, int64_t
     c = (int64_t)(e & ((1u << A.idx_bits) - 1u));
     code = (int)(e >> A.idx_bits);
   }
-  cnt = (int)load_n(A.nrm, A.n_bytes, c);
+  if (A.tlut) {
+    const double2 f = A.tlut[code];
+    fac = make_double2(sign * f.x, sign * f.y);
+  } else {
+    fac = elem_fac(A, code, (int)load_n(A.nrm, A.n_bytes, c), sign);
+  }
   return true;
 }
 
+// Basis index c of the representative of s2 and the element factor (1 with
+// no symmetry maps).  Returns false when s2 has no representative in the basis.
 template <typename I, int M, bool FERM, int LK>
-__device__ __forceinline__ bool resolve(const HvArgs& A, const int64_t* sb, I s2, int64_t& c, int& code, int& cnt,
-                                        int& sign) {
-  code = 0; cnt = 1; sign = 1;
+__device__ __forceinline__ bool resolve(const HvArgs& A, const int64_t* sb, I s2, int64_t& c, double2& fac) {
+  fac = make_double2(1.0, 0.0);
   if constexpr (LK == LOOKUP_TABLE) {
-    return table_entry<I>(A, sector_rank<I>(A.R, sb, s2), c, code, cnt, sign);
+    return table_entry<I>(A, sector_rank<I>(A.R, sb, s2), c, fac);
   } else {
     I rep = s2;
     if constexpr (M != 0) {
       OrbitRes R;
       rep = orbit_max<I, M, FERM>(A.C, s2, R);
-      code = R.code;
-      cnt = R.cnt;
-      sign = R.sign;
+      fac = elem_fac(A, R.code, R.cnt, R.sign);
     }
     c = lookup_index<I, LK>(A, sb, rep);
     return c >= 0;
@@ -293,18 +309,17 @@ __device__ __forceinline__ bool resolve(const HvArgs& A, const int64_t* sb, I s2
 // (rank data rr) when the lookup works by rank.
 template <typename I, int M, bool FERM, int LK>
 __device__ __forceinline__ bool resolve_row(const HvArgs& A, const int64_t* sb, I s, RowRank rr,
-                                            const TermDev* __restrict__ T, I s2, int64_t& c, int& code, int& cnt,
-                                            int& sign) {
+                                            const TermDev* __restrict__ T, I s2, int64_t& c, double2& fac) {
   if constexpr (LK == LOOKUP_TABLE) {
-    code = 0; cnt = 1; sign = 1;
-    return table_entry<I>(A, sector_rank_inc<I>(A.R, sb, s, rr, s2, T->lo, T->hi), c, code, cnt, sign);
+    fac = make_double2(1.0, 0.0);
+    return table_entry<I>(A, sector_rank_inc<I>(A.R, sb, s, rr, s2, T->lo, T->hi), c, fac);
   } else if constexpr (LK == LOOKUP_RANK) {
-    code = 0; cnt = 1; sign = 1;
+    fac = make_double2(1.0, 0.0);
     const int64_t rk = sector_rank_inc<I>(A.R, sb, s, rr, s2, T->lo, T->hi);
     c = rk < 0 ? (int64_t)-1 : A.R->Ns - 1 - rk;
     return c >= 0;
   } else {
-    return resolve<I, M, FERM, LK>(A, sb, s2, c, code, cnt, sign);
+    return resolve<I, M, FERM, LK>(A, sb, s2, c, fac);
   }
 }
 
@@ -325,19 +340,19 @@ __device__ __forceinline__ void term_element(const HvArgs& A, const int64_t* sb,
   int ip;
   if (!apply_term<I>(T, A.C->fermion != 0, s2, mag, ip)) return;
   int64_t c;
-  int code, cnt, sign;
-  if (!resolve_row<I, M, FERM, LK>(A, sb, s, rr, T, s2, c, code, cnt, sign)) return;
-  double w = mag * (double)sign;
-  if constexpr (M != 0) w *= sqrt_small(cnt) * rsq;
+  double2 fac;
+  if (!resolve_row<I, M, FERM, LK>(A, sb, s, rr, T, s2, c, fac)) return;
+  double w = mag;
+  if constexpr (M != 0) w *= rsq;
   if constexpr (CPLX) {
     double2 coef = cmul(make_double2(T->jr, T->ji), ipow_c(ip));
     coef = make_double2(coef.x * w, coef.y * w);
-    if constexpr (M != 0) coef = cmul(coef, A.ph[code]);
+    if constexpr (M != 0) coef = cmul(coef, fac);
     if (conj) coef = cconj(coef);
     f(c, coef);
   } else {
     double v = T->jr * w * ((ip & 2) ? -1.0 : 1.0);
-    if constexpr (M != 0) v *= A.ph[code].x;
+    if constexpr (M != 0) v *= fac.x;
     f(c, make_double2(v, 0.0));
   }
 }
@@ -510,59 +525,35 @@ __global__ void __launch_bounds__(256) k_hv_rows_batch(HvArgs A) {
       }
     }
     int64_t c[KB];
-    int code[KB], sg[KB];
+    double2 fac[KB];
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
       c[k] = -1;
-      code[k] = 0;
-      sg[k] = 1;
+      fac[k] = make_double2(1.0, 0.0);
       if constexpr (LK == LOOKUP_TABLE) {
-        if (rk[k] >= 0) {
-          if (A.table_u64) {
-            const uint64_t e = ((const uint64_t*)A.table)[rk[k]];
-            if (e != ~0ull) {
-              c[k] = (int64_t)(e & ((1ull << 40) - 1));
-              code[k] = (int)((e >> 40) & 0x7FFFFF);
-              sg[k] = (e >> 63) ? -1 : 1;
-            }
-          } else {
-            const uint32_t e = ((const uint32_t*)A.table)[rk[k]];
-            if (e != 0xFFFFFFFFu) {
-              c[k] = (int64_t)(e & ((1u << A.idx_bits) - 1u));
-              code[k] = (int)(e >> A.idx_bits);
-            }
-          }
-        }
+        if (!table_entry<I>(A, rk[k], c[k], fac[k])) c[k] = -1;
       } else {
         c[k] = rk[k] < 0 ? (int64_t)-1 : A.R->Ns - 1 - rk[k];
       }
     }
     double2 x[KB];
-    double w[KB];
 #pragma unroll
-    for (int k = 0; k < KB; ++k) {
-      x[k] = make_double2(0.0, 0.0);
-      w[k] = 0.0;
-      if (c[k] >= 0) {
-        x[k] = VTr<VT>::ld(vin + c[k]);
-        double ww = mg[k] * (double)sg[k];
-        if constexpr (M != 0) ww *= sqrt_small((int)load_n(A.nrm, A.n_bytes, c[k])) * rsq;
-        w[k] = ww;
-      }
-    }
+    for (int k = 0; k < KB; ++k) x[k] = c[k] >= 0 ? VTr<VT>::ld(vin + c[k]) : make_double2(0.0, 0.0);
 #pragma unroll
     for (int k = 0; k < KB; ++k) {
       if (c[k] < 0) continue;
       const TermDev* T = A.off + t0 + k;
+      double w = mg[k];
+      if constexpr (M != 0) w *= rsq;
       if constexpr (CPLX) {
         double2 coef = cmul(make_double2(T->jr, T->ji), ipow_c(ipw[k]));
-        coef = make_double2(coef.x * w[k], coef.y * w[k]);
-        if constexpr (M != 0) coef = cmul(coef, A.ph[code[k]]);
+        coef = make_double2(coef.x * w, coef.y * w);
+        if constexpr (M != 0) coef = cmul(coef, fac[k]);
         if (A.off_conj) coef = cconj(coef);
         acc = cadd(acc, cmul(coef, x[k]));
       } else {
-        double v = T->jr * w[k] * ((ipw[k] & 2) ? -1.0 : 1.0);
-        if constexpr (M != 0) v *= A.ph[code[k]].x;
+        double v = T->jr * w * ((ipw[k] & 2) ? -1.0 : 1.0);
+        if constexpr (M != 0) v *= fac[k].x;
         acc.x += v * x[k].x;
       }
     }
@@ -635,20 +626,18 @@ __global__ void __launch_bounds__(QBLK) k_hv_queue(HvArgs A) {
     const int n = q_n;
     for (int e = tid; e < n; e += QBLK) {
       int64_t c;
-      int code, cnt, sign;
+      double2 fac;
       double2 val = make_double2(0.0, 0.0);
-      if (resolve<I, M, FERM, LK>(A, sb, q_s[e], c, code, cnt, sign)) {
-        double w = (double)sign;
-        if constexpr (M != 0) w *= sqrt_small(cnt);
+      if (resolve<I, M, FERM, LK>(A, sb, q_s[e], c, fac)) {
         const double2 x = VTr<VT>::ld(vin + c);
         if constexpr (CPLX) {
-          double2 f = make_double2(w, 0.0);
-          if constexpr (M != 0) f = make_double2(w * A.ph[code].x, w * A.ph[code].y);
+          double2 f = make_double2(1.0, 0.0);
+          if constexpr (M != 0) f = fac;
           if (A.off_conj) f = cconj(f);
           val = cmul(f, x);
         } else {
-          double f = w;
-          if constexpr (M != 0) f *= A.ph[code].x;
+          double f = 1.0;
+          if constexpr (M != 0) f = fac.x;
           val = make_double2(f * x.x, 0.0);
         }
       }
