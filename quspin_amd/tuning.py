@@ -124,25 +124,40 @@ def time_candidates(op, ctx, vdt, nvec, reps=5):
     x = t.ones(shape, dtype=L.torch_dtype(vdt), device="cuda")
     y = t.empty_like(x)
     lib = L.load()
-    best, best_ms = None, float("inf")
-    results = []
-    for cfg in candidates(op, vdt, nvec):
+    st = t.cuda.current_stream()
+
+    def timed(cfg, n):
         ctx.configure(cfg, op.basis)
         if ctx.cfg["lookup"] != cfg["lookup"]:
-            continue
+            return None
         for _ in range(2):
             lib.qsx_hv_ctx_apply(ctx.h, L.ptr(x), L.ptr(y), 1.0, 0.0, 1.0, 0.0, 0, L.stream_handle())
-        st = t.cuda.current_stream()
         e0, e1 = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
         e0.record(st)
-        for _ in range(reps):
+        for _ in range(n):
             lib.qsx_hv_ctx_apply(ctx.h, L.ptr(x), L.ptr(y), 1.0, 0.0, 1.0, 0.0, 0, L.stream_handle())
         e1.record(st)
         t.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / reps
-        results.append((cfg, ms))
-        if ms < best_ms:
-            best, best_ms = dict(cfg), ms
+        return e0.elapsed_time(e1) / n
+
+    results = []
+    for cfg in candidates(op, vdt, nvec):
+        ms = timed(cfg, reps)
+        if ms is not None:
+            results.append((cfg, ms))
+    # candidates within 15 % of the best are timed again, longer and in
+    # alternation, so that launch-to-launch noise does not pick the winner
+    first = min(ms for _, ms in results)
+    close = [i for i, (_, ms) in enumerate(results) if ms <= 1.15 * first]
+    if len(close) > 1:
+        refined = {i: float("inf") for i in close}
+        for _ in range(2):
+            for i in close:
+                refined[i] = min(refined[i], timed(results[i][0], 8 * reps))
+        for i in close:
+            results[i] = (results[i][0], refined[i])
+    bi = min(range(len(results)), key=lambda i: results[i][1])
+    best, best_ms = dict(results[bi][0]), results[bi][1]
     best["ms"] = round(best_ms, 5)
     best["timings"] = [[c["lookup"], c["schedule"], round(ms, 5)] for c, ms in results]
     return best
