@@ -111,7 +111,8 @@ def main():
     ms_per = ms_total / args.steps
     value = 1000.0 / ms_per
 
-    nbytes = Ns * (4 + 1) + 2 * Ns * 8
+    # per reduced state: 4 B state word, 1 B n, 8 B folded diagonal, 8 B x read, 8 B y written
+    nbytes = Ns * (4 + 1 + 8) + 2 * Ns * 8
     achieved = nbytes / (ms_per * 1e-3) / 1e9
 
     # 200-step Lanczos on the same operator (device recurrence)

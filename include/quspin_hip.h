@@ -141,6 +141,11 @@ void qsx_hv_ctx_destroy(qsx_hv_ctx* ctx);
 /* lut: the coefficient table of qsx_table_build (NULL when lut_len was 0) */
 int qsx_hv_ctx_set_table(qsx_hv_ctx* ctx, const void* table, int entry_bytes, int idx_bits, const void* lut);
 int qsx_hv_ctx_set_schedule(qsx_hv_ctx* ctx, int sched);
+/* Replace the context's diagonal (diagonal terms and extra diagonal) by a
+ * precomputed per-row array (Ns doubles, or Ns (re, im) pairs when
+ * diag_arr_cplx), as quantum_LinearOperator keeps `_diagonal`
+ * (quantum_LinearOperator_core.py:137-156); the caller keeps it alive. */
+int qsx_hv_ctx_set_diag(qsx_hv_ctx* ctx, const void* diag_arr, int diag_arr_cplx);
 int qsx_hv_ctx_apply(qsx_hv_ctx* ctx, const void* x, void* y, double a_off_re, double a_off_im, double a_diag_re,
                      double a_diag_im, int beta, void* stream);
 

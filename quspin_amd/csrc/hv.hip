@@ -634,6 +634,24 @@ extern "C" int qsx_hv_ctx_create(qsx_core* c, qsx_oplist* l, const void* basis, 
   return QSX_OK;
 }
 
+// Replace the whole diagonal of the context (its diagonal terms, with their
+// conjugation, and any extra diagonal) by the per-row array diag_arr (Ns
+// doubles, or Ns (re, im) pairs when diag_arr_cplx).  The caller computes it
+// once (the product with a_off = 0 on a vector of ones) and keeps it alive.
+extern "C" int qsx_hv_ctx_set_diag(qsx_hv_ctx* X, const void* diag_arr, int diag_arr_cplx) {
+  if (!X || !diag_arr) { qsx_set_error("qsx_hv_ctx_set_diag: bad arguments"); return QSX_ERR_ARG; }
+  if ((X->dtype == QSX_F32 || X->dtype == QSX_F64) && diag_arr_cplx) {
+    qsx_set_error("real dtype requested for an operator with a complex diagonal");
+    return QSX_ERR_COMPLEX;
+  }
+  X->A.dg = nullptr;
+  X->A.n_diag = 0;
+  X->A.diag_conj = 0;
+  X->A.diag_arr = diag_arr;
+  X->A.diag_arr_cplx = diag_arr_cplx ? 1 : 0;
+  return QSX_OK;
+}
+
 extern "C" void qsx_hv_ctx_destroy(qsx_hv_ctx* X) { delete X; }
 
 // y = beta*y + a_diag*D x + a_off*Op x

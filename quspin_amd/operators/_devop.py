@@ -143,7 +143,7 @@ class MatrixFreeOp:
     def Ns(self):
         return self.basis.Ns
 
-    def _new_ctx(self, vdt, nvec):
+    def _raw_ctx(self, vdt, nvec):
         h = L._vp()
         dcplx = 0
         if self.diag is not None:
@@ -153,6 +153,37 @@ class MatrixFreeOp:
                                         int(self.diag_conj), ptr(self.diag), dcplx, C.byref(h))
         L.check(rc, "qsx_hv_ctx_create")
         return _Ctx(h)
+
+    def _folded_diagonal(self):
+        """The operator's whole diagonal (diagonal terms and extra diagonal)
+        as one device vector, computed once: float64 when real, else
+        complex128.  None when the operator has no diagonal part."""
+        if not hasattr(self, "_dfold"):
+            self._dfold = None
+            if self.oplist.n_diag > 0 or self.diag is not None:
+                ctx = self._raw_ctx(np.dtype(np.complex128), 1)
+                t = L.torch()
+                ones = t.ones(self.Ns, dtype=t.complex128, device="cuda")
+                d = L.zeros(self.Ns, np.complex128)
+                L.check(L.load().qsx_hv_ctx_apply(ctx.h, ptr(ones), ptr(d), 0.0, 0.0, 1.0, 0.0, 0,
+                                                  stream_handle()), "qsx_hv_ctx_apply")
+                del ctx
+                if bool((d.imag != 0).any().item()):
+                    self._dfold = d
+                else:
+                    self._dfold = d.real.contiguous()
+        return self._dfold
+
+    def _new_ctx(self, vdt, nvec):
+        """H.v context whose diagonal is read from the folded per-row array
+        (one 8- or 16-byte read per row instead of evaluating every diagonal
+        term per product), as quantum_LinearOperator keeps ``_diagonal``."""
+        ctx = self._raw_ctx(vdt, nvec)
+        d = self._folded_diagonal()
+        if d is not None:
+            cplx = 1 if L.np_dtype(d.dtype).kind == "c" else 0
+            L.check(L.load().qsx_hv_ctx_set_diag(ctx.h, ptr(d), cplx), "qsx_hv_ctx_set_diag")
+        return ctx
 
     def _context(self, vdt, nvec):
         key = (vdt, nvec)
