@@ -89,6 +89,43 @@ int compile_term(const qsx_core* c, const char* opstr, const int32_t* indx, int 
   return QSX_OK;
 }
 
+bool mergeable(const TermDev& a, const TermDev& b) {
+  return a.fast && b.fast && !a.has2 && !b.has2 && a.flip == b.flip && a.req_mask == b.req_mask &&
+         a.req_val != b.req_val && a.zmask == b.zmask && a.ymask == b.ymask && a.nz == b.nz && a.ny == b.ny &&
+         a.mag0 == b.mag0 && a.jr == b.jr && a.ji == b.ji;
+}
+
+// Merge off-diagonal terms that differ only in the bits they require (same
+// flip, amplitude form and coupling): at most one of the two applies to a
+// state and both reach s ^ flip with the same element, so the H.v checks
+// both patterns and resolves the target once.  The pairing is applied to the
+// term list and its Hermitian-conjugate list together (they stay aligned).
+void merge_pairs(std::vector<TermDev>& off, std::vector<TermDev>& hc) {
+  const size_t n = off.size();
+  std::vector<char> gone(n, 0);
+  for (size_t i = 0; i < n; ++i) {
+    if (gone[i] || !off[i].fast) continue;
+    for (size_t j = i + 1; j < n; ++j) {
+      if (gone[j] || !mergeable(off[i], off[j]) || !mergeable(hc[i], hc[j])) continue;
+      off[i].req_val2 = off[j].req_val;
+      off[i].has2 = 1;
+      hc[i].req_val2 = hc[j].req_val;
+      hc[i].has2 = 1;
+      gone[j] = 1;
+      break;
+    }
+  }
+  size_t k = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (gone[i]) continue;
+    off[k] = off[i];
+    hc[k] = hc[i];
+    ++k;
+  }
+  off.resize(k);
+  hc.resize(k);
+}
+
 }  // namespace
 
 struct qsx_oplist_full {
@@ -121,6 +158,7 @@ extern "C" int qsx_oplist_create(qsx_core* c, int n_terms, const char* opstrs, c
     if (T.flip == 0) L->base.diag_h.push_back(T);
     else { L->base.off_h.push_back(T); L->off_hc_h.push_back(Th); }
   }
+  merge_pairs(L->base.off_h, L->off_hc_h);
   L->base.n_off = (int)L->base.off_h.size();
   L->base.n_diag = (int)L->base.diag_h.size();
   auto up = [](std::vector<TermDev>& v, TermDev** d) -> hipError_t {
