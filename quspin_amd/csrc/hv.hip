@@ -134,6 +134,7 @@ struct qsx_oplist_full {
   std::vector<TermDev> off_hc_h;
   int real_ok = 1;            // all factors real for real dtypes
   int symmetric = 1;          // list commutes with the symmetry group (gather form exact)
+  int all_fast = 0;           // every off-diagonal term (and its conjugate) in the bit-parallel form
 };
 
 extern "C" int qsx_oplist_create(qsx_core* c, int n_terms, const char* opstrs, const int32_t* lens,
@@ -159,6 +160,9 @@ extern "C" int qsx_oplist_create(qsx_core* c, int n_terms, const char* opstrs, c
     else { L->base.off_h.push_back(T); L->off_hc_h.push_back(Th); }
   }
   merge_pairs(L->base.off_h, L->off_hc_h);
+  L->all_fast = 1;
+  for (size_t i = 0; i < L->base.off_h.size(); ++i)
+    if (!L->base.off_h[i].fast || !L->off_hc_h[i].fast) L->all_fast = 0;
   L->base.n_off = (int)L->base.off_h.size();
   L->base.n_diag = (int)L->base.diag_h.size();
   auto up = [](std::vector<TermDev>& v, TermDev** d) -> hipError_t {
@@ -231,8 +235,11 @@ static int hv_launch(const qsx_core* c, const qsx_oplist_full* L, HvArgs A, int 
     A.off = transposed ? L->base.off : L->off_hc;
     A.n_off = L->base.n_off;
     A.off_conj = (transposed == conjugated) ? 1 : 0;
+    A.fast_rows = (L->all_fast && A.n_diag == 0 && !fm && c->rank_h.mode == 0 &&
+                   (lookup == LOOKUP_TABLE || lookup == LOOKUP_RANK)) ? 1 : 0;
     return dispatch_hv(A, dtype, N, nm, fm, lookup, 0, st);
   }
+  A.fast_rows = 0;
   HvArgs D = A;
   D.off = nullptr;
   D.n_off = 0;

@@ -115,6 +115,7 @@ struct HvArgs {
   // (sign-free chi(g) * sqrt(n_c) of the target), so no n_c read is needed
   const double2* tlut;
   int sched;          // 0: one lane per row, 1: compacted queue (one vector)
+  int fast_rows;      // every off-diagonal term bit-parallel, no diagonal terms, single hcb sector
 };
 
 __device__ __forceinline__ double load_n(const void* p, int nb, int64_t i) {
@@ -438,6 +439,75 @@ __global__ void __launch_bounds__(256) k_hv_rows(HvArgs A) {
   }
 }
 
+// One vector, one lane per row, for the common case the host flags with
+// fast_rows: every off-diagonal term is in the bit-parallel form, the
+// diagonal is a folded per-row array (no diagonal terms) and the lookup is
+// the combinadic rank of one hard-core-boson sector (through the sector
+// table, or directly without symmetry).  Same elements as k_hv_rows without
+// the generic operator-string and multi-mode rank paths in the loop.
+template <typename I, typename VT, int M, int LK>
+__global__ void __launch_bounds__(256) k_hv_rows_fast(HvArgs A) {
+  static_assert(LK == LOOKUP_TABLE || LK == LOOKUP_RANK, "rank-based lookups only");
+  __shared__ int64_t sb[QSX_SB];
+  if (A.skip && *A.skip) return;
+  stage_binom<LK>(A.R, sb);
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= A.Ns) return;
+  constexpr bool CPLX = VTr<VT>::cplx;
+  const I s = ((const I*)A.basis)[r];
+  double rsq = 1.0;
+  if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, r));
+  const int Np = A.R->Np;
+  const int64_t ra = comb_rank_sb<I>(sb, s);
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ vout = (VT*)A.v_out;
+  double2 acc = make_double2(0.0, 0.0);
+  for (int t = 0; t < A.n_off; ++t) {
+    const TermDev* __restrict__ T = A.off + t;
+    const I v = s & (I)T->req_mask;
+    if (v != (I)T->req_val && !(T->has2 && v == (I)T->req_val2)) continue;
+    const I s2 = s ^ (I)T->flip;
+    if (popc<I>(s2) != Np) continue;
+    const int64_t rk = ra - window_rank<I>(sb, s, T->lo, T->hi) + window_rank<I>(sb, s2, T->lo, T->hi);
+    int64_t c;
+    double2 fac = make_double2(1.0, 0.0);
+    if constexpr (LK == LOOKUP_TABLE) {
+      if (!table_entry<I>(A, rk, c, fac)) continue;
+    } else {
+      c = A.R->Ns - 1 - rk;
+    }
+    const int zs = T->nz - popc<I>(s & (I)T->zmask);
+    double w = (zs & 1) ? -T->mag0 : T->mag0;
+    if constexpr (M != 0) w *= rsq;
+    const int ip = 3 * T->ny - 2 * popc<I>(s & (I)T->ymask);
+    const double2 x = VTr<VT>::ld(vin + c);
+    if constexpr (CPLX) {
+      double2 coef = cmul(make_double2(T->jr, T->ji), ipow_c(ip));
+      coef = make_double2(coef.x * w, coef.y * w);
+      if constexpr (M != 0) coef = cmul(coef, fac);
+      if (A.off_conj) coef = cconj(coef);
+      acc = cadd(acc, cmul(coef, x));
+    } else {
+      double vv = T->jr * w * ((ip & 2) ? -1.0 : 1.0);
+      if constexpr (M != 0) vv *= fac.x;
+      acc.x += vv * x.x;
+    }
+  }
+  double2 d = make_double2(0.0, 0.0);
+  if (A.diag_arr) {
+    if (A.diag_arr_cplx) d = ((const double2*)A.diag_arr)[r];
+    else d.x = ((const double*)A.diag_arr)[r];
+  }
+  const double2 dsum = cadd(cmul(d, A.alpha_diag), A.shift);
+  const double2 xr = VTr<VT>::ld(vin + r);
+  double2 res;
+  if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc), cmul(dsum, xr));
+  else res = make_double2(A.alpha_off.x * acc.x + dsum.x * xr.x, 0.0);
+  if (A.beta) res = cadd(res, VTr<VT>::ld(vout + r));
+  VTr<VT>::st(vout + r, res);
+}
+
 // One vector, G lanes per row: lane j of a row handles terms j, j+G, ... and
 // the partial sums are combined with cross-lane shuffles (more waves and
 // shorter per-lane dependency chains than one lane per row).
@@ -742,6 +812,10 @@ void launch_hv_one(const HvArgs& A, hipStream_t st) {
       if (A.sched == 5) hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 4>), grid, dim3(blk), 0, st, A);
       else hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 8>), grid, dim3(blk), 0, st, A);
     }
+  }
+  else if (A.nvec == 1 && A.fast_rows && A.sched == 0 && !FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
+    if constexpr (!FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK))
+      hipLaunchKernelGGL((k_hv_rows_fast<I, VT, M, LK>), grid, dim3(blk), 0, st, A);
   }
   else if (A.nvec == 1)
     hipLaunchKernelGGL((k_hv_rows<I, VT, M, FERM, LK, 1>), grid, dim3(blk), 0, st, A);
