@@ -469,7 +469,17 @@ __global__ void __launch_bounds__(256) k_hv_rows_fast(HvArgs A) {
     if (v != (I)T->req_val && !(T->has2 && v == (I)T->req_val2)) continue;
     const I s2 = s ^ (I)T->flip;
     if (popc<I>(s2) != Np) continue;
-    const int64_t rk = ra - window_rank<I>(sb, s, T->lo, T->hi) + window_rank<I>(sb, s2, T->lo, T->hi);
+    int64_t rk;
+    if (T->hi == T->lo + 1) {
+      // one particle hops between adjacent bits lo and lo + 1: with j
+      // particles below lo, its term of the rank changes by
+      // C(lo + 1, j + 1) - C(lo, j + 1) = C(lo, j)
+      const int lo = T->lo;
+      const int64_t dl = sb[lo * 33 + popc<I>(s & ((((I)1) << lo) - (I)1))];
+      rk = ((s >> lo) & (I)1) ? ra + dl : ra - dl;
+    } else {
+      rk = ra - window_rank<I>(sb, s, T->lo, T->hi) + window_rank<I>(sb, s2, T->lo, T->hi);
+    }
     int64_t c;
     double2 fac = make_double2(1.0, 0.0);
     if constexpr (LK == LOOKUP_TABLE) {
