@@ -31,7 +31,7 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.0, write
 # x1.0); keyed by (L, lookup, schedule, dtype).  Source and commands:
 # profiles/k_hv_rows_heisenberg_L28_kpz.md
-MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 88.4e6}
+MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 86.4e6}
 
 
 def chain_static(L, J=1.0, Delta=1.0):
