@@ -395,6 +395,97 @@ extern "C" int qsx_lanczos_update(int dtype, const void* v, const double* b_prev
   return QSX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Lanczos recurrence without fences: a kernel writes one partial per block
+// (plain stores); the next kernel, after the launch boundary, sums them in a
+// fixed order on every block (the same value everywhere, deterministic) and
+// block 0 records the scalar.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nb) {
+  __shared__ double sh[256];
+  double a = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) a += part[k];
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ part) {
+  __shared__ double sh[256];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// b = sqrt(sum of the norm partials) -> bout; q = r / b
+template <typename VT>
+__global__ void __launch_bounds__(256) k_lzr_div(const VT* __restrict__ r, const double* __restrict__ pnrm, int nb,
+                                                 VT* __restrict__ q, int64_t n, double* __restrict__ bout) {
+  const double b = sqrt(sum_partials(pnrm, nb));
+  if (blockIdx.x == 0 && threadIdx.x == 0) bout[0] = b;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 v = VTr<VT>::ld(r + i);
+    VTr<VT>::st(q + i, make_double2(v.x / b, v.y / b));
+  }
+}
+
+// r -= b_prev v (b_prev device scalar, or none); partial of Re <q, r>
+template <typename VT>
+__global__ void __launch_bounds__(256) k_lzr_dot(const VT* __restrict__ v, const double* __restrict__ bprev,
+                                                 const VT* __restrict__ q, VT* __restrict__ r, int64_t n,
+                                                 double* __restrict__ pdot) {
+  const double b = bprev ? bprev[0] : 0.0;
+  double ar = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 rv = VTr<VT>::ld(r + i);
+    if (bprev) {
+      const double2 vv = VTr<VT>::ld(v + i);
+      rv.x -= b * vv.x;
+      rv.y -= b * vv.y;
+      VTr<VT>::st(r + i, rv);
+      rv = VTr<VT>::rnd(rv);
+    }
+    const double2 qq = VTr<VT>::ld(q + i);
+    ar += qq.x * rv.x + qq.y * rv.y;
+  }
+  block_partial(ar, pdot);
+}
+
+// a = sum of the dot partials -> aout; r -= a q; partial of ||r||^2
+template <typename VT>
+__global__ void __launch_bounds__(256) k_lzr_nrm(const double* __restrict__ pdot, int nb, const VT* __restrict__ q,
+                                                 VT* __restrict__ r, int64_t n, double* __restrict__ aout,
+                                                 double* __restrict__ pnrm) {
+  const double a = sum_partials(pdot, nb);
+  if (blockIdx.x == 0 && threadIdx.x == 0) aout[0] = a;
+  double s2 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double2 rv = VTr<VT>::ld(r + i);
+    const double2 qq = VTr<VT>::ld(q + i);
+    rv.x -= a * qq.x;
+    rv.y -= a * qq.y;
+    VTr<VT>::st(r + i, rv);
+    rv = VTr<VT>::rnd(rv);
+    s2 += rv.x * rv.x + rv.y * rv.y;
+  }
+  block_partial(s2, pnrm);
+}
+
+__global__ void __launch_bounds__(256) k_lzr_fin(const double* __restrict__ pnrm, int nb, double* __restrict__ bout) {
+  const double b = sqrt(sum_partials(pnrm, nb));
+  if (threadIdx.x == 0) bout[0] = b;
+}
+
 // q = r / b (device scalar)
 extern "C" int qsx_lanczos_div(int dtype, const void* r, const double* b, void* q, int64_t n, void* stream) {
   hipStream_t st = (hipStream_t)stream;
@@ -1044,20 +1135,33 @@ extern "C" int qsx_lanczos_run(int op_kind, void* hv_ctx, const void* indptr, co
     if (op_kind == 0) return qsx_hv_ctx_apply(hv_ctx, x, r, 1.0, 0.0, 1.0, 0.0, 0, q);
     return qsx_csr_matvec(indptr, indices, data, idx64, mat_dtype, n, nnz, x, r, dtype, 1, 1.0, 0.0, 0, group, q);
   };
+  // per step: q_i = r / b_{i-1} (the norm partials of the previous step
+  // reduced on every block), r = A q_i, r -= b_{i-1} q_{i-1} with the
+  // partials of <q_i, r>, r -= a_i q_i with the partials of ||r||^2
+  const int nb = red_grid(n);
+  double* pdot = R.partials + 4 * RED_BLOCKS;
+  double* pnrm = R.partials + 6 * RED_BLOCKS;
   auto record = [&](hipStream_t q) -> int {
-    int e = apply(qi(0), q);
-    if (e) return e;
-    e = qsx_lanczos_update(dtype, nullptr, nullptr, qi(0), r, n, ab, ab + 2, q);
-    if (e) return e;
-    for (int i = 1; i < m; ++i) {
-      double* bp = ab + 4 * (int64_t)(i - 1) + 2;
-      e = qsx_lanczos_div(dtype, r, bp, qi(i), n, q);
+    for (int i = 0; i < m; ++i) {
+      double* bp = ab + 4 * (int64_t)(i - 1) + 2;  // b_{i-1}
+      if (i > 0) {
+        QSX_VT_SWITCH(dtype, hipLaunchKernelGGL(k_lzr_div<VT>, dim3(nb), dim3(256), 0, q, (const VT*)r, pnrm, nb,
+                                                (VT*)qi(i), n, bp));
+        QSX_LAUNCH_CHECK();
+      }
+      int e = apply(qi(i), q);
       if (e) return e;
-      e = apply(qi(i), q);
-      if (e) return e;
-      e = qsx_lanczos_update(dtype, qi(i - 1), bp, qi(i), r, n, ab + 4 * (int64_t)i, ab + 4 * (int64_t)i + 2, q);
-      if (e) return e;
+      QSX_VT_SWITCH(dtype, hipLaunchKernelGGL(k_lzr_dot<VT>, dim3(nb), dim3(256), 0, q,
+                                              i > 0 ? (const VT*)qi(i - 1) : (const VT*)nullptr,
+                                              i > 0 ? bp : (const double*)nullptr, (const VT*)qi(i), (VT*)r, n,
+                                              pdot));
+      QSX_LAUNCH_CHECK();
+      QSX_VT_SWITCH(dtype, hipLaunchKernelGGL(k_lzr_nrm<VT>, dim3(nb), dim3(256), 0, q, (const double*)pdot, nb,
+                                              (const VT*)qi(i), (VT*)r, n, ab + 4 * (int64_t)i, pnrm));
+      QSX_LAUNCH_CHECK();
     }
+    hipLaunchKernelGGL(k_lzr_fin, dim3(1), dim3(256), 0, q, (const double*)pnrm, nb, ab + 4 * (int64_t)(m - 1) + 2);
+    QSX_LAUNCH_CHECK();
     return QSX_OK;
   };
   if (!use_graph) return record(st);
