@@ -280,3 +280,43 @@ def test_forced_configs_vs_oracle(lookup, schedule, monkeypatch):
         cfg = Q._qsx_operator().config
         if len(B._pers):
             assert all(c["lookup"] == lookup for c in cfg.values()), cfg
+
+
+@pytest.mark.parametrize("lookup", ["table", "search"])
+def test_merged_term_pairs_vs_oracle(lookup, monkeypatch):
+    """Op lists whose '+-' / '-+' terms share a bond with equal couplings (one
+    merged term with two accepted bit patterns), unequal couplings (kept
+    apart), complex couplings and extra one-site flips, on a symmetric sector
+    (table or orbit search) and on a sector without symmetry (direct rank):
+    every product against the oracle's matrix."""
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    monkeypatch.setenv("QSX_LOOKUP", lookup)
+    monkeypatch.setenv("QSX_SCHEDULE", "rows")
+    L = 12
+    eq = [[0.5, i, (i + 1) % L] for i in range(L)]
+    # translation-invariant couplings (the k-sector product is the gather form)
+    ne_p = [[0.6, i, (i + 2) % L] for i in range(L)]
+    ne_m = [[0.8, i, (i + 2) % L] for i in range(L)]
+    cj = [[0.2 + 0.4j, i, (i + 3) % L] for i in range(L)]
+    static_c = [["+-", eq], ["-+", eq], ["+-", ne_p], ["-+", ne_m], ["+-", cj],
+                ["-+", [[np.conj(b[0]), b[1], b[2]] for b in cj]], ["zz", eq]]
+    static_r = [["+-", eq], ["-+", eq], ["+-", ne_p], ["-+", ne_m], ["zz", eq], ["z", [[0.3, 2]]]]
+    m = chain_maps(L)
+    cases = [(spin_basis_general(L, Nup=6, pauli=0, kblock=(m["T"], 2)), static_c, np.complex128),
+             (spin_basis_general(L, Nup=6, pauli=0), static_r, np.float64),
+             (spin_basis_general(L, Nup=5, pauli=0), static_c, np.complex128)]
+    for B, static, dt in cases:
+        print(f"case lookup={lookup} Ns={B.Ns} maps={len(B._pers)} dtype={np.dtype(dt).name}", flush=True)
+        Q = quantum_LinearOperator(static, basis=B, dtype=dt, check_symm=False, check_herm=False, check_pcon=False)
+        H = _oracle_H(B, static)
+        v = _vec(B.Ns, dt, seed=7)
+        got = Q.dot(v)
+        ref = H @ v.astype(np.complex128)
+        np.testing.assert_allclose(got, ref if np.dtype(dt).kind == "c" else ref.real, atol=1e-11, rtol=0)
+        # the adjoint applies the transposed term list; Q.T applies conj(H)
+        # (quantum_LinearOperator passes (conjugated, transposed) into the
+        # (transposed, conjugated) slots, quantum_LinearOperator_core.py:335-343)
+        for op, M in ((Q.H, H.conj().T), (Q.T, H.conj())):
+            e = M @ v.astype(np.complex128)
+            np.testing.assert_allclose(op.dot(v), e if np.dtype(dt).kind == "c" else e.real, atol=1e-11, rtol=0)
