@@ -445,77 +445,104 @@ __global__ void __launch_bounds__(256) k_hv_rows(HvArgs A) {
 // the combinadic rank of one hard-core-boson sector (through the sector
 // table, or directly without symmetry).  Same elements as k_hv_rows without
 // the generic operator-string and multi-mode rank paths in the loop.
-template <typename I, typename VT, int M, int LK>
+// RPL rows per lane (r, r + 256, ... within the block's 256 RPL rows): the
+// uniform term fields are read once for all of them.  Two pay off for the
+// direct rank (no table reads); with the table, one row per lane keeps more
+// waves in flight for the lookups (measured on MI355X).
+template <typename I, typename VT, int M, int LK, int RPL>
 __global__ void __launch_bounds__(256) k_hv_rows_fast(HvArgs A) {
   static_assert(LK == LOOKUP_TABLE || LK == LOOKUP_RANK, "rank-based lookups only");
   __shared__ int64_t sb[QSX_SB];
   if (A.skip && *A.skip) return;
   stage_binom<LK>(A.R, sb);
   __syncthreads();
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= A.Ns) return;
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x * RPL + threadIdx.x;
+  if (r0 >= A.Ns) return;
   constexpr bool CPLX = VTr<VT>::cplx;
-  const I s = ((const I*)A.basis)[r];
-  double rsq = 1.0;
-  if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, r));
-  const int Np = A.R->Np;
-  const int64_t ra = comb_rank_sb<I>(sb, s);
   const VT* __restrict__ vin = (const VT*)A.v_in;
   VT* __restrict__ vout = (VT*)A.v_out;
-  double2 acc = make_double2(0.0, 0.0);
+  const int Np = A.R->Np;
+  int64_t rr[RPL];
+  bool live[RPL];
+  I s[RPL];
+  int64_t ra[RPL];
+  double rsq[RPL];
+  double2 acc[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    rr[k] = r0 + (int64_t)k * blockDim.x;
+    live[k] = rr[k] < A.Ns;
+    const int64_t rc = live[k] ? rr[k] : r0;
+    s[k] = ((const I*)A.basis)[rc];
+    rsq[k] = 1.0;
+    if constexpr (M != 0) rsq[k] = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, rc));
+    ra[k] = comb_rank_sb<I>(sb, s[k]);
+    acc[k] = make_double2(0.0, 0.0);
+  }
   for (int t = 0; t < A.n_off; ++t) {
     const TermDev* __restrict__ T = A.off + t;
-    const I v = s & (I)T->req_mask;
-    if (v != (I)T->req_val && !(T->has2 && v == (I)T->req_val2)) continue;
-    const I s2 = s ^ (I)T->flip;
-    if (popc<I>(s2) != Np) continue;
-    int64_t rk;
-    if (T->hi == T->lo + 1) {
-      // one particle hops between adjacent bits lo and lo + 1: with j
-      // particles below lo, its term of the rank changes by
-      // C(lo + 1, j + 1) - C(lo, j + 1) = C(lo, j)
-      const int lo = T->lo;
-      const int64_t dl = sb[lo * 33 + popc<I>(s & ((((I)1) << lo) - (I)1))];
-      rk = ((s >> lo) & (I)1) ? ra + dl : ra - dl;
-    } else {
-      rk = ra - window_rank<I>(sb, s, T->lo, T->hi) + window_rank<I>(sb, s2, T->lo, T->hi);
-    }
-    int64_t c;
-    double2 fac = make_double2(1.0, 0.0);
-    if constexpr (LK == LOOKUP_TABLE) {
-      if (!table_entry<I>(A, rk, c, fac)) continue;
-    } else {
-      c = A.R->Ns - 1 - rk;
-    }
-    const int zs = T->nz - popc<I>(s & (I)T->zmask);
-    double w = (zs & 1) ? -T->mag0 : T->mag0;
-    if constexpr (M != 0) w *= rsq;
-    const int ip = 3 * T->ny - 2 * popc<I>(s & (I)T->ymask);
-    const double2 x = VTr<VT>::ld(vin + c);
-    if constexpr (CPLX) {
-      double2 coef = cmul(make_double2(T->jr, T->ji), ipow_c(ip));
-      coef = make_double2(coef.x * w, coef.y * w);
-      if constexpr (M != 0) coef = cmul(coef, fac);
-      if (A.off_conj) coef = cconj(coef);
-      acc = cadd(acc, cmul(coef, x));
-    } else {
-      double vv = T->jr * w * ((ip & 2) ? -1.0 : 1.0);
-      if constexpr (M != 0) vv *= fac.x;
-      acc.x += vv * x.x;
+    const I rq_m = (I)T->req_mask, rq_v = (I)T->req_val, rq_v2 = (I)T->req_val2, fl = (I)T->flip;
+    const bool has2 = T->has2 != 0;
+    const int lo = T->lo, hi = T->hi;
+#pragma unroll
+    for (int k = 0; k < RPL; ++k) {
+      const I sk = s[k];
+      const I v = sk & rq_m;
+      if (v != rq_v && !(has2 && v == rq_v2)) continue;
+      const I s2 = sk ^ fl;
+      if (popc<I>(s2) != Np) continue;
+      int64_t rk;
+      if (hi == lo + 1) {
+        // one particle hops between adjacent bits lo and lo + 1: with j
+        // particles below lo, its term of the rank changes by
+        // C(lo + 1, j + 1) - C(lo, j + 1) = C(lo, j)
+        const int64_t dl = sb[lo * 33 + popc<I>(sk & ((((I)1) << lo) - (I)1))];
+        rk = ((sk >> lo) & (I)1) ? ra[k] + dl : ra[k] - dl;
+      } else {
+        rk = ra[k] - window_rank<I>(sb, sk, lo, hi) + window_rank<I>(sb, s2, lo, hi);
+      }
+      int64_t c;
+      double2 fac = make_double2(1.0, 0.0);
+      if constexpr (LK == LOOKUP_TABLE) {
+        if (!table_entry<I>(A, rk, c, fac)) continue;
+      } else {
+        c = A.R->Ns - 1 - rk;
+      }
+      const int zs = T->nz - popc<I>(sk & (I)T->zmask);
+      double w = (zs & 1) ? -T->mag0 : T->mag0;
+      if constexpr (M != 0) w *= rsq[k];
+      const int ip = 3 * T->ny - 2 * popc<I>(sk & (I)T->ymask);
+      const double2 x = VTr<VT>::ld(vin + c);
+      if constexpr (CPLX) {
+        double2 coef = cmul(make_double2(T->jr, T->ji), ipow_c(ip));
+        coef = make_double2(coef.x * w, coef.y * w);
+        if constexpr (M != 0) coef = cmul(coef, fac);
+        if (A.off_conj) coef = cconj(coef);
+        acc[k] = cadd(acc[k], cmul(coef, x));
+      } else {
+        double vv = T->jr * w * ((ip & 2) ? -1.0 : 1.0);
+        if constexpr (M != 0) vv *= fac.x;
+        acc[k].x += vv * x.x;
+      }
     }
   }
-  double2 d = make_double2(0.0, 0.0);
-  if (A.diag_arr) {
-    if (A.diag_arr_cplx) d = ((const double2*)A.diag_arr)[r];
-    else d.x = ((const double*)A.diag_arr)[r];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    if (!live[k]) continue;
+    const int64_t r = rr[k];
+    double2 d = make_double2(0.0, 0.0);
+    if (A.diag_arr) {
+      if (A.diag_arr_cplx) d = ((const double2*)A.diag_arr)[r];
+      else d.x = ((const double*)A.diag_arr)[r];
+    }
+    const double2 dsum = cadd(cmul(d, A.alpha_diag), A.shift);
+    const double2 xr = VTr<VT>::ld(vin + r);
+    double2 res;
+    if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc[k]), cmul(dsum, xr));
+    else res = make_double2(A.alpha_off.x * acc[k].x + dsum.x * xr.x, 0.0);
+    if (A.beta) res = cadd(res, VTr<VT>::ld(vout + r));
+    VTr<VT>::st(vout + r, res);
   }
-  const double2 dsum = cadd(cmul(d, A.alpha_diag), A.shift);
-  const double2 xr = VTr<VT>::ld(vin + r);
-  double2 res;
-  if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc), cmul(dsum, xr));
-  else res = make_double2(A.alpha_off.x * acc.x + dsum.x * xr.x, 0.0);
-  if (A.beta) res = cadd(res, VTr<VT>::ld(vout + r));
-  VTr<VT>::st(vout + r, res);
 }
 
 // One vector, G lanes per row: lane j of a row handles terms j, j+G, ... and
@@ -824,8 +851,11 @@ void launch_hv_one(const HvArgs& A, hipStream_t st) {
     }
   }
   else if (A.nvec == 1 && A.fast_rows && A.sched == 0 && !FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
-    if constexpr (!FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK))
-      hipLaunchKernelGGL((k_hv_rows_fast<I, VT, M, LK>), grid, dim3(blk), 0, st, A);
+    if constexpr (!FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
+      constexpr int RPL = LK == LOOKUP_RANK ? 2 : 1;
+      hipLaunchKernelGGL((k_hv_rows_fast<I, VT, M, LK, RPL>), dim3((unsigned)qsx_grid(A.Ns, RPL * blk)), dim3(blk),
+                         0, st, A);
+    }
   }
   else if (A.nvec == 1)
     hipLaunchKernelGGL((k_hv_rows<I, VT, M, FERM, LK, 1>), grid, dim3(blk), 0, st, A);
