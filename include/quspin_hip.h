@@ -138,8 +138,10 @@ int qsx_hv_ctx_create(qsx_core* core, qsx_oplist* l, const void* basis, const vo
                       const int64_t* begin, const int64_t* end, int N_p, int dtype, int64_t nvec, int transposed,
                       int conjugated, int diag_conj, const void* diag_arr, int diag_arr_cplx, qsx_hv_ctx** out);
 void qsx_hv_ctx_destroy(qsx_hv_ctx* ctx);
-/* lut: the coefficient table of qsx_table_build (NULL when lut_len was 0) */
-int qsx_hv_ctx_set_table(qsx_hv_ctx* ctx, const void* table, int entry_bytes, int idx_bits, const void* lut);
+/* lut: the coefficient table of qsx_table_build and its length (NULL and 0
+ * when qsx_table_info gave lut_len 0) */
+int qsx_hv_ctx_set_table(qsx_hv_ctx* ctx, const void* table, int entry_bytes, int idx_bits, const void* lut,
+                         int lut_len);
 int qsx_hv_ctx_set_schedule(qsx_hv_ctx* ctx, int sched);
 /* Replace the context's diagonal (diagonal terms and extra diagonal) by a
  * precomputed per-row array (Ns doubles, or Ns (re, im) pairs when

@@ -54,7 +54,7 @@ SIGNATURES = {
     "qsx_hv_ctx_create": (_i32, [_vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _i32, _i32, _i64, _i32, _i32, _i32, _vp,
                                  _i32, C.POINTER(_vp)]),
     "qsx_hv_ctx_destroy": (None, [_vp]),
-    "qsx_hv_ctx_set_table": (_i32, [_vp, _vp, _i32, _i32, _vp]),
+    "qsx_hv_ctx_set_table": (_i32, [_vp, _vp, _i32, _i32, _vp, _i32]),
     "qsx_hv_ctx_set_diag": (_i32, [_vp, _vp, _i32]),
     "qsx_hv_ctx_set_schedule": (_i32, [_vp, _i32]),
     "qsx_table_info": (_i32, [_vp, _i64, _vp, _i32, C.POINTER(_i64), C.POINTER(_i32), C.POINTER(_i32),

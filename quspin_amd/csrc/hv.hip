@@ -5,6 +5,7 @@
 // (src/quspin/basis/base.py:265-304).
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -84,6 +85,17 @@ int compile_term(const qsx_core* c, const char* opstr, const int32_t* indx, int 
           default: break;
         }
       }
+      double re = T.jr * T.mag0, im = T.ji * T.mag0;
+      if (T.nz & 1) { re = -re; im = -im; }
+      switch ((3 * T.ny) & 3) {  // times i^(3 ny)
+        case 1: { const double t = re; re = -im; im = t; } break;
+        case 2: re = -re; im = -im; break;
+        case 3: { const double t = re; re = im; im = -t; } break;
+        default: break;
+      }
+      T.cr = re;
+      T.ci = im;
+      T.smask = T.zmask ^ T.ymask;
     }
   }
   return QSX_OK;
@@ -231,12 +243,14 @@ static int hv_launch(const qsx_core* c, const qsx_oplist_full* L, HvArgs A, int 
   const int N = c->h.N, nm = c->h.nmaps;
   const bool fm = c->h.fermion != 0;
   const bool scatter = nm > 0 && !L->symmetric && !transposed && L->base.n_off > 0;
+  // binomial rows 0..N, rounded up to an even count (16-byte aligned LDS after them)
+  A.n_sb = ((std::min(65, c->rank_h.N + 1) * 33) + 1) & ~1;
   if (!scatter) {
     A.off = transposed ? L->base.off : L->off_hc;
     A.n_off = L->base.n_off;
     A.off_conj = (transposed == conjugated) ? 1 : 0;
     A.fast_rows = (L->all_fast && A.n_diag == 0 && !fm && c->rank_h.mode == 0 &&
-                   (lookup == LOOKUP_TABLE || lookup == LOOKUP_RANK)) ? 1 : 0;
+                   ((lookup == LOOKUP_TABLE && A.tlut) || lookup == LOOKUP_RANK)) ? 1 : 0;
     return dispatch_hv(A, dtype, N, nm, fm, lookup, 0, st);
   }
   A.fast_rows = 0;
@@ -632,12 +646,17 @@ struct qsx_hv_ctx {
 };
 
 // Use a sector lookup table (qsx_table_build) instead of the orbit search.
-extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_bytes, int idx_bits, const void* lut) {
-  if (!X || (table && entry_bytes != 4 && entry_bytes != 8)) { qsx_set_error("qsx_hv_ctx_set_table: bad arguments"); return QSX_ERR_ARG; }
+extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_bytes, int idx_bits, const void* lut,
+                                    int lut_len) {
+  if (!X || (table && entry_bytes != 4 && entry_bytes != 8) || lut_len < 0 || (lut && lut_len == 0)) {
+    qsx_set_error("qsx_hv_ctx_set_table: bad arguments");
+    return QSX_ERR_ARG;
+  }
   X->A.table = table;
   X->A.table_u64 = entry_bytes == 8;
   X->A.idx_bits = idx_bits;
   X->A.tlut = table ? (const double2*)lut : nullptr;
+  X->A.lut_len = X->A.tlut ? lut_len : 0;
   return QSX_OK;
 }
 

@@ -13,6 +13,7 @@
 // original, no conj; H^*: Op^+, no conj; H^H: original, conj).  Diagonal terms
 // are evaluated on the fly.  Accumulation is FP64 for every vector dtype.
 #pragma once
+#include <type_traits>
 #include "qsx_host.h"
 
 template <typename VT> struct VTr;
@@ -114,6 +115,8 @@ struct HvArgs {
   // with a coefficient table the entry's code field is an id into tlut
   // (sign-free chi(g) * sqrt(n_c) of the target), so no n_c read is needed
   const double2* tlut;
+  int lut_len;        // entries of tlut
+  int n_sb;           // binomial entries staged by k_hv_rows_fast_batch (even)
   int sched;          // 0: one lane per row, 1: compacted queue (one vector)
   int fast_rows;      // every off-diagonal term bit-parallel, no diagonal terms, single hcb sector
 };
@@ -143,13 +146,16 @@ __device__ __forceinline__ int64_t lookup_index(const HvArgs& A, const int64_t* 
 }
 
 // Binomial table C(n, k), n <= 64, k <= 32, staged in LDS by the kernels
-// that rank states (rows of 33 entries).
+// that rank states (rows of 33 entries).  Ranks only read rows n < N (bit
+// positions), so a block stages rows 0..N and leaves the rest unset.
 constexpr int QSX_SB = 65 * 33;
 
 template <int LK>
 __device__ __forceinline__ void stage_binom(const RankDev* __restrict__ R, int64_t* sb) {
   if constexpr (LK == LOOKUP_TABLE || LK == LOOKUP_RANK) {
-    for (int i = threadIdx.x; i < QSX_SB; i += blockDim.x) sb[i] = R->binom[i / 33][i % 33];
+    const int n = min(QSX_SB, (R->N + 1) * 33);
+    const int64_t* __restrict__ b = &R->binom[0][0];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sb[i] = b[i];
   }
 }
 
@@ -446,9 +452,9 @@ __global__ void __launch_bounds__(256) k_hv_rows(HvArgs A) {
 // table, or directly without symmetry).  Same elements as k_hv_rows without
 // the generic operator-string and multi-mode rank paths in the loop.
 // RPL rows per lane (r, r + 256, ... within the block's 256 RPL rows): the
-// uniform term fields are read once for all of them.  Two pay off for the
-// direct rank (no table reads); with the table, one row per lane keeps more
-// waves in flight for the lookups (measured on MI355X).
+// uniform term fields are read once for all of them.  One term at a time:
+// a wave skips a term no lane of it needs, and a lane skips the loads of a
+// term that gives its row nothing.
 template <typename I, typename VT, int M, int LK, int RPL>
 __global__ void __launch_bounds__(256) k_hv_rows_fast(HvArgs A) {
   static_assert(LK == LOOKUP_TABLE || LK == LOOKUP_RANK, "rank-based lookups only");
@@ -503,26 +509,208 @@ __global__ void __launch_bounds__(256) k_hv_rows_fast(HvArgs A) {
       }
       int64_t c;
       double2 fac = make_double2(1.0, 0.0);
+      double2 x;
       if constexpr (LK == LOOKUP_TABLE) {
-        if (!table_entry<I>(A, rk, c, fac)) continue;
+        // decode the entry, then issue the x gather and the coefficient
+        // read together: both depend only on the entry
+        int code;
+        double sg = 1.0;
+        if (A.table_u64) {
+          const uint64_t e = ((const uint64_t*)A.table)[rk];
+          if (e == ~0ull) continue;
+          c = (int64_t)(e & ((1ull << 40) - 1));
+          code = (int)((e >> 40) & 0x7FFFFF);
+          if (e >> 63) sg = -1.0;
+        } else {
+          const uint32_t e = ((const uint32_t*)A.table)[rk];
+          if (e == 0xFFFFFFFFu) continue;
+          c = (int64_t)(e & ((1u << A.idx_bits) - 1u));
+          code = (int)(e >> A.idx_bits);
+        }
+        x = VTr<VT>::ld(vin + c);
+        const double2 f = A.tlut[code];
+        fac = make_double2(sg * f.x, sg * f.y);
       } else {
         c = A.R->Ns - 1 - rk;
+        x = VTr<VT>::ld(vin + c);
       }
-      const int zs = T->nz - popc<I>(sk & (I)T->zmask);
-      double w = (zs & 1) ? -T->mag0 : T->mag0;
+      double w = (popc<I>(sk & (I)T->smask) & 1) ? -1.0 : 1.0;
       if constexpr (M != 0) w *= rsq[k];
-      const int ip = 3 * T->ny - 2 * popc<I>(sk & (I)T->ymask);
-      const double2 x = VTr<VT>::ld(vin + c);
       if constexpr (CPLX) {
-        double2 coef = cmul(make_double2(T->jr, T->ji), ipow_c(ip));
-        coef = make_double2(coef.x * w, coef.y * w);
+        double2 coef = make_double2(T->cr * w, T->ci * w);
         if constexpr (M != 0) coef = cmul(coef, fac);
         if (A.off_conj) coef = cconj(coef);
         acc[k] = cadd(acc[k], cmul(coef, x));
       } else {
-        double vv = T->jr * w * ((ip & 2) ? -1.0 : 1.0);
+        double vv = T->cr * w;
         if constexpr (M != 0) vv *= fac.x;
         acc[k].x += vv * x.x;
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    if (!live[k]) continue;
+    const int64_t r = rr[k];
+    double2 d = make_double2(0.0, 0.0);
+    if (A.diag_arr) {
+      if (A.diag_arr_cplx) d = ((const double2*)A.diag_arr)[r];
+      else d.x = ((const double*)A.diag_arr)[r];
+    }
+    const double2 dsum = cadd(cmul(d, A.alpha_diag), A.shift);
+    const double2 xr = VTr<VT>::ld(vin + r);
+    double2 res;
+    if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc[k]), cmul(dsum, xr));
+    else res = make_double2(A.alpha_off.x * acc[k].x + dsum.x * xr.x, 0.0);
+    if (A.beta) res = cadd(res, VTr<VT>::ld(vout + r));
+    VTr<VT>::st(vout + r, res);
+  }
+}
+
+// k_hv_rows_fast with the terms in batches of B: first the B ranks, then
+// the B table reads (TW-byte entries; TW = 0: direct rank, no table), then
+// the B gathers of x (and coefficient reads), then the B products, so a lane
+// keeps B independent requests in flight instead of one dependent chain per
+// term.  Rows a term does not reach read entry 0 of the table and of x and
+// add zero: the loads stay unconditional, so none waits behind a branch.
+template <typename I, typename VT, int M, int LK, int RPL, int B, int TW, bool LUTL>
+__global__ void __launch_bounds__(256) k_hv_rows_fast_batch(HvArgs A) {
+  static_assert(LK == LOOKUP_TABLE || LK == LOOKUP_RANK, "rank-based lookups only");
+  static_assert((LK == LOOKUP_TABLE) == (TW == 4 || TW == 8), "table entries are 4 or 8 bytes");
+  using E = typename std::conditional<TW == 8, uint64_t, uint32_t>::type;
+  // dynamic LDS: A.n_sb binomials (rows 0..N), then with LUTL the lut_len
+  // coefficients (read per element: from LDS rather than through the vector
+  // memory path)
+  extern __shared__ int64_t fast_smem[];
+  int64_t* sb = fast_smem;
+  double2* slut = (double2*)(fast_smem + A.n_sb);
+  if (A.skip && *A.skip) return;
+  {
+    const int n = min(A.n_sb, QSX_SB);
+    const int64_t* __restrict__ bsrc = &A.R->binom[0][0];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) sb[i] = bsrc[i];
+  }
+  if constexpr (LUTL)
+    for (int i = threadIdx.x; i < A.lut_len; i += blockDim.x) slut[i] = A.tlut[i];
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x * RPL + threadIdx.x;
+  if (r0 >= A.Ns) return;
+  constexpr bool CPLX = VTr<VT>::cplx;
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ vout = (VT*)A.v_out;
+  const int Np = A.R->Np;
+  int64_t rr[RPL];
+  bool live[RPL];
+  I s[RPL];
+  int64_t ra[RPL];
+  double rsq[RPL];
+  double2 acc[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    rr[k] = r0 + (int64_t)k * blockDim.x;
+    live[k] = rr[k] < A.Ns;
+    const int64_t rc = live[k] ? rr[k] : r0;
+    s[k] = ((const I*)A.basis)[rc];
+    rsq[k] = 1.0;
+    if constexpr (M != 0) rsq[k] = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, rc));
+    ra[k] = comb_rank_sb<I>(sb, s[k]);
+    acc[k] = make_double2(0.0, 0.0);
+  }
+  const int nt = A.n_off;
+  for (int t0 = 0; t0 < nt; t0 += B) {
+    // ranks of the images (rank 0, a valid index, where the term gives none)
+    int64_t rk[B][RPL];
+    bool ok[B][RPL];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const bool tv = t0 + b < nt;
+      const TermDev* __restrict__ T = A.off + (tv ? t0 + b : nt - 1);
+      const I rq_m = (I)T->req_mask, rq_v = (I)T->req_val, rq_v2 = (I)T->req_val2, fl = (I)T->flip;
+      const bool has2 = T->has2 != 0;
+      const int lo = T->lo, hi = T->hi;
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const I sk = s[k];
+        const I v = sk & rq_m;
+        const I s2 = sk ^ fl;
+        const bool a = tv && (v == rq_v || (has2 && v == rq_v2)) && popc<I>(s2) == Np;
+        int64_t q = 0;
+        if (hi == lo + 1) {
+          // one particle hops between adjacent bits lo and lo + 1: with j
+          // particles below lo, its term of the rank changes by
+          // C(lo + 1, j + 1) - C(lo, j + 1) = C(lo, j)
+          const int64_t dl = sb[lo * 33 + popc<I>(sk & ((((I)1) << lo) - (I)1))];
+          q = ((sk >> lo) & (I)1) ? ra[k] + dl : ra[k] - dl;
+        } else if (a) {
+          q = ra[k] - window_rank<I>(sb, sk, lo, hi) + window_rank<I>(sb, s2, lo, hi);
+        }
+        ok[b][k] = a;
+        rk[b][k] = a ? q : 0;
+      }
+    }
+    // basis index and element factor of each image, and the gathers
+    double2 x[B][RPL], fac[B][RPL];
+    if constexpr (LK == LOOKUP_TABLE) {
+      E e[B][RPL];
+#pragma unroll
+      for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) e[b][k] = ((const E*)A.table)[rk[b][k]];
+#pragma unroll
+      for (int b = 0; b < B; ++b) {
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          const E ek = e[b][k];
+          const bool hit = ok[b][k] && ek != ~(E)0;
+          int64_t c;
+          int code;
+          double sg = 1.0;
+          if constexpr (TW == 8) {
+            c = (int64_t)(ek & ((1ull << 40) - 1));
+            code = (int)((ek >> 40) & 0x7FFFFF);
+            if (ek >> 63) sg = -1.0;
+          } else {
+            c = (int64_t)(ek & ((1u << A.idx_bits) - 1u));
+            code = (int)(ek >> A.idx_bits);
+          }
+          c = hit ? c : 0;
+          code = hit ? code : 0;
+          ok[b][k] = hit;
+          x[b][k] = VTr<VT>::ld(vin + c);
+          double2 f;
+          if constexpr (LUTL) f = slut[code];
+          else f = A.tlut[code];
+          fac[b][k] = make_double2(sg * f.x, sg * f.y);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int b = 0; b < B; ++b)
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) {
+          x[b][k] = VTr<VT>::ld(vin + (A.R->Ns - 1 - rk[b][k]));
+          fac[b][k] = make_double2(1.0, 0.0);
+        }
+    }
+    // products (zero weight where the term gives no element)
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const TermDev* __restrict__ T = A.off + (t0 + b < nt ? t0 + b : nt - 1);
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        double w = (popc<I>(s[k] & (I)T->smask) & 1) ? -1.0 : 1.0;
+        if constexpr (M != 0) w *= rsq[k];
+        w = ok[b][k] ? w : 0.0;
+        if constexpr (CPLX) {
+          double2 coef = make_double2(T->cr * w, T->ci * w);
+          if constexpr (M != 0) coef = cmul(coef, fac[b][k]);
+          if (A.off_conj) coef = cconj(coef);
+          acc[k] = cadd(acc[k], cmul(coef, x[b][k]));
+        } else {
+          double vv = T->cr * w;
+          if constexpr (M != 0) vv *= fac[b][k].x;
+          acc[k].x += vv * x[b][k].x;
+        }
       }
     }
   }
@@ -829,6 +1017,53 @@ __global__ void __launch_bounds__(256) k_hv_scatter(HvArgs A) {
     default: { constexpr int MM = -1; BODY; } break;           \
   }
 
+// Fast-row launch.  RPL = 2 rows per lane for the direct rank (no table
+// reads); with the table one row per lane keeps more waves in flight for the
+// lookups.  nb terms per batch (1: k_hv_rows_fast; 2, 4, 8:
+// k_hv_rows_fast_batch); nb = 0 picks the default measured on MI355X: 4 with
+// the table, 1 for the direct rank, where the branch-free batches lose to
+// skipping the gathers of terms that give a row nothing.
+// Coefficient tables up to this many entries (16 KB) go to LDS.
+constexpr int QSX_LUT_LDS = 1024;
+
+template <typename I, typename VT, int M, int LK, int RPL, int TW, bool LUTL>
+void launch_hv_fast_l(const HvArgs& A, int nb, size_t smem, hipStream_t st) {
+  const int blk = 256;
+  const dim3 grid((unsigned)qsx_grid(A.Ns, RPL * blk));
+  if (nb == 8) hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 8, TW, LUTL>), grid, dim3(blk), smem, st, A);
+  else if (nb == 4) hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 4, TW, LUTL>), grid, dim3(blk), smem, st, A);
+  else hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 2, TW, LUTL>), grid, dim3(blk), smem, st, A);
+}
+
+template <typename I, typename VT, int M, int LK, int RPL, int TW>
+void launch_hv_fast_b(HvArgs A, int nb, hipStream_t st) {
+  if (nb <= 1) {
+    hipLaunchKernelGGL((k_hv_rows_fast<I, VT, M, LK, RPL>), dim3((unsigned)qsx_grid(A.Ns, RPL * 256)), dim3(256), 0,
+                       st, A);
+    return;
+  }
+  if (A.n_sb <= 0) A.n_sb = QSX_SB + 1;  // every row (the caller did not size it)
+  const size_t sb_bytes = (size_t)A.n_sb * sizeof(int64_t);
+  if constexpr (TW != 0) {
+    if (A.tlut && A.lut_len > 0 && A.lut_len <= QSX_LUT_LDS) {
+      launch_hv_fast_l<I, VT, M, LK, RPL, TW, true>(A, nb, sb_bytes + (size_t)A.lut_len * sizeof(double2), st);
+      return;
+    }
+  }
+  launch_hv_fast_l<I, VT, M, LK, RPL, TW, false>(A, nb, sb_bytes, st);
+}
+
+template <typename I, typename VT, int M, int LK>
+void launch_hv_fast(const HvArgs& A, int nb, hipStream_t st) {
+  if constexpr (LK == LOOKUP_RANK) {
+    launch_hv_fast_b<I, VT, M, LK, 2, 0>(A, nb == 0 ? 1 : nb, st);
+  } else {
+    if (nb == 0) nb = 4;
+    if (A.table_u64) launch_hv_fast_b<I, VT, M, LK, 1, 8>(A, nb, st);
+    else launch_hv_fast_b<I, VT, M, LK, 1, 4>(A, nb, st);
+  }
+}
+
 // Dispatch over (maps, fermion sign, lookup, schedule, NV).
 // schedule codes: 0 one lane per row, 1 compacted queue, 2/3/4 = 2/4/8 lanes
 // per row, 5/6 = batches of 4/8 terms (rank-based lookups)
@@ -844,17 +1079,15 @@ void launch_hv_one(const HvArgs& A, hipStream_t st) {
     hipLaunchKernelGGL((k_hv_rows_g<I, VT, M, FERM, LK, 4>), dim3((unsigned)qsx_grid(A.Ns * 4, blk)), dim3(blk), 0, st, A);
   else if (A.nvec == 1 && A.sched == 4)
     hipLaunchKernelGGL((k_hv_rows_g<I, VT, M, FERM, LK, 8>), dim3((unsigned)qsx_grid(A.Ns * 8, blk)), dim3(blk), 0, st, A);
+  else if (A.nvec == 1 && A.fast_rows && (A.sched == 0 || A.sched == 5 || A.sched == 6) && !FERM &&
+           (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
+    if constexpr (!FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK))
+      launch_hv_fast<I, VT, M, LK>(A, A.sched == 0 ? 0 : A.sched == 5 ? 4 : 8, st);
+  }
   else if (A.nvec == 1 && (A.sched == 5 || A.sched == 6) && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
     if constexpr (LK == LOOKUP_TABLE || LK == LOOKUP_RANK) {
       if (A.sched == 5) hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 4>), grid, dim3(blk), 0, st, A);
       else hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 8>), grid, dim3(blk), 0, st, A);
-    }
-  }
-  else if (A.nvec == 1 && A.fast_rows && A.sched == 0 && !FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
-    if constexpr (!FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
-      constexpr int RPL = LK == LOOKUP_RANK ? 2 : 1;
-      hipLaunchKernelGGL((k_hv_rows_fast<I, VT, M, LK, RPL>), dim3((unsigned)qsx_grid(A.Ns, RPL * blk)), dim3(blk),
-                         0, st, A);
     }
   }
   else if (A.nvec == 1)

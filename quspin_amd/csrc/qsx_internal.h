@@ -63,6 +63,12 @@ struct TermDev {
   uint64_t req_mask, req_val, zmask, ymask, req_val2;
   double mag0;
   int32_t fast, nz, ny, has2;
+  // the same element with the state-independent factors folded:
+  // J * amplitude = (cr + i ci) * (-1)^popc(s & smask), where
+  // cr + i ci = J mag0 (-1)^nz i^(3 ny) and smask = zmask ^ ymask
+  // (i^(-2 p) = (-1)^p, and popc(a) + popc(b) = popc(a ^ b) mod 2)
+  double cr, ci;
+  uint64_t smask;
 };
 
 // Sector description used by LOOKUP_RANK (combinadic index, no symmetry).

@@ -377,13 +377,14 @@ class _Ctx:
             tab = basis.lookup_table()
             if tab is None:
                 self.cfg["lookup"] = "search"
-                L.check(lib.qsx_hv_ctx_set_table(self.h, None, 0, 0, None), "qsx_hv_ctx_set_table")
+                L.check(lib.qsx_hv_ctx_set_table(self.h, None, 0, 0, None, 0), "qsx_hv_ctx_set_table")
             else:
                 t, eb, ib, lut = tab
-                L.check(lib.qsx_hv_ctx_set_table(self.h, ptr(t), eb, ib, ptr(lut) if lut is not None else None),
+                L.check(lib.qsx_hv_ctx_set_table(self.h, ptr(t), eb, ib, ptr(lut) if lut is not None else None,
+                                                  int(lut.numel()) if lut is not None else 0),
                         "qsx_hv_ctx_set_table")
         else:
-            L.check(lib.qsx_hv_ctx_set_table(self.h, None, 0, 0, None), "qsx_hv_ctx_set_table")
+            L.check(lib.qsx_hv_ctx_set_table(self.h, None, 0, 0, None, 0), "qsx_hv_ctx_set_table")
         L.check(lib.qsx_hv_ctx_set_schedule(self.h, tuning.SCHEDULES.get(cfg.get("schedule"), 0)),
                 "qsx_hv_ctx_set_schedule")
 

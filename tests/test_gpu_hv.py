@@ -320,3 +320,36 @@ def test_merged_term_pairs_vs_oracle(lookup, monkeypatch):
         for op, M in ((Q.H, H.conj().T), (Q.T, H.conj())):
             e = M @ v.astype(np.complex128)
             np.testing.assert_allclose(op.dot(v), e if np.dtype(dt).kind == "c" else e.real, atol=1e-11, rtol=0)
+
+
+@pytest.mark.parametrize("lookup", ["table", "search"])
+@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8"])
+def test_fast_rows_pauli_strings_vs_oracle(lookup, schedule, monkeypatch):
+    """The fast-row kernels (one term at a time, and batches of 4 and 8 terms)
+    on x / y / z strings: odd and even numbers of y (factors i^(3 ny)), z
+    factors on flipped and unflipped sites, complex couplings, a term count
+    that is not a multiple of the batch, on a k-sector (sector table) and on
+    a sector without symmetry (direct rank); products, adjoints and
+    transposes against the oracle's matrix."""
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    monkeypatch.setenv("QSX_LOOKUP", lookup)
+    monkeypatch.setenv("QSX_SCHEDULE", schedule)
+    L = 12
+    m = chain_maps(L)
+    b1 = [[0.7, i, (i + 1) % L] for i in range(L)]
+    b2 = [[0.3 - 0.2j, i, (i + 2) % L] for i in range(L)]
+    tri = [[0.45, i, (i + 1) % L, (i + 2) % L] for i in range(L)]
+    static = [["xy", b1], ["yx", [[np.conj(b[0]), b[1], b[2]] for b in b1]], ["xx", b2],
+              ["yy", [[np.conj(b[0]), b[1], b[2]] for b in b2]], ["+z-", tri], ["-z+", tri], ["zz", b1],
+              ["yzy", tri]]
+    cases = [(spin_basis_general(L, Nup=6, pauli=0, kblock=(m["T"], 1)), np.complex128),
+             (spin_basis_general(L, Nup=5, pauli=0), np.complex128)]
+    for B, dt in cases:
+        print(f"case lookup={lookup} schedule={schedule} Ns={B.Ns} maps={len(B._pers)}", flush=True)
+        Q = quantum_LinearOperator(static, basis=B, dtype=dt, check_symm=False, check_herm=False, check_pcon=False)
+        H = _oracle_H(B, static)
+        v = _vec(B.Ns, dt, seed=11)
+        np.testing.assert_allclose(Q.dot(v), H @ v, atol=1e-11, rtol=0)
+        for op, M in ((Q.H, H.conj().T), (Q.T, H.conj())):
+            np.testing.assert_allclose(op.dot(v), M @ v, atol=1e-11, rtol=0)
