@@ -1019,10 +1019,11 @@ __global__ void __launch_bounds__(256) k_hv_scatter(HvArgs A) {
 
 // Fast-row launch.  RPL = 2 rows per lane for the direct rank (no table
 // reads); with the table one row per lane keeps more waves in flight for the
-// lookups.  nb terms per batch (1: k_hv_rows_fast; 2, 4, 8:
-// k_hv_rows_fast_batch); nb = 0 picks the default measured on MI355X: 4 with
-// the table, 1 for the direct rank, where the branch-free batches lose to
-// skipping the gathers of terms that give a row nothing.
+// lookups.  nb terms per batch (1: k_hv_rows_fast; 4..8:
+// k_hv_rows_fast_batch); nb = 0 picks the default measured on MI355X:
+// batches (fast_batch_default) with the table, 1 for the direct rank, where
+// the branch-free batches lose to skipping the gathers of terms that give a
+// row nothing.
 // Coefficient tables up to this many entries (16 KB) go to LDS.
 constexpr int QSX_LUT_LDS = 1024;
 
@@ -1030,9 +1031,23 @@ template <typename I, typename VT, int M, int LK, int RPL, int TW, bool LUTL>
 void launch_hv_fast_l(const HvArgs& A, int nb, size_t smem, hipStream_t st) {
   const int blk = 256;
   const dim3 grid((unsigned)qsx_grid(A.Ns, RPL * blk));
-  if (nb == 8) hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 8, TW, LUTL>), grid, dim3(blk), smem, st, A);
-  else if (nb == 4) hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 4, TW, LUTL>), grid, dim3(blk), smem, st, A);
-  else hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 2, TW, LUTL>), grid, dim3(blk), smem, st, A);
+  switch (nb) {
+    case 5: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 5, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
+    case 6: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 6, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
+    case 7: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 7, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
+    case 8: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 8, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
+    default: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 4, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
+  }
+}
+
+// Default batch for n terms: the fewest batches of at most 8 terms, then the
+// smallest batch (4..8) that keeps that count, so the last batch is not
+// padded more than needed (28 terms: 4 batches of 7).
+inline int fast_batch_default(int n) {
+  if (n <= 4) return 4;
+  const int nbat = (n + 7) / 8;
+  const int b = (n + nbat - 1) / nbat;
+  return b < 4 ? 4 : b;
 }
 
 template <typename I, typename VT, int M, int LK, int RPL, int TW>
@@ -1058,7 +1073,7 @@ void launch_hv_fast(const HvArgs& A, int nb, hipStream_t st) {
   if constexpr (LK == LOOKUP_RANK) {
     launch_hv_fast_b<I, VT, M, LK, 2, 0>(A, nb == 0 ? 1 : nb, st);
   } else {
-    if (nb == 0) nb = 4;
+    if (nb == 0) nb = fast_batch_default(A.n_off);
     if (A.table_u64) launch_hv_fast_b<I, VT, M, LK, 1, 8>(A, nb, st);
     else launch_hv_fast_b<I, VT, M, LK, 1, 4>(A, nb, st);
   }
