@@ -325,7 +325,8 @@ def test_merged_term_pairs_vs_oracle(lookup, monkeypatch):
 @pytest.mark.parametrize("lookup", ["table", "search"])
 @pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8"])
 def test_fast_rows_pauli_strings_vs_oracle(lookup, schedule, monkeypatch):
-    """The fast-row kernels (one term at a time, and batches of 4 and 8 terms)
+    """The fast-row kernels (rows: one term at a time with the direct rank,
+    batches sized from the term count with the table; batches of 4 and 8)
     on x / y / z strings: odd and even numbers of y (factors i^(3 ny)), z
     factors on flipped and unflipped sites, complex couplings, a term count
     that is not a multiple of the batch, on a k-sector (sector table) and on
