@@ -331,14 +331,18 @@ class basis_general:
         b = self._dev.get("end")
         return np.array([], np.intp) if b is None else L.to_host(b)
 
-    def lookup_table(self, max_bytes=None):
+    def lookup_table(self, max_bytes=None, entry_bytes=None):
         """Device table mapping every state of the particle sector (by
         combinadic rank) to (basis index, group element) of its
         representative, built on first use; None when the basis has no
         symmetry, is not a single sector, or the table would exceed
-        ``max_bytes`` (default: 40 GiB)."""
+        ``max_bytes`` (default: 40 GiB).  ``entry_bytes=8`` builds 8-byte
+        entries where 4 would do (the layout large bases need); the first
+        build is kept for the basis."""
         if not self._made_basis or not len(self._pers):
             return None
+        if entry_bytes not in (None, 4, 8):
+            raise ValueError("entry_bytes must be 4 or 8")
         if "table" in self._dev:
             return self._dev["table"]
         lib = L.load()
@@ -349,6 +353,10 @@ class basis_general:
         if rc != 0:
             self._dev["table"] = None
             return None
+        if entry_bytes == 8:
+            eb.value = 8
+        elif entry_bytes == 4 and eb.value != 4:
+            raise ValueError("this basis needs 8-byte table entries")
         nbytes = ent.value * eb.value
         if nbytes > (max_bytes if max_bytes is not None else 40 * 2 ** 30):
             return None

@@ -354,3 +354,30 @@ def test_fast_rows_pauli_strings_vs_oracle(lookup, schedule, monkeypatch):
         np.testing.assert_allclose(Q.dot(v), H @ v, atol=1e-11, rtol=0)
         for op, M in ((Q.H, H.conj().T), (Q.T, H.conj())):
             np.testing.assert_allclose(op.dot(v), M @ v, atol=1e-11, rtol=0)
+
+
+@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8"])
+def test_fast_rows_u64_table_vs_oracle(schedule, monkeypatch):
+    """Sector tables with 8-byte entries (index | coefficient id << 40, the
+    layout large bases need), forced on small bases: the fast-row kernels
+    read them as the 4-byte ones, complex k-sector and real k=0/p/z sector,
+    against the oracle."""
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    monkeypatch.setenv("QSX_LOOKUP", "table")
+    monkeypatch.setenv("QSX_SCHEDULE", schedule)
+    L = 14
+    m = chain_maps(L)
+    cases = [(spin_basis_general(L, Nup=7, pauli=0, kblock=(m["T"], 3)), xxz_static(L, Delta=0.3), np.complex128),
+             (spin_basis_general(L, Nup=7, pauli=0, kblock=(m["T"], 0), pblock=(m["P"], 1), zblock=(m["Z"], 0)),
+              xxz_static(L, Delta=1.0), np.float64)]
+    for B, static, dt in cases:
+        tab = B.lookup_table(entry_bytes=8)
+        print(f"case schedule={schedule} Ns={B.Ns} entry_bytes={tab[1]} lut={tab[3] is not None}", flush=True)
+        assert tab[1] == 8 and tab[0].element_size() == 8
+        Q = quantum_LinearOperator(static, basis=B, dtype=dt, check_symm=False, check_herm=False, check_pcon=False)
+        H = _oracle_H(B, static)
+        v = _vec(B.Ns, dt, seed=13)
+        ref = H @ v.astype(np.complex128)
+        np.testing.assert_allclose(Q.dot(v), ref if np.dtype(dt).kind == "c" else ref.real, atol=1e-11, rtol=0)
+        assert all(c["lookup"] == "table" for c in Q._qsx_operator().config.values())
