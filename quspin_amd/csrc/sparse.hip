@@ -401,30 +401,36 @@ extern "C" int qsx_lanczos_update(int dtype, const void* v, const double* b_prev
 // fixed order on every block (the same value everywhere, deterministic) and
 // block 0 records the scalar.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nb) {
-  __shared__ double sh[256];
-  double a = 0.0;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x) a += part[k];
-  sh[threadIdx.x] = a;
+// Sum over the 256 lanes of a block, returned on every lane: a shuffle tree
+// inside each wavefront, then the four wavefront sums in a fixed order (the
+// same value on every block for the same inputs).
+__device__ __forceinline__ double block_sum256(double v) {
+  __shared__ double sh[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  const double r = sh[0];
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
   __syncthreads();
   return r;
 }
 
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nb) {
+  double a = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) a += part[k];
+  return block_sum256(a);
+}
+
 __device__ __forceinline__ void block_partial(double v, double* __restrict__ part) {
-  __shared__ double sh[256];
-  sh[threadIdx.x] = v;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+  const double r = block_sum256(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
+// Lanczos recurrence grid: about two elements per lane, at most 1,024 blocks
+// (the partial slots), so a 361 k vector spreads over all CUs.
+static int lz_grid(int64_t n) {
+  const int64_t g = (n + 511) / 512;
+  return (int)(g < 1 ? 1 : (g > 1024 ? 1024 : g));
 }
 
 // b = sqrt(sum of the norm partials) -> bout; q = r / b
@@ -1138,8 +1144,8 @@ extern "C" int qsx_lanczos_run(int op_kind, void* hv_ctx, const void* indptr, co
   // per step: q_i = r / b_{i-1} (the norm partials of the previous step
   // reduced on every block), r = A q_i, r -= b_{i-1} q_{i-1} with the
   // partials of <q_i, r>, r -= a_i q_i with the partials of ||r||^2
-  const int nb = red_grid(n);
-  double* pdot = R.partials + 4 * RED_BLOCKS;
+  const int nb = lz_grid(n);
+  double* pdot = R.partials + 4 * RED_BLOCKS;  // 1,024 slots each
   double* pnrm = R.partials + 6 * RED_BLOCKS;
   auto record = [&](hipStream_t q) -> int {
     for (int i = 0; i < m; ++i) {
