@@ -62,8 +62,14 @@ def static_rule(op, vdt, nvec):
     lookup = "table" if G > 1 else "search"
     if nvec != 1:
         schedule = "rows"
-    elif lookup == "table" or _rank_lookup(op):
+    elif lookup == "table":
         schedule = "batch8"
+    elif _rank_lookup(op):
+        # spin / hard-core-boson sectors: "rows" launches the fast one-term-at-
+        # a-time kernel when every term is bit-parallel, which outruns the
+        # branch-free batches on the direct rank; other sectors keep batches
+        from .basis.base_general import hcb_basis_general
+        schedule = "rows" if isinstance(op.basis, hcb_basis_general) else "batch8"
     else:
         schedule = "queue" if op.oplist.n_off >= 4 else "rows"
     return {"lookup": lookup, "schedule": schedule}

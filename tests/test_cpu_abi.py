@@ -155,10 +155,20 @@ def test_tuning_static_rule_and_candidates():
         def __init__(self, b):
             self.basis, self.oplist, self.transposed, self.conjugated = b, _OL(), False, False
 
+    from quspin_amd.basis.base_general import hcb_basis_general
+
+    class _HB(hcb_basis_general):
+        def __init__(self, Np):
+            self._pers = np.zeros(0, np.int32)
+            self._qs = np.zeros(0, np.int32)
+            self._Np, self._N, self._Ns = Np, 16, 1000
+
     sym = _Op(_B([16, 2], 8))
     free = _Op(_B([], 8))
     assert tuning.static_rule(sym, np.float64, 1) == {"lookup": "table", "schedule": "batch8"}
     assert tuning.static_rule(free, np.float64, 1) == {"lookup": "search", "schedule": "batch8"}
+    # a spin / hard-core-boson sector without symmetry: the one-term-at-a-time kernel
+    assert tuning.static_rule(_Op(_HB(8)), np.float64, 1) == {"lookup": "search", "schedule": "rows"}
     assert tuning.static_rule(sym, np.float64, 4)["schedule"] == "rows"
     names = {(c["lookup"], c["schedule"]) for c in tuning.candidates(sym, np.float64, 1)}
     assert ("table", "batch8") in names and ("search", "queue") in names
