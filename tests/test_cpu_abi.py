@@ -74,6 +74,24 @@ def test_status_codes_map_to_reference_exceptions(libpath):
             _lib.check(rc)
 
 
+def test_bad_handles_and_arguments_rejected_before_any_device_call(libpath):
+    """Entry points check their arguments on the host and return the
+    argument error (ValueError in Python) with a message before any HIP call;
+    here: null contexts."""
+    from quspin_amd import _lib
+    lib = _lib.load()
+    calls = [lambda: lib.qsx_hv_ctx_set_table(None, None, 0, 0, None, 0),
+             lambda: lib.qsx_hv_ctx_set_schedule(None, 0),
+             lambda: lib.qsx_hv_ctx_set_diag(None, None, 0),
+             lambda: lib.qsx_hv_ctx_apply(None, None, None, 1.0, 0.0, 1.0, 0.0, 0, None)]
+    for call in calls:
+        rc = call()
+        assert rc == _lib.QSX_ERR_ARG
+        with pytest.raises(ValueError):
+            _lib.check(rc)
+        assert _lib.last_error()
+
+
 def test_process_map_periods_and_errors():
     """Periods of translation, parity, inversion (closed forms); the product's
     map processing agrees with the oracle's independent one."""
