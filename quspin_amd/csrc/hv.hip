@@ -703,7 +703,8 @@ extern "C" int qsx_hv_ctx_create(qsx_core* c, qsx_oplist* l, const void* basis, 
 // doubles, or Ns (re, im) pairs when diag_arr_cplx).  The caller computes it
 // once (the product with a_off = 0 on a vector of ones) and keeps it alive.
 extern "C" int qsx_hv_ctx_set_diag(qsx_hv_ctx* X, const void* diag_arr, int diag_arr_cplx) {
-  if (!X || !diag_arr) { qsx_set_error("qsx_hv_ctx_set_diag: bad arguments"); return QSX_ERR_ARG; }
+  // an empty sector has no diagonal to hold (its array may be a null pointer)
+  if (!X || (!diag_arr && X->A.Ns > 0)) { qsx_set_error("qsx_hv_ctx_set_diag: bad arguments"); return QSX_ERR_ARG; }
   if ((X->dtype == QSX_F32 || X->dtype == QSX_F64) && diag_arr_cplx) {
     qsx_set_error("real dtype requested for an operator with a complex diagonal");
     return QSX_ERR_COMPLEX;

@@ -381,3 +381,45 @@ def test_fast_rows_u64_table_vs_oracle(schedule, monkeypatch):
         ref = H @ v.astype(np.complex128)
         np.testing.assert_allclose(Q.dot(v), ref if np.dtype(dt).kind == "c" else ref.real, atol=1e-11, rtol=0)
         assert all(c["lookup"] == "table" for c in Q._qsx_operator().config.values())
+
+
+def test_empty_and_single_state_sectors():
+    """Degenerate sectors: no state at all (L=4 half filling with k=1 and
+    z=+1, an eigenvalue pair no state carries; L=6 all down with k=1) and a
+    single state (all down, k=0; all up without symmetry): the basis sizes
+    match the oracle's, products return empty or the one diagonal element,
+    and an expm over a single state is the scalar exponential."""
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    kw = dict(check_symm=False, check_herm=False, check_pcon=False)
+    for L, Nup, blocks in [(4, 2, ("T", 1, "Z", 0)), (6, 0, ("T", 1)), (6, 0, ("T", 0)), (5, 5, ())]:
+        m = chain_maps(L)
+        bk = {}
+        names = {"T": "kblock", "Z": "zblock", "P": "pblock"}
+        for i in range(0, len(blocks), 2):
+            bk[names[blocks[i]]] = (m[blocks[i]], blocks[i + 1])
+        print(f"case L={L} Nup={Nup} blocks={blocks}", flush=True)
+        B = spin_basis_general(L, Nup=Nup, pauli=0, **bk)
+        maps = [m[blocks[i]] for i in range(0, len(blocks), 2)]
+        qs = [blocks[i + 1] for i in range(0, len(blocks), 2)]
+        g = O.Group(L, maps, [O.process_map(x)[2] for x in maps], qs)
+        res = O.make_basis(L, g, Np=Nup)
+        ref_basis = res[0] if isinstance(res, tuple) else res
+        assert B.Ns == len(ref_basis)
+        static = xxz_static(L, Delta=0.7)
+        dt = np.complex128
+        Q = quantum_LinearOperator(static, basis=B, dtype=dt, **kw)
+        H = hamiltonian(static, [], basis=B, dtype=dt, **kw)
+        v = _vec(B.Ns, dt, seed=3)
+        if B.Ns == 0:
+            assert Q.dot(v).shape == (0,) and H.dot(v).shape == (0,)
+            continue
+        Hm = _oracle_H(B, static)
+        ref = Hm @ v
+        np.testing.assert_allclose(Q.dot(v), ref, atol=1e-12, rtol=0)
+        np.testing.assert_allclose(H.dot(v), ref, atol=1e-12, rtol=0)
+        if B.Ns == 1:
+            e = complex(Hm.toarray()[0, 0]) if hasattr(Hm, "toarray") else complex(np.asarray(Hm)[0, 0])
+            out = expm_multiply_parallel(Q, a=-0.3j, dtype=dt).dot(v.copy())
+            np.testing.assert_allclose(out, np.exp(-0.3j * e) * v, atol=1e-12, rtol=0)
