@@ -271,9 +271,11 @@ __global__ void __launch_bounds__(256) k_kron_up(KronArgs K, int64_t nunits, int
       // only round trips of the loop are the partner reads (issued eight at a
       // time: two steps of the four rows, branch-free; a row past its degree
       // reads entry 0 with weight 0)
-      const int64_t kl = kb[0] + (lane < ne ? lane : 0);
-      const int ci = iu[kl];
-      const double2 ml = MTr<MT>::ld(du + kl);
+      // (lanes past the rows' entries read nothing: with no entries at all kb[0]
+      // may be the factor's entry count)
+      const int64_t kl = kb[0] + lane;
+      const int ci = lane < ne ? iu[kl] : 0;
+      const double2 ml = lane < ne ? MTr<MT>::ld(du + kl) : make_double2(0.0, 0.0);
       for (int64_t j = 0; j < maxdeg; j += 2) {
         int64_t c[2][RW];
         double2 m[2][RW];

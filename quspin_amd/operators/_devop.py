@@ -27,11 +27,17 @@ class DeviceCSR:
     is_matrix_free = False
 
     def __init__(self, indptr, indices, data, shape, dtype):
+        self.nnz = int(indices.shape[0])
+        if self.nnz == 0:
+            # the kernels take non-null entry pointers: a matrix without
+            # entries keeps one element that no row's indptr range reaches
+            t = L.torch()
+            indices = t.zeros(1, dtype=indices.dtype, device=indices.device)
+            data = t.zeros(1, dtype=data.dtype, device=data.device)
         self.indptr, self.indices, self.data = indptr, indices, data
         self.shape = tuple(shape)
         self.dtype = np.dtype(dtype)
         self.idx64 = 1 if indptr.dtype == L.torch().int64 else 0
-        self.nnz = int(indices.shape[0])
         self._host = None
 
     @property
@@ -57,8 +63,8 @@ class DeviceCSR:
 
     def to_scipy(self):
         if self._host is None:
-            self._host = sp.csr_matrix((L.to_host(self.data), L.to_host(self.indices), L.to_host(self.indptr)),
-                                       shape=self.shape)
+            self._host = sp.csr_matrix((L.to_host(self.data)[:self.nnz], L.to_host(self.indices)[:self.nnz],
+                                        L.to_host(self.indptr)), shape=self.shape)
         return self._host
 
     def apply(self, x, y, a=1.0, beta=0):

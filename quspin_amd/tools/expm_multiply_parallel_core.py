@@ -96,7 +96,8 @@ class expm_multiply_parallel:
             self._dtype = np.dtype(dtype)
         tol = np.finfo(self._dtype).eps / 2
         self._tol = np.array(tol, dtype=np.finfo(self._dtype).eps.dtype)
-        mu = op.trace() / op.shape[0]
+        # (an operator on an empty sector has no diagonal to average: mu = 0)
+        mu = op.trace() / op.shape[0] if op.shape[0] else 0.0
         self._mu = np.array(mu, dtype=self._dtype)
         self._A_1_norm = op.one_norm(-complex(self._mu))
         self._partition_override = None

@@ -187,3 +187,26 @@ def test_batched_expm_complex64_fp64_reductions():
     E64.set_partitions([(m + 4 * (k % 3), s) for k, (m, s) in enumerate(parts)])
     F64b = L.to_host(E64.dot(E64.batch.stack(xs)))
     np.testing.assert_allclose(F64b, F64, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("form", ["mf", "csr"])
+def test_batched_expm_with_empty_sectors(form):
+    """Sector lists that contain empty sectors (L=4 half filling with z=+1:
+    k=1 and k=3 hold no state): the batch stacks, evolves and unstacks them
+    as zero-row blocks, and the other sectors equal their dense exponential."""
+    from quspin_amd import _lib as L
+    from quspin_amd.tools.evolution import expm_multiply_sectors
+    m = chain_maps(4)
+    print(f"case L=4 z=+1 k=0..3 form={form}", flush=True)
+    bases, ops, dense = _sectors(4, range(4), extra={"zblock": (m["Z"], 0)})
+    assert [b.Ns for b in bases].count(0) == 2
+    nv = 3
+    E = expm_multiply_sectors(ops, a=-2.0j, dtype=np.complex128, nvec=nv, graph=False, form=form)
+    rng = np.random.default_rng(11)
+    xs = [_rand(rng, (b.Ns, nv), np.complex128) for b in bases]
+    got = E.batch.unstack(E.dot(E.batch.stack(xs)))
+    for k in range(4):
+        g = L.to_host(got[k])
+        assert g.shape == (bases[k].Ns, nv)
+        if bases[k].Ns:
+            np.testing.assert_allclose(g, expm(-2.0j * dense[k]) @ xs[k], rtol=0, atol=1e-11)
