@@ -168,3 +168,43 @@ def test_lanczos_device_C2_ground_energy():
     from tests.test_gpu_hv import _oracle_H
     E_exact = np.linalg.eigvalsh(_oracle_H(B, xxz_static(L)).toarray())[0]
     assert abs(E[0] - E_exact) < 1e-9
+
+
+def test_degenerate_operators_and_steps():
+    """An operator with no terms (products are zero, its exponential is the
+    identity), a zero time step (e^{0 H} v = v), and a Krylov space as large
+    as the sector allows (m = Ns - 1): every Ritz value equals the oracle
+    recurrence's on the same matrix and start vector."""
+    import torch
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    from quspin_amd.tools.lanczos import lanczos_full
+    from tests.test_gpu_hv import _oracle_H
+    kw = dict(check_symm=False, check_herm=False, check_pcon=False)
+    L = 8
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=4, pauli=0, kblock=(m["T"], 0), zblock=(m["Z"], 0))
+    rng = np.random.default_rng(5)
+    v = (rng.normal(size=B.Ns) + 1j * rng.normal(size=B.Ns)).astype(np.complex128)
+    print(f"case L={L} Ns={B.Ns} no terms", flush=True)
+    Q0 = quantum_LinearOperator([], basis=B, dtype=np.complex128, **kw)
+    np.testing.assert_array_equal(Q0.dot(v), np.zeros_like(v))
+    # (hamiltonian([], [], basis=...) has no shape to take, as in the
+    # reference: hamiltonian_core.py:359-391)
+    with pytest.raises(ValueError):
+        hamiltonian([], [], basis=B, dtype=np.complex128, **kw)
+    np.testing.assert_allclose(expm_multiply_parallel(Q0, a=-1.0j, dtype=np.complex128).dot(v.copy()), v,
+                               atol=1e-14, rtol=0)
+    print("case zero step", flush=True)
+    Q = quantum_LinearOperator(xxz_static(L), basis=B, dtype=np.complex128, **kw)
+    np.testing.assert_allclose(expm_multiply_parallel(Q, a=0.0, dtype=np.complex128).dot(v.copy()), v,
+                               atol=1e-14, rtol=0)
+    print("case Lanczos m = Ns - 1", flush=True)
+    Qr = quantum_LinearOperator(xxz_static(L), basis=B, dtype=np.float64, **kw)
+    Hd = _oracle_H(B, xxz_static(L)).toarray().real
+    v0 = rng.normal(size=B.Ns)
+    E, _, _ = lanczos_full(Qr, torch.from_numpy(v0).cuda(), B.Ns - 1)
+    E = E.cpu().numpy() if hasattr(E, "cpu") else np.asarray(E)
+    Eo = O.lanczos_full(lambda x: Hd @ x, v0 / np.linalg.norm(v0), B.Ns - 1)[0]
+    np.testing.assert_allclose(np.sort(E), np.sort(Eo), atol=1e-9, rtol=0)
