@@ -423,3 +423,31 @@ def test_empty_and_single_state_sectors():
             e = complex(Hm.toarray()[0, 0]) if hasattr(Hm, "toarray") else complex(np.asarray(Hm)[0, 0])
             out = expm_multiply_parallel(Q, a=-0.3j, dtype=dt).dot(v.copy())
             np.testing.assert_allclose(out, np.exp(-0.3j * e) * v, atol=1e-12, rtol=0)
+
+
+def test_degenerate_fermion_and_1d_sectors():
+    """Fermion sectors with a single state (the spinful vacuum, a filled
+    spinless chain) and an empty spin_basis_1d sector: basis sizes, products
+    against the oracle's matrix, and empty products."""
+    from quspin_amd.basis import spin_basis_1d, spinful_fermion_basis_general, spinless_fermion_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    kw = dict(check_symm=False, check_herm=False, check_pcon=False)
+    L = 4
+    T = (np.arange(L) + 1) % L
+    cases = [("spinful vacuum", spinful_fermion_basis_general(L, Nf=(0, 0), kblock=(T, 0)),
+              hubbard_static(L, U=2.0, mu=0.3), True, L),
+             ("spinless filled", spinless_fermion_basis_general(L, Nf=L),
+              [["+-", [[-1.0, i, (i + 1) % L] for i in range(L)]], ["-+", [[1.0, i, (i + 1) % L] for i in range(L)]],
+               ["n", [[0.5, i] for i in range(L)]]], True, None)]
+    for name, B, static, ferm, sl in cases:
+        print(f"case {name} Ns={B.Ns}", flush=True)
+        assert B.Ns == 1
+        Q = quantum_LinearOperator(static, basis=B, dtype=np.complex128, **kw)
+        H = _oracle_H(B, static, fermion=ferm, spinful_L=sl)
+        v = _vec(B.Ns, np.complex128, seed=2)
+        np.testing.assert_allclose(Q.dot(v), H @ v, atol=1e-12, rtol=0)
+    print("case spin_basis_1d empty k-sector", flush=True)
+    B1 = spin_basis_1d(6, Nup=0, kblock=1)
+    assert B1.Ns == 0
+    Q1 = quantum_LinearOperator(xxz_static(6), basis=B1, dtype=np.complex128, **kw)
+    assert Q1.dot(np.zeros(0, np.complex128)).shape == (0,)
