@@ -451,3 +451,42 @@ def test_degenerate_fermion_and_1d_sectors():
     assert B1.Ns == 0
     Q1 = quantum_LinearOperator(xxz_static(6), basis=B1, dtype=np.complex128, **kw)
     assert Q1.dot(np.zeros(0, np.complex128)).shape == (0,)
+
+
+@pytest.mark.parametrize("lookup", ["table", "search"])
+def test_64bit_states_vs_oracle(lookup, monkeypatch):
+    """Bases wider than 32 bits (64-bit state words): an L=36 chain with two
+    up spins, in the k=1 sector (sector table or orbit search) and without
+    symmetry (direct rank); the basis is bit-exact against the oracle's
+    representatives of the sector (built from its 630 states) and products
+    equal the oracle's matrix."""
+    import itertools
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    monkeypatch.setenv("QSX_LOOKUP", lookup)
+    L, Nup = 36, 2
+    m = chain_maps(L)
+    static = xxz_static(L, Delta=0.6)
+    cand = np.array(sorted((sum(1 << b for b in c) for c in itertools.combinations(range(L), Nup)), reverse=True),
+                    dtype=np.uint64)
+    for blocks in ({"kblock": (m["T"], 1)}, {}):
+        print(f"case L={L} Nup={Nup} lookup={lookup} blocks={list(blocks)}", flush=True)
+        B = spin_basis_general(L, Nup=Nup, pauli=0, **blocks)
+        if blocks:
+            items = O.order_maps(blocks)
+            g = O.Group(L, [it[1] for it in items], [it[2] for it in items], [it[3] for it in items])
+            rep, _, _ = g.representative(cand)
+            c = cand[rep == cand]
+            n = g.normalization(c)
+            b, n = c[n > 0], n[n > 0]
+        else:
+            g = O.Group(L, [], [], [])
+            b, n = cand, np.ones(len(cand), np.int64)
+        assert B.states.dtype == np.uint64
+        np.testing.assert_array_equal(B.states, b)
+        H = O.make_csr(b, n, g, flat_terms(static), np.complex128)
+        Q = quantum_LinearOperator(static, basis=B, dtype=np.complex128, check_symm=False, check_herm=False,
+                                   check_pcon=False)
+        for nvec in (None, 2):
+            v = _vec(B.Ns, np.complex128, seed=9, nvec=nvec)
+            np.testing.assert_allclose(Q.dot(v), H @ v, atol=1e-12, rtol=0)
