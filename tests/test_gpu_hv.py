@@ -490,3 +490,36 @@ def test_64bit_states_vs_oracle(lookup, monkeypatch):
         for nvec in (None, 2):
             v = _vec(B.Ns, np.complex128, seed=9, nvec=nvec)
             np.testing.assert_allclose(Q.dot(v), H @ v, atol=1e-12, rtol=0)
+
+
+def test_hamiltonian_dynamic_terms_and_out_semantics():
+    """hamiltonian with a time-dependent term (hamiltonian_core.py dot):
+    H(t) v = H_s v + f(t, *args) H_d v at several times; out= with
+    overwrite_out=False accumulates a H(t) v into out; a vector of times
+    applies H(t_i) to column i."""
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian
+    L = 10
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=5, pauli=0, kblock=(m["T"], 0))
+    static = xxz_static(L, Delta=0.8)
+    nnn = [[0.4, i, (i + 2) % L] for i in range(L)]
+
+    def f(t, w):
+        return np.cos(w * t)
+
+    H = hamiltonian(static, [["zz", nnn, f, [2.0]]], basis=B, dtype=np.complex128, check_symm=False,
+                    check_herm=False, check_pcon=False)
+    Hs = _oracle_H(B, static)
+    Hd = _oracle_H(B, [["zz", nnn]])
+    v = _vec(B.Ns, np.complex128, seed=4)
+    for t in (0.0, 0.7, 1.9):
+        print(f"case t={t}", flush=True)
+        np.testing.assert_allclose(H.dot(v, time=t), Hs @ v + f(t, 2.0) * (Hd @ v), atol=1e-12, rtol=0)
+    out = np.ones_like(v)
+    H.dot(v, time=0.7, out=out, overwrite_out=False, a=0.5)
+    np.testing.assert_allclose(out, 1.0 + 0.5 * (Hs @ v + f(0.7, 2.0) * (Hd @ v)), atol=1e-12, rtol=0)
+    V2 = np.stack([v, 2.0 * v], axis=1)
+    got = H.dot(V2, time=[0.3, 1.1])
+    for i, t in enumerate((0.3, 1.1)):
+        np.testing.assert_allclose(got[:, i], Hs @ V2[:, i] + f(t, 2.0) * (Hd @ V2[:, i]), atol=1e-12, rtol=0)
