@@ -208,3 +208,25 @@ def test_degenerate_operators_and_steps():
     E = E.cpu().numpy() if hasattr(E, "cpu") else np.asarray(E)
     Eo = O.lanczos_full(lambda x: Hd @ x, v0 / np.linalg.norm(v0), B.Ns - 1)[0]
     np.testing.assert_allclose(np.sort(E), np.sort(Eo), atol=1e-9, rtol=0)
+
+
+def test_scipy_eigsh_on_linear_operator():
+    """quantum_LinearOperator is a scipy LinearOperator (the reference's usage
+    in eigsh(H, k, which="SA")): scipy's Lanczos driven by the device product
+    gives the lowest eigenvalues of the oracle's matrix, for a real k=0/p/z
+    sector and a complex k-sector."""
+    from scipy.sparse.linalg import eigsh
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    from tests.test_gpu_hv import _oracle_H
+    kw = dict(check_symm=False, check_herm=False, check_pcon=False)
+    L = 14
+    m = chain_maps(L)
+    for blocks, dt in (({"kblock": (m["T"], 0), "pblock": (m["P"], 0), "zblock": (m["Z"], 0)}, np.float64),
+                       ({"kblock": (m["T"], 3)}, np.complex128)):
+        B = spin_basis_general(L, Nup=L // 2, pauli=0, **blocks)
+        print(f"case L={L} Ns={B.Ns} dtype={np.dtype(dt).name}", flush=True)
+        Q = quantum_LinearOperator(xxz_static(L, Delta=0.9), basis=B, dtype=dt, **kw)
+        ev = np.linalg.eigvalsh(_oracle_H(B, xxz_static(L, Delta=0.9)).toarray())
+        w = np.sort(eigsh(Q, k=3, which="SA", return_eigenvectors=False))
+        np.testing.assert_allclose(w, ev[:3], atol=1e-9, rtol=0)
