@@ -230,3 +230,36 @@ def test_scipy_eigsh_on_linear_operator():
         ev = np.linalg.eigvalsh(_oracle_H(B, xxz_static(L, Delta=0.9)).toarray())
         w = np.sort(eigsh(Q, k=3, which="SA", return_eigenvectors=False))
         np.testing.assert_allclose(w, ev[:3], atol=1e-9, rtol=0)
+
+
+def test_expm_set_a_work_array_and_overwrite():
+    """ExpmMultiplyParallel options (EMP.html:117-374): set_a re-times the
+    same operator (a new partition), several vectors at once with a caller's
+    work_array, overwrite_v writing into the caller's array; against dense
+    exponentials of the oracle's matrix."""
+    from scipy.linalg import expm
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    from tests.test_gpu_hv import _oracle_H
+    L = 12
+    m = chain_maps(L)
+    B = spin_basis_general(L, Nup=6, pauli=0, kblock=(m["T"], 2))
+    static = xxz_static(L, Delta=0.5)
+    Q = quantum_LinearOperator(static, basis=B, dtype=np.complex128, check_symm=False, check_herm=False,
+                               check_pcon=False)
+    Hd = _oracle_H(B, static).toarray()
+    rng = np.random.default_rng(8)
+    V = (rng.normal(size=(B.Ns, 3)) + 1j * rng.normal(size=(B.Ns, 3))).astype(np.complex128)
+    E = expm_multiply_parallel(Q, a=-0.5j, dtype=np.complex128)
+    print(f"case Ns={B.Ns} a=-0.5i", flush=True)
+    np.testing.assert_allclose(E.dot(V[:, 0]), expm(-0.5j * Hd) @ V[:, 0], atol=1e-11, rtol=0)
+    for a in (-1.3j, 0.2):
+        print(f"case set_a({a})", flush=True)
+        E.set_a(a)
+        work = np.zeros(2 * V.size, np.complex128)
+        np.testing.assert_allclose(E.dot(V, work_array=work), expm(a * Hd) @ V, atol=1e-10, rtol=0)
+    W = V.copy()
+    out = E.dot(W, overwrite_v=True)
+    assert out is W
+    np.testing.assert_allclose(W, expm(0.2 * Hd) @ V, atol=1e-10, rtol=0)
