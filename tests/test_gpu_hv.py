@@ -523,3 +523,24 @@ def test_hamiltonian_dynamic_terms_and_out_semantics():
     got = H.dot(V2, time=[0.3, 1.1])
     for i, t in enumerate((0.3, 1.1)):
         np.testing.assert_allclose(got[:, i], Hs @ V2[:, i] + f(t, 2.0) * (Hd @ V2[:, i]), atol=1e-12, rtol=0)
+
+
+def test_spinful_without_double_occupancy_vs_oracle():
+    """Spinful fermions with double_occupancy=False (basis_general drops the
+    doubly occupied states): hops that would create a doubly occupied site
+    leave the basis and give no element; products with and without
+    translation symmetry against the oracle's matrix on the same basis."""
+    from quspin_amd.basis import spinful_fermion_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    L = 5
+    T = (np.arange(L) + 1) % L
+    static = hubbard_static(L, U=0.0, mu=0.2)
+    for blocks in ({}, {"kblock": (T, 1)}):
+        B = spinful_fermion_basis_general(L, Nf=(2, 2), double_occupancy=False, **blocks)
+        print(f"case L={L} Nf=(2,2) no double occupancy blocks={list(blocks)} Ns={B.Ns}", flush=True)
+        assert not np.any((B.states >> L) & B.states & ((1 << L) - 1))
+        Q = quantum_LinearOperator(static, basis=B, dtype=np.complex128, check_symm=False, check_herm=False,
+                                   check_pcon=False)
+        H = _oracle_H(B, static, fermion=True, spinful_L=L)
+        v = _vec(B.Ns, np.complex128, seed=6)
+        np.testing.assert_allclose(Q.dot(v), H @ v, atol=1e-12, rtol=0)
