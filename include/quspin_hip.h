@@ -52,14 +52,16 @@ void qsx_core_destroy(qsx_core* core);
 int qsx_core_group_order(const qsx_core* core);
 int qsx_core_map_info(const qsx_core* core, int k, int* rev, int* ngroups, int* per);
 
-/* Replaces _core.make_basis(basis, n, Np=, count=) (base_general.py:864-867).
+/* Replaces _core.make_basis(basis, n, Np=, count=) (base_general.py:864-873).
  * Np: host int32 particle numbers (hcb: n_Np values; spinful: n_Np
  * (Nup, Ndown) pairs; n_Np = 0: no conservation).  basis_out == NULL counts
  * only.  Writes the representatives DESCENDING (uint32 for N <= 32, else
- * uint64) and n (n_bytes = 1, 2 or 4 per entry).  Returns Ns, or -(Ns)-1 when
- * capacity is too small (the reference's "Ns < 0" overflow signal). */
-int64_t qsx_make_basis(qsx_core* core, const int32_t* Np, int n_Np, void* basis_out, void* n_out, int n_bytes,
-                       int64_t capacity, void* stream);
+ * uint64) and n (n_bytes = 1, 2 or 4 per entry).  *Ns_out receives Ns.  When
+ * capacity < Ns nothing is written and QSX_ERR_OVERFLOW is returned with
+ * *Ns_out = the size needed (the reference's core signals this case by a
+ * negative return; a binding maps one onto the other). */
+int qsx_make_basis(qsx_core* core, const int32_t* Np, int n_Np, void* basis_out, void* n_out, int n_bytes,
+                   int64_t capacity, int64_t* Ns_out, void* stream);
 
 /* Replaces _core.make_basis_blocks(N_p, basis) (base_general.py:945-947):
  * device int64 begin/end[2^N_p] bounding the states with each N_p-bit prefix. */
@@ -108,11 +110,13 @@ void qsx_oplist_destroy(qsx_oplist* l);
 int qsx_oplist_info(const qsx_oplist* l, int* n_off, int* n_diag, int* complex_me);
 
 /* Replaces _core.op(row, col, ME, opstr, indx, J, basis, n, begin, end, N_p)
- * (base_general.py:324-336): one element per basis state (column); ME is
- * complex128, zero where the element vanishes; row/col int32 or int64. */
-int qsx_op(qsx_core* core, const char* opstr, const int32_t* indx, int n_op, double J_re, double J_im,
-           const void* basis, const void* n, int n_bytes, int64_t Ns, const int64_t* begin, const int64_t* end,
-           int N_p, void* row, void* col, int idx_bytes, void* me_out, void* stream);
+ * (base_general.py:324-336), argument order kept (each array followed by its
+ * element size or length): one element per basis state (column); ME is
+ * complex128, zero where the element vanishes; row/col int32 or int64
+ * (idx_bytes 4 or 8). */
+int qsx_op(qsx_core* core, void* row, void* col, int idx_bytes, void* ME, const char* opstr, const int32_t* indx,
+           int n_op, double J_re, double J_im, const void* basis, const void* n, int n_bytes, int64_t Ns,
+           const int64_t* begin, const int64_t* end, int N_p, void* stream);
 
 /* Replaces _core.inplace_op(v_in, v_out, conjugated, transposed, opstr, indx,
  * J, basis, n, begin, end, N_p) (base_general.py:395-408), argument order
@@ -137,6 +141,9 @@ int qsx_hv(qsx_core* core, qsx_oplist* l, const void* basis, const void* n, int 
 int qsx_hv_ctx_create(qsx_core* core, qsx_oplist* l, const void* basis, const void* n, int n_bytes, int64_t Ns,
                       const int64_t* begin, const int64_t* end, int N_p, int dtype, int64_t nvec, int transposed,
                       int conjugated, int diag_conj, const void* diag_arr, int diag_arr_cplx, qsx_hv_ctx** out);
+/* Destroys the context and the launch sequences captured with it
+ * (qsx_lanczos_run).  The set_table / set_schedule / set_diag calls below
+ * likewise drop the captures made before them. */
 void qsx_hv_ctx_destroy(qsx_hv_ctx* ctx);
 /* lut: the coefficient table of qsx_table_build and its length (NULL and 0
  * when qsx_table_info gave lut_len 0) */
@@ -151,6 +158,17 @@ int qsx_hv_ctx_set_diag(qsx_hv_ctx* ctx, const void* diag_arr, int diag_arr_cplx
 int qsx_hv_ctx_apply(qsx_hv_ctx* ctx, const void* x, void* y, double a_off_re, double a_off_im, double a_diag_re,
                      double a_diag_im, int beta, void* stream);
 
+/* Column 1-norms colabs[j] = sum_i |A_ij + shift delta_ij| (Ns doubles) and
+ * the diagonal dg[j] = A_jj (Ns (re, im) pairs) of the context's operator,
+ * without assembling it: duplicates of one (i, j) are summed before |.|, as
+ * in the canonical CSR whose _wrapper_csr_trace / _wrapper_csr_1_norm
+ * (EMP.html:196-202) give ExpmMultiplyParallel's mu and ||A - mu I||_1.
+ * QSX_ERR_UNSUPPORTED (nothing usable written) when a column holds more
+ * than 64 distinct rows, or for a transposed context whose list does not
+ * commute with the group: assemble the CSR then. */
+int qsx_hv_ctx_column_norms(qsx_hv_ctx* ctx, double shift_re, double shift_im, double* colabs, double* dg,
+                            void* stream);
+
 /* Replaces the COO -> CSR assembly of basis._make_matrix (base.py:265-304)
  * for hamiltonian: canonical CSR (sorted columns, duplicates summed, zeros
  * dropped) of a*Op.  Two steps: create (returns nnz), finish (writes
@@ -163,6 +181,49 @@ int qsx_csr_builder_finish(qsx_csr_builder* b, int dtype, int idx_bytes, void* i
                            void* stream);
 void qsx_csr_builder_destroy(qsx_csr_builder* b);
 
+/* ---- sectors and projections --------------------------------------------- */
+/* The projector of a reduced basis, P[:, r] = c_r sum_g chi(g) sign(g, r)
+ * |g.r>, c_r = 1/sqrt(n_r |G|) (get_proj, base_general.py:521-582, the
+ * coefficient at :571-575).  "Full space": the 2^N states in descending order
+ * (pcon = 0, Ns_full = 2^N) or the states of the particle sector recorded by
+ * the last qsx_make_basis, descending (pcon = 1, Ns_full = its size). */
+
+/* Replaces _core.op_shift_sector(v_in, v_out, opstr, indx, J, basis, n,
+ * other_basis, other_n) (base_general.py:504-514), argument order kept (each
+ * array followed by its element size or length): one term, v_out (Ns x nvec)
+ * += P^H Op P_other v_in (other_Ns x nvec), where core describes the target
+ * basis (basis, n, its prefix blocks) and other_core the source basis; both
+ * must carry the same symmetry maps (characters may differ).  dtype
+ * QSX_F64 or QSX_C128. */
+int qsx_op_shift_sector(qsx_core* core, qsx_core* other_core, const void* v_in, void* v_out, int dtype,
+                        int64_t nvec, const char* opstr, const int32_t* indx, int n_op, double J_re, double J_im,
+                        const void* basis, const void* n, int n_bytes, int64_t Ns, const int64_t* begin,
+                        const int64_t* end, int N_p, const void* other_basis, const void* other_n, int other_n_bytes,
+                        int64_t other_Ns, void* stream);
+
+/* Replaces _core.get_amp(states, out, n, mode) (base_general.py:1254): out[i]
+ * = P[s_i, rep(s_i)] = sign conj(chi(g)) sqrt(n_rep / |G|) (g.s_i = rep,
+ * 0 outside the sector), in out_dtype; mode 0 "representative", 1
+ * "full_basis" (states, device, are replaced by their representatives). */
+int qsx_get_amp(qsx_core* core, void* states, void* out, int out_dtype, int64_t n, int mode, void* stream);
+
+/* Replace _core.project_from_dense(basis, n, v0, v_out, basis_pcon=) and
+ * _core.project_to_dense(...) (base_general.py:787-789, 677-679): v_out
+ * (Ns_full x nvec, zeros on entry) = P v0 (Ns x nvec), and v_out (Ns x nvec)
+ * = P^H v0 (Ns_full x nvec). */
+int qsx_project_from_dense(qsx_core* core, const void* basis, const void* n, int n_bytes, int64_t Ns,
+                           const void* v0, void* v_out, int dtype, int64_t nvec, int pcon, int64_t Ns_full,
+                           void* stream);
+int qsx_project_to_dense(qsx_core* core, const void* basis, const void* n, int n_bytes, int64_t Ns, const void* v0,
+                         void* v_out, int dtype, int64_t nvec, int pcon, int64_t Ns_full, void* stream);
+
+/* Replaces _core.get_proj(basis, dtype, sign, c, indices, indptr,
+ * basis_pcon=) (base_general.py:580-582): P as CSC arrays, rows ascending
+ * within a column.  Call with indices == NULL for *nnz_out, then with
+ * indptr (Ns + 1), indices and data (nnz) allocated (idx_bytes 4 or 8). */
+int qsx_get_proj(qsx_core* core, const void* basis, const void* n, int n_bytes, int64_t Ns, int pcon, int64_t Ns_full,
+                 int dtype, int idx_bytes, void* indptr, void* indices, void* data, int64_t* nnz_out, void* stream);
+
 /* ---- sparse tools -------------------------------------------------------- */
 
 /* Replaces _matvec(A, x, out=, overwrite_out=, a=) for csr A
@@ -172,6 +233,10 @@ void qsx_csr_builder_destroy(qsx_csr_builder* b);
 int qsx_csr_matvec(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype, int64_t Ns,
                    int64_t nnz, const void* x, void* y, int vec_dtype, int64_t nvec, double a_re, double a_im,
                    int accumulate, int group, void* stream);
+
+/* Deterministic device reductions: op 0, out[0] + i out[1] = sum of n
+ * (re, im) pairs x; op 1, out[0] = max of n doubles x. */
+int qsx_reduce(int op, const double* x, int64_t n, double* out, void* stream);
 
 /* Replace _wrapper_csr_trace(indptr, indices, data) and
  * _wrapper_csr_1_norm(indptr, indices, data, shift) (EMP.html:196-202);
@@ -211,8 +276,10 @@ int qsx_lanczos_div(int dtype, const void* r, const double* b, void* q, int64_t 
  * (_lanczos_utils.py:158-227) on the device: op_kind 0 = H.v context
  * (hv_ctx), 1 = CSR (indptr/indices/data).  Q holds q0 in row 0 and receives
  * all m vectors when store_Q, else it is a ring of 3 vectors; ab = m x
- * (a re/im, b re/im) doubles.  use_graph captures the sequence once per set
- * of buffers and replays it. */
+ * (a re/im, b re/im) doubles.  use_graph captures the sequence once per
+ * (operator, buffers, sizes) and replays it; a capture on an H.v context
+ * lives until that context changes or is destroyed, and holds reduction
+ * scratch of its own. */
 int qsx_lanczos_run(int op_kind, void* hv_ctx, const void* indptr, const void* indices, const void* data, int idx64,
                     int mat_dtype, int64_t nnz, int group, int dtype, int64_t n, int m, void* Q, int store_Q, void* r,
                     double* ab, int use_graph, void* stream);
@@ -263,10 +330,6 @@ int qsx_expm_multiply_batch(qsx_hv_batch* b, const int32_t* s, const int32_t* m,
                             const double* mu, double a_re, double a_im, void* v, void* work, int use_graph,
                             void* stream);
 
-/* Extra diagonal over the stacked rows of a batch: products use
- * H + (re + i im) diag(xdiag).  For time-dependent diagonal terms. */
-int qsx_hv_batch_set_diag(qsx_hv_batch* b, const double* xdiag, double re, double im);
-
 /* ---- time stepping with learned partitions ------------------------------------ */
 
 /* Device record of one Taylor step of one sector: a, mu, factor of the extra
@@ -275,24 +338,17 @@ int qsx_hv_batch_set_diag(qsx_hv_batch* b, const double* xdiag, double re, doubl
  * int s, int m, int pad[2], then padding to a multiple of 16 bytes (80). */
 int qsx_expm_record_bytes(void);
 
-/* nsteps consecutive steps v <- e^{a_t (H + dscale_t D)} v, every parameter
- * read from the device records es[t*nsec + k]; launch bounds s_max, m_max;
- * fail[t*nsec + k] set when a stage of step t ran all its m terms without
- * passing the truncation test (scipy _expm_multiply.py:298-300).  No
- * counterpart in the reference (whose callers loop on the host over
- * expm_multiply_parallel(H(t), a).dot); use_graph captures all steps once. */
-int qsx_evolve_steps(qsx_hv_batch* b, int nsteps, void* es, int s_max, int m_max, void* v, void* work, int* fail,
-                     int use_graph, void* stream);
-
-/* Single-matrix form: nsteps steps v <- e^{a_t (A + dscale_t diag(xdiag))} v
+/* nsteps consecutive steps v <- e^{a_t (A + dscale_t diag(xdiag))} v
  * with A a device CSR (as qsx_expm_multiply_csr) and the records es[t]
  * (ExpmSec layout above); the same tail check into fail[t].  ctl_work:
  * qsx_evolve_csr_ctl_bytes(s_max, m_max) bytes of device scratch owned by the
  * caller.  vec_dtype is the dtype of the Taylor terms (work) and the SpMV;
  * state_dtype that of v, the accumulated sum: the same, or its FP64
  * counterpart (FP32 terms with FP64 accumulation), or -1 for the same.
- * use_graph captures the whole sequence once per set of buffers
- * (qsx_graph_cache_clear drops the captures). */
+ * use_graph captures the whole sequence once per set of operator arrays,
+ * sizes and buffers and replays it on later calls with the same set; at
+ * most 64 captures are kept (oldest dropped first).  qsx_graph_cache_clear
+ * drops every capture. */
 int64_t qsx_evolve_csr_ctl_bytes(int s_max, int m_max);
 int qsx_evolve_csr_steps(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
                          int64_t Ns, int64_t nnz, const double* xdiag, int nsteps, const void* es, int s_max,
@@ -316,7 +372,7 @@ typedef struct qsx_kron qsx_kron;
 int qsx_kron_create(const void* ip_u, const void* ind_u, const void* dat_u, int idx64_u, int64_t Nu, int64_t nnz_u,
                     const void* ip_d, const void* ind_d, const void* dat_d, int idx64_d, int64_t Nd, int64_t nnz_d,
                     const double* sdiag, int mat_dtype, qsx_kron** out);
-/* also drops every captured step sequence (qsx_graph_cache_clear) */
+/* also drops the step sequences captured with this operator */
 void qsx_kron_destroy(qsx_kron* K);
 /* Kernel form: 0 by size (default: one pass while psi fits in L2, else a
  * row pass over H_down and an L2-resident column-strip pass over H_up), 1 one

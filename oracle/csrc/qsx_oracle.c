@@ -11,7 +11,8 @@
  * and the reduced element is J m chi(g) sqrt(n_r'/n_r) (hcb spin-1/2,
  * base_general.py:354-410 computed per term, as the reference's one pass per
  * term; a fused all-terms variant is timed as well).
- * Pins: tests/test_oracle_c.py checks this file against the numpy oracle
+ * Pins: tests/test_oracle_pins.py (test_c_oracle_matches_numpy_oracle) checks
+ * this file against the numpy oracle
  * (brute force, projector identity) on small chains.
  */
 #include <math.h>

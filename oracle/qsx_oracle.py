@@ -22,6 +22,9 @@ Pins (checked in ``tests/test_oracle_*.py``):
 * ``lanczos_full``                — dense ``eigh`` ground energy to 1e-10
   (test_Lanczos.py:56-70); Krylov orthonormality.
 * ``mlp_*``                       — finite differences of the loss.
+* ``op_shift_sector``/``get_amp`` — built on ``projector`` (pinned above);
+  P^H P = 1 and the shifted product of a symmetric operator equals the
+  reduced operator (tests/test_oracle_pins.py).
 """
 
 import itertools
@@ -36,7 +39,7 @@ __all__ = [
     "apply_term", "reduced_op", "projector", "full_basis_states",
     "op_coo", "make_csr", "inplace_op", "index_of",
     "expm_multiply", "fragment_3_1_partition", "lanczos_full",
-    "csr_trace", "csr_1_norm", "mlp_forward", "mlp_backward",
+    "csr_trace", "csr_1_norm", "mlp_forward", "mlp_backward", "op_shift_sector", "get_amp",
 ]
 
 
@@ -400,6 +403,32 @@ def projector(basis, n, group, full_states):
         assert np.all(rows >= 0)
         np.add.at(P, (rows, np.arange(b.size)), c * chars[e] * sg[e])
     return P
+
+
+def op_shift_sector(basis2, n2, group2, basis1, n1, group1, full_states, N, op_list, v_in, fermion=False):
+    """P_2^H Op P_1 v_in: the "formally equivalent" computation the
+    reference's ``Op_shift_sector`` docstring states
+    (base_general.py:419-427; checked that way by
+    test/test_Op_shift_sector.py:31-46), with dense projectors and the brute
+    force operator on ``full_states``."""
+    P1 = projector(basis1, n1, group1, full_states)
+    P2 = projector(basis2, n2, group2, full_states)
+    H = full_hamiltonian(full_states, N, op_list, fermion)
+    return P2.conj().T @ (H @ (P1 @ np.asarray(v_in)))
+
+
+def get_amp(basis, n, group, full_states, states):
+    """psi_s / psi_r^sym = P[s, rep(s)] read off the dense projector
+    (``get_amp`` base_general.py:1183-1269, which test/test_get_amp.py:127-150
+    checks against ``get_vec``); 0 for states whose representative is not in
+    the basis.  Returns (C, representatives)."""
+    P = projector(basis, n, group, full_states)
+    st = np.asarray(states, dtype=np.uint64)
+    rep = group.representative(st)[0] if len(group.pers) else st
+    rows = index_of(full_states, st)
+    cols = index_of(basis, rep)
+    C = np.where(cols >= 0, P[rows, np.maximum(cols, 0)], 0.0)
+    return C, rep
 
 
 def full_hamiltonian(full_states, N, static, fermion=False):

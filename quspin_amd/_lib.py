@@ -38,7 +38,7 @@ SIGNATURES = {
     "qsx_core_destroy": (None, [_vp]),
     "qsx_core_group_order": (_i32, [_vp]),
     "qsx_core_map_info": (_i32, [_vp, _i32, C.POINTER(_i32), C.POINTER(_i32), C.POINTER(_i32)]),
-    "qsx_make_basis": (_i64, [_vp, _vp, _i32, _vp, _vp, _i32, _i64, _vp]),
+    "qsx_make_basis": (_i32, [_vp, _vp, _i32, _vp, _vp, _i32, _i64, C.POINTER(_i64), _vp]),
     "qsx_make_basis_blocks": (_i32, [_vp, _i32, _vp, _i64, _vp, _vp, _vp]),
     "qsx_representative": (_i32, [_vp, _vp, _vp, _vp, _vp, _i64, _vp]),
     "qsx_normalization": (_i32, [_vp, _vp, _vp, _i32, _i64, _vp]),
@@ -49,8 +49,8 @@ SIGNATURES = {
                       _dbl, _dbl, _dbl, _dbl, _i32, _vp, _i32, _vp]),
     "qsx_inplace_op": (_i32, [_vp, _vp, _vp, _i32, _i64, _i32, _i32, C.c_char_p, _vp, _i32, _dbl, _dbl, _vp, _vp,
                               _i32, _i64, _vp, _vp, _i32, _vp]),
-    "qsx_op": (_i32, [_vp, C.c_char_p, _vp, _i32, _dbl, _dbl, _vp, _vp, _i32, _i64, _vp, _vp, _i32, _vp, _vp, _i32,
-                      _vp, _vp]),
+    "qsx_op": (_i32, [_vp, _vp, _vp, _i32, _vp, C.c_char_p, _vp, _i32, _dbl, _dbl, _vp, _vp, _i32, _i64, _vp, _vp,
+                      _i32, _vp]),
     "qsx_hv_ctx_create": (_i32, [_vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _i32, _i32, _i64, _i32, _i32, _i32, _vp,
                                  _i32, C.POINTER(_vp)]),
     "qsx_hv_ctx_destroy": (None, [_vp]),
@@ -61,6 +61,15 @@ SIGNATURES = {
                               C.POINTER(_i64)]),
     "qsx_table_build": (_i32, [_vp, _vp, _i64, _vp, _vp, _i32, _vp, _i32, _vp, _i32, _i32, _vp, _i64, _vp]),
     "qsx_hv_ctx_apply": (_i32, [_vp, _vp, _vp, _dbl, _dbl, _dbl, _dbl, _i32, _vp]),
+    "qsx_hv_ctx_column_norms": (_i32, [_vp, _dbl, _dbl, _vp, _vp, _vp]),
+    "qsx_reduce": (_i32, [_i32, _vp, _i64, _vp, _vp]),
+    "qsx_op_shift_sector": (_i32, [_vp, _vp, _vp, _vp, _i32, _i64, C.c_char_p, _vp, _i32, _dbl, _dbl, _vp, _vp, _i32,
+                                   _i64, _vp, _vp, _i32, _vp, _vp, _i32, _i64, _vp]),
+    "qsx_get_amp": (_i32, [_vp, _vp, _vp, _i32, _i64, _i32, _vp]),
+    "qsx_project_from_dense": (_i32, [_vp, _vp, _vp, _i32, _i64, _vp, _vp, _i32, _i64, _i32, _i64, _vp]),
+    "qsx_project_to_dense": (_i32, [_vp, _vp, _vp, _i32, _i64, _vp, _vp, _i32, _i64, _i32, _i64, _vp]),
+    "qsx_get_proj": (_i32, [_vp, _vp, _vp, _i32, _i64, _i32, _i64, _i32, _i32, _vp, _vp, _vp, C.POINTER(_i64),
+                            _vp]),
     "qsx_csr_builder_create": (_i32, [_vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _i32, _i32, _i32, _dbl, _dbl,
                                       C.POINTER(_vp), C.POINTER(_i64), _vp]),
     "qsx_csr_builder_complex": (_i32, [_vp]),
@@ -92,9 +101,7 @@ SIGNATURES = {
     "qsx_hv_batch_set_schedule": (_i32, [_vp, _i32, _i32]),
     "qsx_hv_batch_apply": (_i32, [_vp, _vp, _vp, _dbl, _dbl, _i32, _vp]),
     "qsx_expm_multiply_batch": (_i32, [_vp, _vp, _vp, _vp, _vp, _dbl, _dbl, _vp, _vp, _i32, _vp]),
-    "qsx_hv_batch_set_diag": (_i32, [_vp, _vp, _dbl, _dbl]),
     "qsx_expm_record_bytes": (_i32, []),
-    "qsx_evolve_steps": (_i32, [_vp, _i32, _vp, _i32, _i32, _vp, _vp, _vp, _i32, _vp]),
     "qsx_evolve_csr_ctl_bytes": (_i64, [_i32, _i32]),
     "qsx_evolve_csr_steps": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _vp, _i32, _vp, _i32, _i32, _vp, _vp, _i32,
                                     _i32, _i64, _vp, _vp, _i64, _i32, _i32, _vp]),

@@ -269,8 +269,10 @@ class basis_general:
             Ns_cap = max([self._Ns, 1000, self._Ns_block_est])
         Np, nNp = self._Np_spec()
         st = stream_handle()
-        Ns = lib.qsx_make_basis(self._core.h, Np.ctypes.data_as(L._vp), nNp, None, None, 1, 0, st)
-        L.check(Ns, "qsx_make_basis")
+        Ns_c = L.C.c_int64(0)
+        L.check(lib.qsx_make_basis(self._core.h, Np.ctypes.data_as(L._vp), nNp, None, None, 1, 0, L.C.byref(Ns_c),
+                                   st), "qsx_make_basis")
+        Ns = int(Ns_c.value)
         if Ns > Ns_cap:
             raise ValueError(
                 "estimate for size of reduced Hilbert-space is too low, please double check that transformation "
@@ -281,7 +283,8 @@ class basis_general:
         basis_d = L.empty(max(Ns, 1), np.int32 if word.itemsize == 4 else np.int64)
         n_d = L.empty(max(Ns, 1), {1: np.uint8, 2: np.int16, 4: np.int32}[nb])
         if Ns > 0:
-            rc = lib.qsx_make_basis(self._core.h, Np.ctypes.data_as(L._vp), nNp, ptr(basis_d), ptr(n_d), nb, Ns, st)
+            rc = lib.qsx_make_basis(self._core.h, Np.ctypes.data_as(L._vp), nNp, ptr(basis_d), ptr(n_d), nb, Ns,
+                                    L.C.byref(Ns_c), st)
             L.check(rc, "qsx_make_basis")
         self._dev = {"basis": basis_d[:Ns], "n": n_d[:Ns], "n_bytes": nb}
         self._host_basis = None
@@ -487,8 +490,9 @@ class basis_general:
         me = L.empty(self._Ns, np.complex128)
         indx = np.ascontiguousarray(indx, dtype=np.int32)
         Jc = complex(J)
-        rc = L.load().qsx_op(self._core.h, opstr.encode(), indx.ctypes.data_as(L._vp), len(opstr), Jc.real, Jc.imag,
-                             *self._dev_args(), ptr(row), ptr(col), idx_bytes, ptr(me), stream_handle())
+        rc = L.load().qsx_op(self._core.h, ptr(row), ptr(col), idx_bytes, ptr(me), opstr.encode(),
+                             indx.ctypes.data_as(L._vp), len(opstr), Jc.real, Jc.imag, *self._dev_args(),
+                             stream_handle())
         L.check(rc, "qsx_op")
         ME = L.to_host(me)
         if dtype.kind != "c":
@@ -561,6 +565,214 @@ class basis_general:
             v_out[...] = res
             return v_out.squeeze()
         return res.squeeze()
+
+    # ------------------------------------------ sectors and projections
+    def _require_made(self):
+        if not self._made_basis:
+            raise AttributeError("this function requires the basis to be constructed first; use basis.make().")
+
+    def Op_shift_sector(self, other_basis, op_list, v_in, v_out=None, dtype=None):
+        """Apply an operator that moves states of ``other_basis``' sector
+        into this basis' sector: v_out = P_self^H Op P_other v_in
+        (base_general.py:412-519), one device pass per term as the reference
+        calls its core per term.  Both bases must carry the same symmetry
+        maps.  ``v_in``: (other_basis.Ns,) or (other_basis.Ns, nvec), host or
+        device; the result lives where ``v_in`` lives."""
+        if not isinstance(other_basis, self.__class__):
+            raise ValueError("other_basis must be the same type as the given basis.")
+        self._require_made()
+        other_basis._require_made()
+        _, _, J_list = zip(*op_list)
+        J_list = np.asarray(J_list)
+        if dtype is not None:
+            J_list = J_list.astype(dtype)
+        on_dev = L.is_device_tensor(v_in)
+        vdt = L.np_dtype(v_in.dtype) if on_dev else np.asanyarray(v_in).dtype
+        result_dtype = np.result_type(np.float32, J_list.dtype, vdt)
+        shape = tuple(v_in.shape)
+        nvecs = int(np.prod(shape[1:])) if len(shape) > 1 else 1
+        if shape[0] != other_basis.Ns:
+            raise ValueError("invalid shape for v_in")
+        if v_out is not None:
+            odt = L.np_dtype(v_out.dtype) if L.is_device_tensor(v_out) else v_out.dtype
+            if odt != result_dtype:
+                raise TypeError("v_out does not have the correct data type.")
+            if not L.is_device_tensor(v_out) and not v_out.flags["CARRAY"]:
+                raise ValueError("v_out is not a writable C-contiguous array")
+            if tuple(v_out.shape) not in ((self.Ns, nvecs), (self.Ns,) if nvecs == 1 else (self.Ns, nvecs)):
+                raise ValueError("invalid shape for v_out")
+        # the kernels accumulate in float64 / complex128
+        kdt = np.dtype(np.complex128) if result_dtype.kind == "c" else np.dtype(np.float64)
+        vin_d = L.to_device(v_in, kdt).reshape(other_basis.Ns, nvecs).contiguous()
+        if v_out is None:
+            vout_d = L.zeros((self.Ns, nvecs), kdt)
+        else:
+            vout_d = L.to_device(v_out, kdt).reshape(self.Ns, nvecs).contiguous()
+        lib = L.load()
+        st = stream_handle()
+        for (opstr, indx, _), J in zip(op_list, J_list):
+            opstr, indx, J = self._opstr_prepare(opstr, indx, J)
+            self._validate_op(opstr, indx)
+            idx = np.ascontiguousarray(indx, dtype=np.int32)
+            Jc = complex(J)
+            od = other_basis._dev
+            rc = lib.qsx_op_shift_sector(self._core.h, other_basis._core.h, ptr(vin_d), ptr(vout_d), L.dtype_code(kdt),
+                                         nvecs, opstr.encode(), idx.ctypes.data_as(L._vp), len(opstr), Jc.real, Jc.imag,
+                                         *self._dev_args(), ptr(od["basis"]), ptr(od["n"]), od["n_bytes"],
+                                         other_basis.Ns, st)
+            L.check(rc, "qsx_op_shift_sector")
+        res = vout_d.to(L.torch_dtype(result_dtype))
+        if v_out is not None:
+            if L.is_device_tensor(v_out):
+                v_out.copy_(res.reshape(v_out.shape))
+                return v_out
+            v_out[...] = L.to_host(res).reshape(v_out.shape)
+            return v_out.squeeze() if nvecs == 1 else v_out
+        if on_dev:
+            return res.squeeze(1) if nvecs == 1 else res
+        out = L.to_host(res)
+        return out.squeeze() if nvecs == 1 else out
+
+    def _full_space(self, pcon):
+        """(pcon flag, size of the space the projector maps into): the 2^N
+        states, or the basis' particle sector (get_proj, base_general.py:553-566)."""
+        if pcon:
+            if not self._get_proj_pcon:
+                raise TypeError("pcon=True only works for basis of a single particle number sector.")
+            info = np.zeros(8, np.int64)
+            L.check(L.load().qsx_sector_rank(self._core.h, None, 0, None, info.ctypes.data_as(L._vp), None),
+                    "qsx_sector_rank")
+            if not info[5] or info[0] > 1:
+                raise TypeError("pcon=True only works for basis of a single particle number sector.")
+            return 1, int(info[4])
+        return 0, int(self._sps) ** int(self._N)
+
+    def get_proj(self, dtype, pcon=False):
+        """Projector P (full x reduced, scipy CSC) from the reduced basis to
+        the full space or its particle sector (get_proj,
+        base_general.py:521-582): column r holds c_r chi(g) sign(g, r) on the
+        rows of the orbit images g.r, c_r = 1/sqrt(n_r |G|), built on the
+        device."""
+        import scipy.sparse as sp
+        self._require_made()
+        pc, Ns_full = self._full_space(pcon)
+        dtype = np.dtype(dtype)
+        index_type = np.result_type(np.min_scalar_type(-Ns_full), np.int32)
+        idx_bytes = np.dtype(index_type).itemsize
+        if self._Ns == 0:
+            return sp.csc_matrix((Ns_full, 0), dtype=dtype)
+        lib = L.load()
+        st = stream_handle()
+        nnz = L.C.c_int64(0)
+        d = self._dev
+        args = (self._core.h, ptr(d["basis"]), ptr(d["n"]), d["n_bytes"], self._Ns, pc, Ns_full,
+                L.dtype_code(dtype), idx_bytes)
+        L.check(lib.qsx_get_proj(*args, None, None, None, L.C.byref(nnz), st), "qsx_get_proj")
+        indptr = L.empty(self._Ns + 1, index_type)
+        indices = L.empty(max(nnz.value, 1), index_type)
+        data = L.empty(max(nnz.value, 1), dtype)
+        L.check(lib.qsx_get_proj(*args, ptr(indptr), ptr(indices), ptr(data), L.C.byref(nnz), st), "qsx_get_proj")
+        n = nnz.value
+        return sp.csc_matrix((L.to_host(data)[:n], L.to_host(indices)[:n], L.to_host(indptr)),
+                             shape=(Ns_full, self._Ns))
+
+    def _project(self, v0, sparse, pcon, to_reduced):
+        import scipy.sparse as sp
+        self._require_made()
+        pc, Ns_full = self._full_space(pcon)
+        on_dev = L.is_device_tensor(v0)
+        if not on_dev and not hasattr(v0, "shape"):
+            v0 = np.asanyarray(v0)
+        n_in, n_out = (Ns_full, self._Ns) if to_reduced else (self._Ns, Ns_full)
+        squeeze = v0.ndim == 1
+        if v0.ndim not in (1, 2):
+            raise ValueError("excpecting v0 to have ndim > 0 and at most 2")
+        nv = 1 if squeeze else int(v0.shape[1])
+        vdt = L.np_dtype(v0.dtype) if on_dev else v0.dtype
+        if self._Ns <= 0:
+            if sparse:
+                return sp.csr_matrix(([], ([], [])), shape=(n_out, 0), dtype=vdt)
+            return np.zeros((n_out, 0), dtype=vdt)
+        if v0.shape[0] != n_in:
+            raise ValueError("v0 shape {0} not compatible with Ns_full={1}".format(tuple(v0.shape), Ns_full)
+                             if to_reduced else
+                             "v0 shape {0} not compatible with Ns={1}".format(tuple(v0.shape), self._Ns))
+        if sp.issparse(v0):
+            raise ValueError
+        if sparse and not on_dev:
+            P = self.get_proj(vdt, pcon=pcon)
+            # the reference's sparse forms: P^T (no conjugate) and P
+            # (base_general.py:671, 781)
+            return P.T.dot(sp.csr_matrix(v0.reshape(n_in, nv))) if to_reduced else P.dot(sp.csc_matrix(v0.reshape(n_in, nv)))
+        kdt = np.dtype(vdt)
+        if kdt not in (np.float32, np.float64, np.complex64, np.complex128):
+            kdt = np.result_type(kdt, np.float64)
+        x = L.to_device(v0, kdt).reshape(n_in, nv).contiguous()
+        y = L.zeros((n_out, nv), kdt)
+        fn = L.load().qsx_project_to_dense if to_reduced else L.load().qsx_project_from_dense
+        d = self._dev
+        L.check(fn(self._core.h, ptr(d["basis"]), ptr(d["n"]), d["n_bytes"], self._Ns, ptr(x), ptr(y),
+                   L.dtype_code(kdt), nv, pc, Ns_full, stream_handle()),
+                "qsx_project_to_dense" if to_reduced else "qsx_project_from_dense")
+        if on_dev:
+            return y.squeeze(1) if squeeze else y
+        out = L.to_host(y)
+        return out[:, 0] if squeeze else out
+
+    def project_to(self, v0, sparse=True, pcon=False):
+        """Full-space (or particle-sector) states to the reduced basis:
+        P^H v0 for the dense form (project_to_dense, base_general.py:584-694;
+        test/test_project_to.py:24-29), the reference's P^T v0 for the
+        sparse form."""
+        return self._project(v0, sparse, pcon, True)
+
+    def project_from(self, v0, sparse=True, pcon=False):
+        """Reduced-basis states to the full space (or particle sector):
+        P v0 (project_from_dense, base_general.py:696-794)."""
+        return self._project(v0, sparse, pcon, False)
+
+    def get_vec(self, v0, sparse=True, pcon=False):
+        """Deprecated name of ``project_from`` (base_general.py:684-694)."""
+        return self.project_from(v0, sparse=sparse, pcon=pcon)
+
+    def get_amp(self, states, out=None, amps=None, mode="representative"):
+        """Rescale factor C = psi_s / psi_r^sym between full-basis and
+        reduced amplitudes (get_amp, base_general.py:1183-1269), computed on
+        the device: C = sign conj(chi(g)) sqrt(n_r / |G|) for g.s = r.
+        ``mode="full_basis"`` replaces ``states`` in place by their
+        representatives."""
+        states_a = np.atleast_1d(np.asarray(states, order="C", dtype=self._basis_dtype))
+        states_shape = states_a.shape
+        if out is not None:
+            if states_shape != out.shape:
+                raise TypeError("states and out must have same shape.")
+            if out.dtype not in [np.float32, np.float64, np.complex64, np.complex128]:
+                raise TypeError("out must have datatype numpy.float32, numpy.float64, numpy.complex64, or "
+                                "numpy.complex128.")
+            if not out.flags["CARRAY"]:
+                raise ValueError("out must be C-contiguous array.")
+        elif amps is not None:
+            out = np.zeros(states_shape, dtype=amps.dtype)
+        else:
+            out = np.zeros(states_shape, dtype=np.complex128)
+        if mode not in ("representative", "full_basis"):
+            raise ValueError("mode accepts only the values 'representative' and 'full_basis'.")
+        s_d, _ = self._word_dev(states_a.ravel())
+        o_d = L.empty(states_a.size, out.dtype)
+        rc = L.load().qsx_get_amp(self._core.h, ptr(s_d), ptr(o_d), L.dtype_code(out.dtype), states_a.size,
+                                  1 if mode == "full_basis" else 0, stream_handle())
+        L.check(rc, "qsx_get_amp")
+        out[...] = L.to_host(o_d).reshape(states_shape)
+        if mode == "full_basis" and isinstance(states, np.ndarray) and states.dtype == self._basis_dtype:
+            states[...] = L.to_host(s_d).view(self._basis_dtype).reshape(states.shape)
+        if amps is not None:
+            if states_shape != amps.shape:
+                raise TypeError("states and amps must have same shape.")
+            if mode == "representative":
+                amps *= out
+            else:
+                amps /= out
+        return out.squeeze()
 
     # --------------------------------------------- hermiticity/symmetry checks
     def _expand_term(self, opstr, indx, J):

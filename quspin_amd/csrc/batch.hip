@@ -23,8 +23,6 @@ struct qsx_hv_batch {
   int64_t nvec = 1, total = 0;
   int chunk_rows = 16;
   int xcd_slices = 0;         // 1: each XCD takes a contiguous slice of the chunks
-  const double* xdiag = nullptr;  // optional extra diagonal over the stacked rows
-  double2 xdscale = make_double2(0.0, 0.0);
   std::vector<int64_t> row_off_h;
   HvArgs* sec_d = nullptr;
   CsrSec* csr_d = nullptr;    // CSR form
@@ -40,8 +38,6 @@ struct qsx_hv_batch {
   // captured Taylor loops keyed by (v, work, s_max, m_max)
   hipStream_t cap = nullptr;
   std::map<std::tuple<const void*, const void*, int, int>, hipGraphExec_t> graphs;
-  // captured time-step sequences keyed by (v, work, es, fail, nsteps, s_max, m_max)
-  std::map<std::tuple<const void*, const void*, const void*, const void*, int, int, int>, hipGraphExec_t> step_graphs;
 };
 
 // zero every norm slot and done flag
@@ -54,23 +50,9 @@ __global__ void k_batch_init(double* nB, double* nF, int64_t nslots, int* done, 
     done[i] = 0;
 }
 
-// Tail check of stage i (before its closing scale): a sector whose stage ran
-// all m terms without passing the truncation test gets fail[k] = 1.
-__global__ void k_batch_tail(const ExpmSec* __restrict__ es, const int* __restrict__ done, const double* nB,
-                             const double* nF, int64_t nslot, int mstride, int stage, int nsec, int* fail) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= nsec) return;
-  const ExpmSec E = es[k];
-  if (stage >= E.s || E.m < 1 || done[k]) return;
-  const double* nb = nB + k * nslot + (int64_t)stage * mstride;
-  const double* nf = nF + k * nslot + (int64_t)stage * mstride;
-  if (!(nb[E.m - 1] + nb[E.m] <= E.tol * nf[E.m])) fail[k] = 1;
-}
-
 static void free_batch(qsx_hv_batch* b) {
   if (!b) return;
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-  for (auto& kv : b->step_graphs) (void)hipGraphExecDestroy(kv.second);
   if (b->cap) (void)hipStreamDestroy(b->cap);
   (void)hipFree(b->sec_d);
   (void)hipFree(b->csr_d);
@@ -103,8 +85,6 @@ static int build_chunks(qsx_hv_batch* b) {
   // graphs hold the old chunk pointer
   for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
   b->graphs.clear();
-  for (auto& kv : b->step_graphs) (void)hipGraphExecDestroy(kv.second);
-  b->step_graphs.clear();
   return QSX_OK;
 }
 
@@ -215,25 +195,6 @@ extern "C" int qsx_hv_batch_rows(const qsx_hv_batch* b, int64_t* row_off) {
   return b->nsec;
 }
 
-// Extra diagonal over the stacked rows (device doubles, owned by the caller):
-// every product uses H + (re + i im) * diag(xdiag); null removes it.  Used
-// for time-dependent diagonal terms, whose factor each time step sets in its
-// own ExpmSec record (qsx_evolve_steps).
-extern "C" int qsx_hv_batch_set_diag(qsx_hv_batch* b, const double* xdiag, double re, double im) {
-  if (!b) { qsx_set_error("qsx_hv_batch_set_diag: null batch"); return QSX_ERR_ARG; }
-  const bool real = (b->dtype == QSX_F32 || b->dtype == QSX_F64);
-  if (real && im != 0.0) { qsx_set_error("real vectors with a complex diagonal factor"); return QSX_ERR_COMPLEX; }
-  if (xdiag != b->xdiag) {
-    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-    b->graphs.clear();
-    for (auto& kv : b->step_graphs) (void)hipGraphExecDestroy(kv.second);
-    b->step_graphs.clear();
-  }
-  b->xdiag = xdiag;
-  b->xdscale = make_double2(re, im);
-  return QSX_OK;
-}
-
 // Rows per wavefront chunk and chunk order (tuning knobs; results do not
 // depend on them).
 extern "C" int qsx_hv_batch_set_schedule(qsx_hv_batch* b, int chunk_rows, int xcd_slices) {
@@ -245,8 +206,6 @@ extern "C" int qsx_hv_batch_set_schedule(qsx_hv_batch* b, int chunk_rows, int xc
     b->xcd_slices = xcd_slices ? 1 : 0;
     for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
     b->graphs.clear();
-    for (auto& kv : b->step_graphs) (void)hipGraphExecDestroy(kv.second);
-    b->step_graphs.clear();
   }
   if (chunk_rows == b->chunk_rows) return QSX_OK;
   b->chunk_rows = chunk_rows;
@@ -273,8 +232,8 @@ static BatchArgs base_args(const qsx_hv_batch* b) {
   B.chunk = b->chunk_d;
   B.nchunk = b->nchunk;
   B.ld = b->nvec;
-  B.xdiag = b->xdiag;
-  B.xdscale = b->xdscale;
+  B.xdiag = nullptr;
+  B.xdscale = make_double2(0.0, 0.0);
   return B;
 }
 
@@ -338,7 +297,7 @@ extern "C" int qsx_expm_multiply_batch(qsx_hv_batch* b, const int32_t* s, const 
     std::memset(&E, 0, sizeof(E));
     E.a = make_double2(a_re, a_im);
     E.mu = make_double2(mu[2 * k], mu[2 * k + 1]);
-    E.dscale = b->xdscale;
+    E.dscale = make_double2(0.0, 0.0);
     E.tol = tol[k];
     E.s = s[k];
     E.m = m[k];
@@ -349,8 +308,6 @@ extern "C" int qsx_expm_multiply_batch(qsx_hv_batch* b, const int32_t* s, const 
   if (need > b->slots_cap) {
     for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
     b->graphs.clear();
-    for (auto& kv : b->step_graphs) (void)hipGraphExecDestroy(kv.second);
-    b->step_graphs.clear();
     QSX_CHECK_HIP(hipStreamSynchronize(st));
     (void)hipFree(b->slots_d);
     b->slots_d = nullptr;
@@ -418,106 +375,6 @@ extern "C" int qsx_expm_multiply_batch(qsx_hv_batch* b, const int32_t* s, const 
     (void)hipGraphDestroy(g);
     if (e != hipSuccess) { qsx_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
     it = b->graphs.emplace(key, ex).first;
-  }
-  QSX_CHECK_HIP(hipGraphLaunch(it->second, st));
-  return QSX_OK;
-}
-
-// One Taylor step per record: for step t = 0..nsteps-1, v <- e^{a_t H_t} v
-// with H_t = H + dscale_t * diag(xdiag) and every partition, shift and
-// tolerance read from the device records es[t * nsec + k] (e.g. written by
-// qsx_mlp_partition), so the whole sequence can be captured once and
-// replayed.  The launch shape uses the bounds s_max / m_max; terms beyond a
-// record's own (s, m) do nothing.  fail[t * nsec + k] is set when a stage of
-// step t ran all of its m terms without passing the truncation test (the
-// caller redoes such steps with an exact partition); it is not cleared here.
-extern "C" int qsx_evolve_steps(qsx_hv_batch* b, int nsteps, void* es, int s_max, int m_max, void* v, void* work,
-                                int* fail, int use_graph, void* stream) {
-  if (!b || nsteps < 1 || !es || s_max < 1 || m_max < 1 || !v || !work || !fail) {
-    qsx_set_error("qsx_evolve_steps: bad arguments");
-    return QSX_ERR_ARG;
-  }
-  if (b->nvec > 64) { qsx_set_error("qsx_evolve_steps: at most 64 vectors per sector"); return QSX_ERR_UNSUPPORTED; }
-  hipStream_t st = (hipStream_t)stream;
-  if (b->nchunk == 0) return QSX_OK;
-  const int64_t nslot = (int64_t)s_max * (m_max + 1);
-  const int64_t need = 2 * nslot * b->nsec;
-  if (need > b->slots_cap) {
-    for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
-    b->graphs.clear();
-    for (auto& kv : b->step_graphs) (void)hipGraphExecDestroy(kv.second);
-    b->step_graphs.clear();
-    QSX_CHECK_HIP(hipStreamSynchronize(st));
-    (void)hipFree(b->slots_d);
-    b->slots_d = nullptr;
-    QSX_CHECK_HIP(hipMalloc((void**)&b->slots_d, sizeof(double) * need));
-    b->slots_cap = need;
-  }
-  size_t esz = (b->dtype == QSX_F32) ? 4 : (b->dtype == QSX_F64 || b->dtype == QSX_C64) ? 8 : 16;
-  void* Bb[2] = {work, (char*)work + esz * b->total * b->nvec};
-  int xb;
-  const dim3 grid = wave_grid(b, &xb);
-  BatchArgs B0 = base_args(b);
-  B0.xcd_blocks = xb;
-  B0.ncol = (int)b->nvec;
-  B0.F = v;
-  B0.done = b->done_d;
-  B0.nB = b->slots_d;
-  B0.nF = b->slots_d + nslot * b->nsec;
-  B0.nslot = nslot;
-  B0.mstride = m_max + 1;
-  ExpmSec* E = (ExpmSec*)es;
-
-  auto record = [&](hipStream_t q) -> int {
-    for (int t = 0; t < nsteps; ++t) {
-      BatchArgs S0 = B0;
-      S0.es = E + (int64_t)t * b->nsec;
-      hipLaunchKernelGGL(k_batch_init, dim3(64), dim3(256), 0, q, S0.nB, S0.nF, nslot * b->nsec, b->done_d, b->nsec);
-      QSX_LAUNCH_CHECK();
-      BatchArgs S = S0;
-      S.y = Bb[0];
-      S.stage = 0;
-      wave_launch(b, S, 0, 1, grid, q);
-      QSX_LAUNCH_CHECK();
-      for (int i = 0; i < s_max; ++i) {
-        for (int j = 0; j < m_max; ++j) {
-          BatchArgs T = S0;
-          T.x = Bb[j & 1];
-          T.y = Bb[(j + 1) & 1];
-          T.stage = i;
-          T.term = j;
-          wave_launch(b, T, 1, 0, grid, q);
-          QSX_LAUNCH_CHECK();
-        }
-        hipLaunchKernelGGL(k_batch_tail, dim3((b->nsec + 63) / 64), dim3(64), 0, q, S0.es, b->done_d, S0.nB, S0.nF,
-                           nslot, m_max + 1, i, b->nsec, fail + (int64_t)t * b->nsec);
-        QSX_LAUNCH_CHECK();
-        BatchArgs C = S0;
-        C.y = Bb[0];
-        C.stage = i;
-        wave_launch(b, C, 0, 2, grid, q);
-        QSX_LAUNCH_CHECK();
-      }
-    }
-    return QSX_OK;
-  };
-  if (!use_graph) return record(st);
-  const auto key = std::make_tuple((const void*)v, (const void*)work, (const void*)es, (const void*)fail, nsteps, s_max,
-                                   m_max);
-  auto it = b->step_graphs.find(key);
-  if (it == b->step_graphs.end()) {
-    if (!b->cap) QSX_CHECK_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
-    QSX_CHECK_HIP(hipStreamBeginCapture(b->cap, hipStreamCaptureModeThreadLocal));
-    int rc = record(b->cap);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(b->cap, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) { qsx_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
-    hipGraphExec_t ex = nullptr;
-    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) { qsx_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
-    it = b->step_graphs.emplace(key, ex).first;
   }
   QSX_CHECK_HIP(hipGraphLaunch(it->second, st));
   return QSX_OK;

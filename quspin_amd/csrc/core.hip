@@ -443,10 +443,12 @@ static int64_t binom_host(int n, int k) {
 
 // Np: hcb -> list of particle numbers (n_Np entries, n_Np = 0 for no
 // conservation); spinful -> n_Np pairs (Nup, Ndown) flattened.
-// basis_out == NULL: count only.  Returns Ns >= 0, or -(Ns) - 1 ... see header.
-extern "C" int64_t qsx_make_basis(qsx_core* c, const int32_t* Np, int n_Np, void* basis_out, void* n_out,
-                                  int n_bytes, int64_t capacity, void* stream) {
-  if (!c) { qsx_set_error("qsx_make_basis: null core"); return QSX_ERR_ARG; }
+// basis_out == NULL: count only.  *Ns_out = Ns; QSX_ERR_OVERFLOW when
+// capacity < Ns (then nothing is written and *Ns_out is the size needed).
+extern "C" int qsx_make_basis(qsx_core* c, const int32_t* Np, int n_Np, void* basis_out, void* n_out, int n_bytes,
+                              int64_t capacity, int64_t* Ns_out, void* stream) {
+  if (!c || !Ns_out) { qsx_set_error("qsx_make_basis: null core or Ns_out"); return QSX_ERR_ARG; }
+  *Ns_out = -1;
   hipStream_t st = (hipStream_t)stream;
   const int N = c->h.N, L = c->h.L;
   if (n_bytes != 1 && n_bytes != 2 && n_bytes != 4) { qsx_set_error("n_bytes must be 1, 2 or 4"); return QSX_ERR_ARG; }
@@ -510,9 +512,9 @@ extern "C" int64_t qsx_make_basis(qsx_core* c, const int32_t* Np, int n_Np, void
     if (ferm) { QSX_M_SWITCH(c->h.nmaps, rc = (make_basis_impl<uint64_t, MM, true>(c, E, basis_out, n_out, n_bytes, capacity, &Ns, st))); }
     else { QSX_M_SWITCH(c->h.nmaps, rc = (make_basis_impl<uint64_t, MM, false>(c, E, basis_out, n_out, n_bytes, capacity, &Ns, st))); }
   }
-  if (rc == QSX_ERR_OVERFLOW) return -Ns - 1;
-  if (rc != QSX_OK) return rc;
-  return Ns;
+  if (rc == QSX_OK || rc == QSX_ERR_OVERFLOW) *Ns_out = Ns;
+  if (rc == QSX_ERR_OVERFLOW) qsx_set_error("qsx_make_basis: capacity below the number of states");
+  return rc;
 }
 
 // ---------------------------------------------------------------------------

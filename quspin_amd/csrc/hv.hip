@@ -140,6 +140,11 @@ void merge_pairs(std::vector<TermDev>& off, std::vector<TermDev>& hc) {
 
 }  // namespace
 
+int qsx_compile_term(const qsx_core* c, const char* opstr, const int32_t* indx, int n_op, double jr, double ji, int hc,
+                     TermDev& T, int& n_y) {
+  return compile_term(c, opstr, indx, n_op, jr, ji, hc, T, n_y);
+}
+
 struct qsx_oplist_full {
   qsx_oplist base;
   TermDev* off_hc = nullptr;  // device, Hermitian-conjugate off-diagonal terms
@@ -404,10 +409,9 @@ static void launch_op_coo(const HvArgs& A, const TermDev* T, void* row, void* co
 
 // me_out: complex128 (double2) per basis state; zero where the element
 // vanishes.  The caller casts and masks (base_general.py:338-350).
-extern "C" int qsx_op(qsx_core* c, const char* opstr, const int32_t* indx, int n_op, double J_re, double J_im,
-                      const void* basis, const void* nrm, int n_bytes, int64_t Ns, const int64_t* begin,
-                      const int64_t* end, int N_p, void* row, void* col, int idx_bytes, void* me_out,
-                      void* stream) {
+extern "C" int qsx_op(qsx_core* c, void* row, void* col, int idx_bytes, void* me_out, const char* opstr,
+                      const int32_t* indx, int n_op, double J_re, double J_im, const void* basis, const void* nrm,
+                      int n_bytes, int64_t Ns, const int64_t* begin, const int64_t* end, int N_p, void* stream) {
   if (!c || (idx_bytes != 4 && idx_bytes != 8)) { qsx_set_error("qsx_op: bad arguments"); return QSX_ERR_ARG; }
   hipStream_t st = (hipStream_t)stream;
   TermDev T;
@@ -548,6 +552,68 @@ extern "C" void qsx_csr_builder_destroy(qsx_csr_builder* B) {
   delete B;
 }
 
+// ---------------------------------------------------------------------------
+// Column 1-norms and diagonal of a matrix-free operator without assembling
+// it: lane r evaluates row r of the operator's transpose (= column r of the
+// operator) with the CSR builder's element code, sums the elements that land
+// on one row before taking |.| (as the canonical CSR of the reference holds
+// them, summed duplicates), and adds the shift on the diagonal.
+// ---------------------------------------------------------------------------
+constexpr int NORM_ROW_MAX = 64;  // distinct off-diagonal rows per column
+
+template <typename I, int M, bool FERM, int LK>
+__global__ void __launch_bounds__(256) k_norm_cols(HvArgs A, double2 shift, double* __restrict__ colabs,
+                                                   double2* __restrict__ dg, int* __restrict__ overflow) {
+  __shared__ int64_t sb[(LK == LOOKUP_TABLE || LK == LOOKUP_RANK) ? QSX_SB : 1];
+  stage_binom<LK>(A.R, sb);
+  __syncthreads();
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= A.Ns) return;
+  const I s = ((const I*)A.basis)[r];
+  double rsq = 1.0;
+  if constexpr (M != 0) rsq = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, r));
+  double2 d = row_diag<I, M, FERM, LK, true>(A, r, s);
+  const RowRank rr = row_rank_if<I, LK>(A, sb, s);
+  int64_t cols[NORM_ROW_MAX];
+  double2 vals[NORM_ROW_MAX];
+  int n = 0;
+  bool over = false;
+  for (int t = 0; t < A.n_off; ++t) {
+    term_element<I, M, FERM, LK, true>(A, sb, rr, A.off + t, s, rsq, A.off_conj, [&](int64_t c, double2 coef) {
+      if (c == r) { d = cadd(d, coef); return; }
+      int k = 0;
+      while (k < n && cols[k] != c) ++k;
+      if (k < n) vals[k] = cadd(vals[k], coef);
+      else if (n < NORM_ROW_MAX) { cols[n] = c; vals[n] = coef; ++n; }
+      else over = true;
+    });
+  }
+  dg[r] = d;
+  double sum = hypot(d.x + shift.x, d.y + shift.y);
+  for (int k = 0; k < n; ++k) sum += hypot(vals[k].x, vals[k].y);
+  colabs[r] = sum;
+  if (over) *overflow = 1;
+}
+
+template <typename I, int M, bool FERM, int LK>
+static void launch_norm_cols(const HvArgs& A, double2 shift, double* colabs, double2* dg, int* ovf, hipStream_t st) {
+  hipLaunchKernelGGL((k_norm_cols<I, M, FERM, LK>), dim3((unsigned)qsx_grid(A.Ns, 256)), dim3(256), 0, st, A, shift,
+                     colabs, dg, ovf);
+}
+
+template <typename I>
+static void norm_cols_dispatch(const HvArgs& A, double2 shift, double* colabs, double2* dg, int* ovf, int nmaps,
+                               bool fm, int lk, hipStream_t st) {
+  if (nmaps == 0) {
+    if (lk == LOOKUP_RANK) launch_norm_cols<I, 0, false, LOOKUP_RANK>(A, shift, colabs, dg, ovf, st);
+    else launch_norm_cols<I, 0, false, LOOKUP_SEARCH>(A, shift, colabs, dg, ovf, st);
+  } else if (fm) {
+    QSX_HV_MSWITCH(nmaps, (launch_norm_cols<I, MM, true, LOOKUP_SEARCH>(A, shift, colabs, dg, ovf, st)));
+  } else {
+    QSX_HV_MSWITCH(nmaps, (launch_norm_cols<I, MM, false, LOOKUP_SEARCH>(A, shift, colabs, dg, ovf, st)));
+  }
+}
+
 // Assemble a*Op (all terms of l; transposed/conjugated per flags) as CSR.
 // Returns nnz through *nnz; finish with qsx_csr_builder_finish.
 extern "C" int qsx_csr_builder_create(qsx_core* c, qsx_oplist* l, const void* basis, const void* nrm, int n_bytes,
@@ -643,7 +709,16 @@ struct qsx_hv_ctx {
   const qsx_core* c;
   const qsx_oplist_full* L;
   int dtype, transposed, conjugated;
+  uint64_t owner;  // id of the captures that hold A (renewed when A changes)
 };
+
+uint64_t qsx_hv_ctx_owner(const qsx_hv_ctx* X) { return X ? X->owner : 0; }
+
+// A changed: captures recorded with the old fields are dropped
+static void ctx_changed(qsx_hv_ctx* X) {
+  qsx_graph_drop_owner(X->owner);
+  X->owner = qsx_owner_new();
+}
 
 // Use a sector lookup table (qsx_table_build) instead of the orbit search.
 extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_bytes, int idx_bits, const void* lut,
@@ -657,6 +732,7 @@ extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_
   X->A.idx_bits = idx_bits;
   X->A.tlut = table ? (const double2*)lut : nullptr;
   X->A.lut_len = X->A.tlut ? lut_len : 0;
+  ctx_changed(X);
   return QSX_OK;
 }
 
@@ -666,6 +742,7 @@ extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_
 extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
   if (!X || sched < 0 || sched > 6) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
   X->A.sched = sched;
+  ctx_changed(X);
   return QSX_OK;
 }
 
@@ -694,6 +771,7 @@ extern "C" int qsx_hv_ctx_create(qsx_core* c, qsx_oplist* l, const void* basis, 
   X->dtype = dtype;
   X->transposed = transposed ? 1 : 0;
   X->conjugated = conjugated ? 1 : 0;
+  X->owner = qsx_owner_new();
   *out = X;
   return QSX_OK;
 }
@@ -714,10 +792,64 @@ extern "C" int qsx_hv_ctx_set_diag(qsx_hv_ctx* X, const void* diag_arr, int diag
   X->A.diag_conj = 0;
   X->A.diag_arr = diag_arr;
   X->A.diag_arr_cplx = diag_arr_cplx ? 1 : 0;
+  ctx_changed(X);
   return QSX_OK;
 }
 
-extern "C" void qsx_hv_ctx_destroy(qsx_hv_ctx* X) { delete X; }
+// The captures holding the context's fields go with it.
+extern "C" void qsx_hv_ctx_destroy(qsx_hv_ctx* X) {
+  if (!X) return;
+  qsx_graph_drop_owner(X->owner);
+  delete X;
+}
+
+// Column sums colabs[j] = sum_i |A_ij + shift delta_ij| and the diagonal
+// dg[j] = A_jj (Ns (re, im) pairs) of the context's operator A (its
+// off-diagonal terms with the context's flags, its diagonal terms and extra
+// diagonal).  The inputs of ExpmMultiplyParallel's mu = tr(A)/N and
+// ||A - mu I||_1 (_wrapper_csr_trace / _wrapper_csr_1_norm, EMP.html:196-202)
+// without building A.  QSX_ERR_UNSUPPORTED when a column holds more than 64
+// distinct off-diagonal rows, or for a transposed context whose list does not
+// commute with the group; the caller then assembles the CSR instead.
+extern "C" int qsx_hv_ctx_column_norms(qsx_hv_ctx* X, double shift_re, double shift_im, double* colabs, double* dg,
+                                       void* stream) {
+  if (!X || (X->A.Ns > 0 && (!colabs || !dg))) { qsx_set_error("qsx_hv_ctx_column_norms: bad arguments"); return QSX_ERR_ARG; }
+  if (X->A.Ns == 0) return QSX_OK;
+  const qsx_core* c = X->c;
+  const qsx_oplist_full* L = X->L;
+  // columns of A = rows of A^T: the flags of the transpose
+  const int t2 = X->transposed ? 0 : 1;
+  if (c->h.nmaps > 0 && !L->symmetric && !t2) {
+    qsx_set_error("column norms of a transposed list that does not commute with the symmetry group");
+    return QSX_ERR_UNSUPPORTED;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  HvArgs A = X->A;
+  A.off = t2 ? L->base.off : L->off_hc;
+  A.n_off = L->base.n_off;
+  A.off_conj = (t2 == X->conjugated) ? 1 : 0;
+  A.alpha_off = make_double2(1.0, 0.0);
+  A.alpha_diag = make_double2(1.0, 0.0);
+  A.shift = make_double2(0.0, 0.0);
+  A.table = nullptr;
+  int* ovf = nullptr;
+  QSX_CHECK_HIP(hipMallocAsync((void**)&ovf, sizeof(int), st));
+  QSX_CHECK_HIP(hipMemsetAsync(ovf, 0, sizeof(int), st));
+  const int lk = pick_lookup(c, A.Ns);
+  const double2 sh = make_double2(shift_re, shift_im);
+  if (c->h.N <= 32) norm_cols_dispatch<uint32_t>(A, sh, colabs, (double2*)dg, ovf, c->h.nmaps, c->h.fermion != 0, lk, st);
+  else norm_cols_dispatch<uint64_t>(A, sh, colabs, (double2*)dg, ovf, c->h.nmaps, c->h.fermion != 0, lk, st);
+  QSX_LAUNCH_CHECK();
+  int h = 0;
+  QSX_CHECK_HIP(hipMemcpyAsync(&h, ovf, sizeof(int), hipMemcpyDeviceToHost, st));
+  QSX_CHECK_HIP(hipFreeAsync(ovf, st));
+  QSX_CHECK_HIP(hipStreamSynchronize(st));
+  if (h) {
+    qsx_set_error("qsx_hv_ctx_column_norms: a column holds more than 64 distinct rows");
+    return QSX_ERR_UNSUPPORTED;
+  }
+  return QSX_OK;
+}
 
 // y = beta*y + a_diag*D x + a_off*Op x
 extern "C" int qsx_hv_ctx_apply(qsx_hv_ctx* X, const void* x, void* y, double a_off_re, double a_off_im,

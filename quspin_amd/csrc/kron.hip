@@ -47,7 +47,15 @@ struct qsx_kron {
   mutable void* z = nullptr;
   mutable size_t z_bytes = 0;
   int mode = 0;  // 0: by size, 1: one pass, 2: two passes
+  // id of the captured step sequences holding these fields (renewed when a
+  // field they hold changes)
+  mutable uint64_t owner = 0;
 };
+
+static void kron_changed(const qsx_kron* K) {
+  qsx_graph_drop_owner(K->owner);
+  K->owner = qsx_owner_new();
+}
 
 struct KronArgs {
   CsrArgs A;                 // vectors, scales and the expm fields (A.Ns = Nu * Nd)
@@ -381,8 +389,6 @@ static void launch_kron_t(const KronArgs& KA, const qsx_kron* K, hipStream_t st)
 
 static int64_t vt_size(int vdt) { return (vdt == QSX_F32) ? 4 : (vdt == QSX_F64 || vdt == QSX_C64) ? 8 : 16; }
 
-extern "C" void qsx_graph_cache_clear();
-
 // The row-pass buffer of the two-pass form, allocated outside any capture
 // (the entry points call this before launching or recording).  A larger
 // buffer replaces the old one and drops captured step sequences, which hold
@@ -392,7 +398,7 @@ static int kron_prepare(const qsx_kron* K, int vdt) {
   if (!kron_two_pass(K, vs)) return QSX_OK;
   const size_t need = (size_t)(K->Nu * K->Nd * vs);
   if (K->z_bytes >= need) return QSX_OK;
-  qsx_graph_cache_clear();
+  kron_changed(K);
   if (K->z) {
     QSX_CHECK_HIP(hipFree(K->z));
     K->z = nullptr;
@@ -502,6 +508,7 @@ extern "C" int qsx_kron_create(const void* ip_u, const void* ind_u, const void* 
     qsx_set_error("qsx_kron_create: device allocation or copy failed");
     return QSX_ERR_HIP;
   }
+  K->owner = qsx_owner_new();
   *out = K;
   return QSX_OK;
 }
@@ -510,6 +517,7 @@ extern "C" int qsx_kron_create(const void* ip_u, const void* ind_u, const void* 
 extern "C" int qsx_kron_set_mode(qsx_kron* K, int mode) {
   if (!K || mode < 0 || mode > 2) { qsx_set_error("qsx_kron_set_mode: bad arguments"); return QSX_ERR_ARG; }
   K->mode = mode;
+  kron_changed(K);
   return QSX_OK;
 }
 
@@ -517,7 +525,7 @@ extern "C" int qsx_kron_set_mode(qsx_kron* K, int mode) {
 // operator so that a later operator at recycled addresses is captured anew.
 extern "C" void qsx_kron_destroy(qsx_kron* K) {
   if (!K) return;
-  qsx_graph_cache_clear();
+  qsx_graph_drop_owner(K->owner);
   if (K->sell_off) (void)hipFree(K->sell_off);
   if (K->sell_idx) (void)hipFree(K->sell_idx);
   if (K->sell_val) (void)hipFree(K->sell_val);
@@ -610,7 +618,7 @@ int qsx_evolve_steps_with(int64_t Ns, int64_t nvec, const double* xdiag, int nst
                           int m_max, void* v, void* work, int vec_dtype, int state_dtype, int* fail, void* ctl_work,
                           int64_t ctl_bytes, int64_t part_blocks,
                           int (*term)(const CsrArgs&, const void*, int, int, hipStream_t), const void* term_ctx,
-                          const void* const key_ptr[3], int key_int, int use_graph, hipStream_t st);
+                          const StepOpKey& key, int use_graph, hipStream_t st);
 int qsx_part_reduce_launch(const CsrArgs& A, int64_t nblocks, hipStream_t st);
 
 static int kron_term(const CsrArgs& A, const void* ctx, int vdt, int fdt, hipStream_t st) {
@@ -626,8 +634,8 @@ extern "C" int qsx_evolve_kron_steps(const qsx_kron* K, const double* xdiag, int
   if (!K) { qsx_set_error("qsx_evolve_kron_steps: null operator"); return QSX_ERR_ARG; }
   const int rc = kron_prepare(K, vec_dtype);
   if (rc) return rc;
-  const void* key[3] = {K->dat_u, K->z, K};
+  const StepOpKey key{K->owner, {K->dat_u, K->z, K}, 0, 0, 1024 + 16 * K->mode + K->mat_dtype};
   return qsx_evolve_steps_with(K->Nu * K->Nd, 1, xdiag, nsteps, es, s_max, m_max, v, work, vec_dtype, state_dtype,
                                fail, ctl_work, ctl_bytes, kron_blocks(K, vt_size(vec_dtype)), kron_term, K, key,
-                               1024 + 16 * K->mode + K->mat_dtype, use_graph, (hipStream_t)stream);
+                               use_graph, (hipStream_t)stream);
 }

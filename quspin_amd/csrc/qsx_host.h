@@ -30,6 +30,21 @@ struct qsx_oplist {
 
 void qsx_set_error(const std::string& msg);
 
+// Captured launch sequences (hipGraphExec) hold kernel arguments by value.
+// An object whose fields enter those arguments (an H.v context, a factorised
+// operator) carries an owner id from qsx_owner_new(); captures are keyed on
+// the id, and qsx_graph_drop_owner(id) destroys them when the object changes
+// or is destroyed, so a later object at a recycled address is captured anew.
+uint64_t qsx_owner_new();
+void qsx_graph_drop_owner(uint64_t owner);
+// One operator string compiled for the kernels (hv.hip): characters applied
+// right to left, hc = 1 for the Hermitian conjugate; n_y counts 'y'.
+int qsx_compile_term(const qsx_core* c, const char* opstr, const int32_t* indx, int n_op, double jr, double ji, int hc,
+                     TermDev& T, int& n_y);
+// owner id of an H.v context (hv.hip)
+struct qsx_hv_ctx;
+uint64_t qsx_hv_ctx_owner(const qsx_hv_ctx* X);
+
 #define QSX_CHECK_HIP(expr)                                               \
   do {                                                                    \
     hipError_t _e = (expr);                                               \

@@ -7,7 +7,9 @@
 // Al-Mohy & Higham Alg. 3.2 as in scipy ``_expm_multiply_simple_core``
 // (scipy/sparse/linalg/_expm_multiply.py:280-303).
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <mutex>
 #include <string>
 #include <tuple>
 
@@ -305,21 +307,32 @@ struct RedScratch {
   unsigned int* counter = nullptr;
 };
 
+// partials (2 * RED_BLOCKS * 8 doubles) and 8 tickets; the tickets start at
+// zero and every reduction kernel resets its own after the last block
+static int alloc_red_scratch(RedScratch& R) {
+  QSX_CHECK_HIP(hipMalloc(&R.partials, sizeof(double) * 2 * RED_BLOCKS * 8));
+  QSX_CHECK_HIP(hipMalloc(&R.counter, sizeof(unsigned int) * 8));
+  QSX_CHECK_HIP(hipMemset(R.counter, 0, sizeof(unsigned int) * 8));
+  return QSX_OK;
+}
+
+// One scratch per (device, stream): reductions queued on different streams
+// never share partials or tickets.  Launches on one stream run in order, so
+// they may share it.  Kept until the process ends (64 KiB each).
 static int get_red_scratch(RedScratch& R, hipStream_t st) {
-  (void)st;
-  static thread_local double* p = nullptr;
-  static thread_local unsigned int* cnt = nullptr;
-  static thread_local int dev = -1;
+  static std::mutex mu;
+  static std::map<std::pair<int, hipStream_t>, RedScratch> per_stream;
   int cur = 0;
   QSX_CHECK_HIP(hipGetDevice(&cur));
-  if (!p || dev != cur) {
-    QSX_CHECK_HIP(hipMalloc(&p, sizeof(double) * 2 * RED_BLOCKS * 8));
-    QSX_CHECK_HIP(hipMalloc(&cnt, sizeof(unsigned int) * 8));
-    QSX_CHECK_HIP(hipMemset(cnt, 0, sizeof(unsigned int) * 8));
-    dev = cur;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = per_stream.find({cur, st});
+  if (it == per_stream.end()) {
+    RedScratch S;
+    const int rc = alloc_red_scratch(S);
+    if (rc) return rc;
+    it = per_stream.emplace(std::make_pair(cur, st), S).first;
   }
-  R.partials = p;
-  R.counter = cnt;
+  R = it->second;
   return QSX_OK;
 }
 
@@ -553,6 +566,17 @@ __global__ void __launch_bounds__(256) k_max_reduce(const double* __restrict__ x
   block_finish<1>(m, 0.0, partials, counter, out, 0.0);
 }
 
+// sum of n (re, im) pairs
+__global__ void __launch_bounds__(256) k_sum_c(const double2* __restrict__ x, int64_t n, double* partials,
+                                               unsigned int* counter, double* out) {
+  double a = 0.0, b = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    a += x[i].x;
+    b += x[i].y;
+  }
+  block_finish<0>(a, b, partials, counter, out, 0.0);
+}
+
 __global__ void k_fill_f64(double* p, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -565,6 +589,22 @@ __global__ void k_fill_f64(double* p, int64_t n, double v) {
     case QSX_C128: { using MT = double2; BODY; } break;           \
     default: qsx_set_error("unknown dtype"); return QSX_ERR_ARG;  \
   }
+
+// op 0: out[0] + i out[1] = sum of the n (re, im) pairs x; op 1: out[0] =
+// max of the n doubles x (device output, deterministic order).
+extern "C" int qsx_reduce(int op, const double* x, int64_t n, double* out, void* stream) {
+  if ((op != 0 && op != 1) || n < 0 || !out || (n > 0 && !x)) { qsx_set_error("qsx_reduce: bad arguments"); return QSX_ERR_ARG; }
+  hipStream_t st = (hipStream_t)stream;
+  RedScratch R;
+  int rc = get_red_scratch(R, st);
+  if (rc) return rc;
+  if (op == 0)
+    hipLaunchKernelGGL(k_sum_c, dim3(red_grid(n)), dim3(256), 0, st, (const double2*)x, n, R.partials, R.counter, out);
+  else
+    hipLaunchKernelGGL(k_max_reduce, dim3(red_grid(n)), dim3(256), 0, st, x, n, R.partials, R.counter, out);
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
 
 // out (device, 2 doubles) = trace
 extern "C" int qsx_csr_trace(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
@@ -946,22 +986,119 @@ extern "C" int64_t qsx_evolve_csr_ctl_bytes(int s_max, int m_max) {
 }
 
 namespace {
-using StepKey = std::tuple<const void*, const void*, const void*, const void*, const void*, const void*, const void*,
-                           const void*, const void*, int64_t, int64_t, int, int, int, int, int, int>;
-std::map<StepKey, hipGraphExec_t> g_step_graphs;
-using LzKey = std::tuple<const void*, const void*, const void*, const void*, const void*, const void*, int64_t, int, int,
-                         int, int>;
-std::map<LzKey, hipGraphExec_t> g_lz_graphs;
+// A captured sequence, the object whose fields it holds (0: none; the key
+// then names every array and size the sequence was recorded with) and the
+// reduction scratch it owns (Lanczos captures).
+struct GraphEntry {
+  hipGraphExec_t ex = nullptr;
+  uint64_t owner = 0;
+  uint64_t seq = 0;
+  RedScratch scratch;
+};
+using StepKey = std::tuple<uint64_t, const void*, const void*, const void*, int, int64_t, const void*, const void*,
+                           const void*, const void*, const void*, const void*, int64_t, int64_t, int, int, int, int, int>;
+std::map<StepKey, GraphEntry> g_step_graphs;
+using LzKey = std::tuple<uint64_t, const void*, const void*, const void*, int, int, int64_t, int, const void*,
+                         const void*, const void*, int64_t, int, int, int, int>;
+std::map<LzKey, GraphEntry> g_lz_graphs;
 hipStream_t g_cap_stream = nullptr;
+std::mutex g_graph_mu;
+std::atomic<uint64_t> g_next_owner{1};
+uint64_t g_graph_seq = 0;
+// captures kept at most; the oldest is dropped first (callers that pass new
+// buffers on every call would otherwise grow the cache without bound)
+constexpr size_t GRAPH_CACHE_MAX = 64;
+
+void destroy_entry(GraphEntry& e) {
+  if (e.ex) (void)hipGraphExecDestroy(e.ex);
+  if (e.scratch.partials) (void)hipFree(e.scratch.partials);
+  if (e.scratch.counter) (void)hipFree(e.scratch.counter);
+  e = GraphEntry{};
+}
+
+template <class Map, class Pred>
+size_t erase_if_entries(Map& m, Pred pred) {
+  size_t n = 0;
+  for (auto it = m.begin(); it != m.end();) {
+    if (pred(it->second)) {
+      destroy_entry(it->second);
+      it = m.erase(it);
+      ++n;
+    } else {
+      ++it;
+    }
+  }
+  return n;
+}
+
+bool any_entry(uint64_t owner) {
+  for (auto& kv : g_step_graphs)
+    if (kv.second.owner == owner) return true;
+  for (auto& kv : g_lz_graphs)
+    if (kv.second.owner == owner) return true;
+  return false;
+}
+
+// make room for one more capture (g_graph_mu held)
+void evict_oldest() {
+  while (g_step_graphs.size() + g_lz_graphs.size() >= GRAPH_CACHE_MAX) {
+    uint64_t oldest = UINT64_MAX;
+    for (auto& kv : g_step_graphs) oldest = kv.second.seq < oldest ? kv.second.seq : oldest;
+    for (auto& kv : g_lz_graphs) oldest = kv.second.seq < oldest ? kv.second.seq : oldest;
+    (void)hipDeviceSynchronize();  // a replay of it may still be queued
+    auto old = [&](const GraphEntry& e) { return e.seq == oldest; };
+    erase_if_entries(g_step_graphs, old);
+    erase_if_entries(g_lz_graphs, old);
+  }
+}
 }  // namespace
 
-// Drop every captured step sequence (call before freeing buffers that a
-// capture refers to if their addresses may be reused with other shapes).
+uint64_t qsx_owner_new() { return g_next_owner.fetch_add(1); }
+
+void qsx_graph_drop_owner(uint64_t owner) {
+  if (!owner) return;
+  std::lock_guard<std::mutex> lk(g_graph_mu);
+  if (!any_entry(owner)) return;
+  (void)hipDeviceSynchronize();  // queued replays finish before their graphs go
+  auto mine = [&](const GraphEntry& e) { return e.owner == owner; };
+  erase_if_entries(g_step_graphs, mine);
+  erase_if_entries(g_lz_graphs, mine);
+}
+
+// Drop every captured sequence.
 extern "C" void qsx_graph_cache_clear() {
-  for (auto& kv : g_step_graphs) (void)hipGraphExecDestroy(kv.second);
-  g_step_graphs.clear();
-  for (auto& kv : g_lz_graphs) (void)hipGraphExecDestroy(kv.second);
-  g_lz_graphs.clear();
+  std::lock_guard<std::mutex> lk(g_graph_mu);
+  if (g_step_graphs.empty() && g_lz_graphs.empty()) return;
+  (void)hipDeviceSynchronize();
+  auto all = [](const GraphEntry&) { return true; };
+  erase_if_entries(g_step_graphs, all);
+  erase_if_entries(g_lz_graphs, all);
+}
+
+// Capture record(stream) once into a new cache entry of `cache` under `key`
+// (g_graph_mu held by the caller).
+template <class Map, class Key, class Rec>
+static int capture_into(Map& cache, const Key& key, uint64_t owner, const RedScratch& scratch, Rec&& record,
+                        typename Map::iterator& out) {
+  evict_oldest();
+  if (!g_cap_stream) QSX_CHECK_HIP(hipStreamCreateWithFlags(&g_cap_stream, hipStreamNonBlocking));
+  QSX_CHECK_HIP(hipStreamBeginCapture(g_cap_stream, hipStreamCaptureModeThreadLocal));
+  int rc = record(g_cap_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(g_cap_stream, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) { qsx_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
+  hipGraphExec_t ex = nullptr;
+  e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) { qsx_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
+  GraphEntry ent;
+  ent.ex = ex;
+  ent.owner = owner;
+  ent.seq = ++g_graph_seq;
+  ent.scratch = scratch;
+  out = cache.emplace(key, ent).first;
+  return QSX_OK;
 }
 
 using TermFn = int (*)(const CsrArgs&, const void*, int, int, hipStream_t);
@@ -972,7 +1109,7 @@ using TermFn = int (*)(const CsrArgs&, const void*, int, int, hipStream_t);
 int qsx_evolve_steps_with(int64_t Ns, int64_t nvec, const double* xdiag, int nsteps, const void* es, int s_max,
                           int m_max, void* v, void* work, int vec_dtype, int state_dtype, int* fail, void* ctl_work,
                           int64_t ctl_bytes, int64_t part_blocks, TermFn term, const void* term_ctx,
-                          const void* const key_ptr[3], int key_int, int use_graph, hipStream_t st) {
+                          const StepOpKey& opk, int use_graph, hipStream_t st) {
   if (Ns < 0 || nsteps < 1 || !es || s_max < 1 || m_max < 1 || !v || !work || !fail || !ctl_work || nvec < 1 ||
       ctl_bytes < qsx_evolve_csr_ctl_bytes(s_max, m_max) || part_blocks > EPI_MAX_BLOCKS) {
     qsx_set_error("qsx_evolve_steps: bad arguments");
@@ -1058,26 +1195,17 @@ int qsx_evolve_steps_with(int64_t Ns, int64_t nvec, const double* xdiag, int nst
     return QSX_OK;
   };
   if (!use_graph) return record(st);
-  // key_ptr names the operator's arrays (the cache is keyed on addresses; it
-  // is cleared with qsx_graph_cache_clear when those are freed).
-  const StepKey key{key_ptr[0], key_ptr[1], key_ptr[2], xdiag, es, v, work, fail, ctl_work, Ns, nvec, nsteps, s_max,
-                    m_max, key_int, 0, vec_dtype * 8 + state_dtype};
+  // the key names every operator array, size and buffer the sequence is
+  // recorded with; an operator object's fields are covered by its owner id
+  const StepKey key{opk.owner, opk.p[0], opk.p[1], opk.p[2], opk.idx64, opk.nnz, xdiag, es, v, work, fail,
+                    ctl_work, Ns, nvec, nsteps, s_max, m_max, opk.kind, vec_dtype * 8 + state_dtype};
+  std::lock_guard<std::mutex> lk(g_graph_mu);
   auto it = g_step_graphs.find(key);
   if (it == g_step_graphs.end()) {
-    if (!g_cap_stream) QSX_CHECK_HIP(hipStreamCreateWithFlags(&g_cap_stream, hipStreamNonBlocking));
-    QSX_CHECK_HIP(hipStreamBeginCapture(g_cap_stream, hipStreamCaptureModeThreadLocal));
-    int rc = record(g_cap_stream);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(g_cap_stream, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) { qsx_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
-    hipGraphExec_t ex = nullptr;
-    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) { qsx_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
-    it = g_step_graphs.emplace(key, ex).first;
+    const int rc = capture_into(g_step_graphs, key, opk.owner, RedScratch{}, record, it);
+    if (rc) return rc;
   }
-  QSX_CHECK_HIP(hipGraphLaunch(it->second, st));
+  QSX_CHECK_HIP(hipGraphLaunch(it->second.ex, st));
   return QSX_OK;
 }
 
@@ -1108,10 +1236,10 @@ extern "C" int qsx_evolve_csr_steps(const void* indptr, const void* indices, con
                                     int group, int use_graph, void* stream) {
   if (Ns > 0 && (!indptr || !indices || !data)) { qsx_set_error("qsx_evolve_csr_steps: bad arguments"); return QSX_ERR_ARG; }
   CsrTermCtx c{indptr, indices, data, idx64, mat_dtype, group > 0 ? group : auto_group(Ns, nnz), nnz};
-  const void* key[3] = {indptr, indices, data};
+  const StepOpKey key{0, {indptr, indices, data}, idx64, nnz, c.G * 16 + mat_dtype};
   return qsx_evolve_steps_with(Ns, nvec, xdiag, nsteps, es, s_max, m_max, v, work, vec_dtype, state_dtype, fail,
-                               ctl_work, ctl_bytes, csr_blocks(Ns, c.G, true), csr_term, &c, key,
-                               c.G * 16 + mat_dtype, use_graph, (hipStream_t)stream);
+                               ctl_work, ctl_bytes, csr_blocks(Ns, c.G, true), csr_term, &c, key, use_graph,
+                               (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -1130,9 +1258,10 @@ extern "C" int qsx_lanczos_run(int op_kind, void* hv_ctx, const void* indptr, co
     qsx_set_error("qsx_lanczos_run: bad arguments");
     return QSX_ERR_ARG;
   }
+  // reduction partials: the stream's scratch for a direct launch sequence,
+  // a scratch of the capture's own for a captured one (allocated before the
+  // capture begins and freed with it)
   RedScratch R;
-  int rc = get_red_scratch(R, st);  // allocated before any capture
-  if (rc) return rc;
   const size_t esz = (dtype == QSX_F32) ? 4 : (dtype == QSX_F64 || dtype == QSX_C64) ? 8 : 16;
   auto qi = [&](int i) -> void* {
     return (char*)Q + esz * (size_t)n * (size_t)(store_Q ? i : (i % 3));
@@ -1145,9 +1274,9 @@ extern "C" int qsx_lanczos_run(int op_kind, void* hv_ctx, const void* indptr, co
   // reduced on every block), r = A q_i, r -= b_{i-1} q_{i-1} with the
   // partials of <q_i, r>, r -= a_i q_i with the partials of ||r||^2
   const int nb = lz_grid(n);
-  double* pdot = R.partials + 4 * RED_BLOCKS;  // 1,024 slots each
-  double* pnrm = R.partials + 6 * RED_BLOCKS;
   auto record = [&](hipStream_t q) -> int {
+    double* pdot = R.partials + 4 * RED_BLOCKS;  // 1,024 slots each
+    double* pnrm = R.partials + 6 * RED_BLOCKS;
     for (int i = 0; i < m; ++i) {
       double* bp = ab + 4 * (int64_t)(i - 1) + 2;  // b_{i-1}
       if (i > 0) {
@@ -1170,23 +1299,25 @@ extern "C" int qsx_lanczos_run(int op_kind, void* hv_ctx, const void* indptr, co
     QSX_LAUNCH_CHECK();
     return QSX_OK;
   };
-  if (!use_graph) return record(st);
-  const LzKey key{hv_ctx, indptr, data, Q, r, ab, n, m, store_Q, dtype, op_kind};
+  if (!use_graph) {
+    const int rc = get_red_scratch(R, st);
+    if (rc) return rc;
+    return record(st);
+  }
+  const uint64_t owner = op_kind == 0 ? qsx_hv_ctx_owner((const qsx_hv_ctx*)hv_ctx) : 0;
+  const LzKey key{owner, indptr, indices, data, idx64, mat_dtype, nnz, group, Q, r, ab, n, m, store_Q, dtype,
+                  op_kind};
+  std::lock_guard<std::mutex> lk(g_graph_mu);
   auto it = g_lz_graphs.find(key);
   if (it == g_lz_graphs.end()) {
-    if (!g_cap_stream) QSX_CHECK_HIP(hipStreamCreateWithFlags(&g_cap_stream, hipStreamNonBlocking));
-    QSX_CHECK_HIP(hipStreamBeginCapture(g_cap_stream, hipStreamCaptureModeThreadLocal));
-    rc = record(g_cap_stream);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(g_cap_stream, &g);
-    if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
-    if (e != hipSuccess) { qsx_set_error(std::string("graph capture: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
-    hipGraphExec_t ex = nullptr;
-    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (e != hipSuccess) { qsx_set_error(std::string("graph instantiate: ") + hipGetErrorString(e)); return QSX_ERR_HIP; }
-    it = g_lz_graphs.emplace(key, ex).first;
+    int rc = alloc_red_scratch(R);
+    if (rc == QSX_OK) rc = capture_into(g_lz_graphs, key, owner, R, record, it);
+    if (rc) {
+      if (R.partials) (void)hipFree(R.partials);
+      if (R.counter) (void)hipFree(R.counter);
+      return rc;
+    }
   }
-  QSX_CHECK_HIP(hipGraphLaunch(it->second, st));
+  QSX_CHECK_HIP(hipGraphLaunch(it->second.ex, st));
   return QSX_OK;
 }

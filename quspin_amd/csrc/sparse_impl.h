@@ -52,6 +52,17 @@ __device__ __forceinline__ double2 expm_eta(const ExpmSec& E) {
   return make_double2(mag * cos(ei), mag * sin(ei));
 }
 
+// The operator a captured step sequence was recorded from: the owner id of
+// an object whose fields the term kernel holds (0 for none), its arrays and
+// the sizes and kind that select the launches.
+struct StepOpKey {
+  uint64_t owner;
+  const void* p[3];
+  int idx64;
+  int64_t nnz;
+  int kind;
+};
+
 struct CsrArgs {
   const void* indptr;   // int32 or int64
   const void* indices;  // int32 or int64

@@ -222,16 +222,48 @@ class MatrixFreeOp:
         self.apply(ones, out, a=0.0, a_diag=1.0)
         return out
 
+    def _column_norms(self, shift):
+        """(column 1-norms of A + shift I, diagonal of A as (re, im) pairs),
+        device arrays from one pass over the operator's columns
+        (``qsx_hv_ctx_column_norms``), or None when the operator needs its
+        CSR for them (a column with more than 64 distinct rows, or a
+        transposed list that does not commute with the group)."""
+        ctx = self._raw_ctx(np.dtype(np.complex128), 1)
+        colabs = L.empty(self.Ns, np.float64)
+        dg = L.empty((self.Ns, 2), np.float64)
+        sc = complex(shift)
+        rc = L.load().qsx_hv_ctx_column_norms(ctx.h, sc.real, sc.imag, ptr(colabs), ptr(dg), stream_handle())
+        if rc == L.QSX_ERR_UNSUPPORTED:
+            return None
+        L.check(rc, "qsx_hv_ctx_column_norms")
+        return colabs, dg
+
     def trace(self):
-        d = self.diagonal(np.complex128)
-        ones = L.torch().ones(self.Ns, dtype=L.torch().complex128, device="cuda")
-        return vdot_dev(ones, d)
+        """tr(A) over the whole reduced matrix: the diagonal terms plus the
+        off-diagonal terms that map a representative onto itself (what
+        _wrapper_csr_trace sums on the reference's CSR, EMP.html:196)."""
+        if self.Ns == 0:
+            return 0.0
+        r = self._column_norms(0.0)
+        if r is None:
+            return self.to_csr(self.dtype if self.dtype.kind == "c" else np.complex128).trace()
+        out = L.zeros(2, np.float64)
+        L.check(L.load().qsx_reduce(0, ptr(r[1]), self.Ns, ptr(out), stream_handle()), "qsx_reduce")
+        o = L.to_host(out)
+        return complex(o[0], o[1]) if (self.dtype.kind == "c" or o[1] != 0) else float(o[0])
 
     def one_norm(self, shift=0.0):
-        """max_c sum_r |A_rc + shift delta_rc| through an explicit CSR build
-        (cheap next to the expm it feeds)."""
-        csr = self.to_csr()
-        return csr.one_norm(shift)
+        """max_c sum_r |A_rc + shift delta_rc| (_wrapper_csr_1_norm,
+        EMP.html:200-202) from one pass over the operator's columns; the CSR
+        is assembled only when that pass cannot hold a column."""
+        if self.Ns == 0:
+            return 0.0
+        r = self._column_norms(shift)
+        if r is None:
+            return self.to_csr().one_norm(shift)
+        out = L.zeros(2, np.float64)
+        L.check(L.load().qsx_reduce(1, ptr(r[0]), self.Ns, ptr(out), stream_handle()), "qsx_reduce")
+        return float(L.to_host(out)[0])
 
     def to_csr(self, dtype=np.complex128):
         return build_csr(self.basis, self.oplist, dtype, self.transposed, self.conjugated, diag=self.diag)

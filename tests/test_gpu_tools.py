@@ -147,8 +147,8 @@ def test_lanczos_ground_state_and_expm():
     np.testing.assert_allclose(v2, v1, atol=1e-10, rtol=0)
 
 
-def test_lanczos_device_C2_ground_energy():
-    """200-step Lanczos on the C2 sector from device vectors: ground energy
+def test_lanczos_device_L20_kpz_ground_energy():
+    """100-step Lanczos on the L=20 k=0, p=+1, z=+1 sector from device vectors: ground energy
     agrees with the oracle recurrence on the same device matvec to 1e-9."""
     import torch
     from quspin_amd.basis import spin_basis_general

@@ -157,6 +157,36 @@ def test_expm_against_dense_and_scipy_partition():
         assert part[0] >= 1 and part[1] >= 1
 
 
+def test_partition_inputs_and_theta_branch_choice():
+    """The oracle's mu, ||A - mu I||_1 and theta-branch (m*, s) against what
+    fixes them independently: the dense trace and column sums; and, for
+    |a| ||A - mu I||_1 <= 63.36, the branch's defining property (Al-Mohy &
+    Higham 2011, eq. 3.10 and Table 3.1): s = ceil(|a| ||.||_1 / theta_m) for
+    the chosen m, and no other m of the theta table has a smaller cost m * s
+    (ties to the smaller m, which the table's ascending scan keeps)."""
+    from scipy.sparse.linalg._expm_multiply import _theta
+    rng = np.random.default_rng(1)
+    A = sp.random(80, 80, density=0.1, random_state=5, dtype=np.float64).tocsr()
+    A = (A + 1j * sp.random(80, 80, density=0.05, random_state=6)).tocsr()
+    Ad = A.toarray()
+    mu = O.csr_trace(A) / 80
+    assert abs(mu - np.trace(Ad) / 80) < 1e-14
+    nrm = O.csr_1_norm(A, -mu)
+    assert abs(nrm - np.abs(Ad - mu * np.eye(80)).sum(axis=0).max()) < 1e-12
+    for a in (-0.01j, -0.3j, -2.0j, 1.7, -5.5j):
+        an = abs(a) * nrm
+        assert an <= 63.36
+        m, s = O.fragment_3_1_partition(A, a, mu, nrm, 2.0 ** -53)
+        assert s == int(np.ceil(an / _theta[m]))
+        costs = {mm: mm * int(np.ceil(an / th)) for mm, th in _theta.items() if mm <= 55}
+        best = min(costs.values())
+        assert m * s == best and m == min(mm for mm, c in costs.items() if c == best)
+    assert O.fragment_3_1_partition(A, 0.0, mu, nrm, 2.0 ** -53) == (0, 1)
+    v = rng.normal(size=80) + 0j
+    got, _ = O.expm_multiply(A, v, a=0.0)
+    np.testing.assert_array_equal(got, v)
+
+
 def test_lanczos_ground_energy():
     L = 10
     m = chain_maps(L)
@@ -216,3 +246,49 @@ def test_mlp_backward_finite_differences():
             flat[idx] = old
             fd = np.sum(dy * (yp - ym)) / (2 * h)
             assert abs(fd - G.reshape(-1)[idx]) < 1e-7 * max(1.0, abs(fd)), (bi, idx)
+
+
+def test_shift_sector_and_get_amp_identities():
+    """op_shift_sector: for an operator that commutes with the group it
+    equals the reduced operator of the sector (P^H H P, the A.5 element
+    formula); for the momentum-q operator sum_j e^{-2 pi i q j / L} z_j the
+    shifted parts over all target sectors k2 rebuild H_full P_k1 v
+    (sum_k2 P_k2 P_k2^H = 1).  get_amp: psi_full[s] = C_s psi_rep(s) for every
+    state of the sector, and |C_s|^2 sums to 1 over each orbit."""
+    L = 8
+    m = chain_maps(L)
+    full = O.full_basis_states(L, L // 2)
+    rng = np.random.default_rng(2)
+    groups = {k: O.Group(L, [m["T"]], [L], [k]) for k in range(L)}
+    bases = {k: O.make_basis(L, groups[k], Np=L // 2) for k in range(L)}
+    st = flat_terms(xxz_static(L, Delta=0.4))
+    b, n = bases[3]
+    v = rng.normal(size=len(b)) + 1j * rng.normal(size=len(b))
+    got = O.op_shift_sector(b, n, groups[3], b, n, groups[3], full, L, st, v)
+    np.testing.assert_allclose(got, O.reduced_op(b, n, groups[3], st) @ v, atol=1e-12)
+    q = 2
+    zq = [("z", (j,), np.exp(-2j * np.pi * q * j / L)) for j in range(L)]
+    Hf = O.full_hamiltonian(full, L, zq)
+    P1 = O.projector(b, n, groups[3], full)
+    acc = np.zeros(full.size, complex)
+    for k2 in range(L):
+        b2, n2 = bases[k2]
+        if len(b2) == 0:
+            continue
+        part = O.op_shift_sector(b2, n2, groups[k2], b, n, groups[3], full, L, zq, v)
+        if k2 != (3 + q) % L:
+            assert np.abs(part).max() < 1e-12
+        acc += O.projector(b2, n2, groups[k2], full) @ part
+    np.testing.assert_allclose(acc, Hf @ (P1 @ v), atol=1e-12)
+    # get_amp
+    for k in (0, 3):
+        b, n = bases[k]
+        psi = rng.normal(size=len(b)) + 1j * rng.normal(size=len(b))
+        psi_full = O.projector(b, n, groups[k], full) @ psi
+        C, rep = O.get_amp(b, n, groups[k], full, full)
+        cols = O.index_of(b, rep)
+        ok = cols >= 0
+        np.testing.assert_allclose(psi_full[ok], C[ok] * psi[cols[ok]], atol=1e-12)
+        assert np.abs(psi_full[~ok]).max(initial=0) < 1e-12
+        for c in range(len(b)):
+            assert abs(np.sum(np.abs(C[cols == c]) ** 2) - 1.0) < 1e-12
