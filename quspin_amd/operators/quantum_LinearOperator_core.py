@@ -131,7 +131,24 @@ class quantum_LinearOperator(LinearOperator):
             raise ValueError("diagonal must be 1-d array with size Ns.")
         cplx = np.iscomplexobj(diagonal)
         self._user_diag = L.to_device(diagonal.astype(np.complex128 if cplx else np.float64))
+        if hasattr(self, "_oplist"):
+            # the new array replaces the whole diagonal, the diagonal terms'
+            # part included (the reference overwrites ``_diagonal``,
+            # quantum_LinearOperator_core.py:231-234)
+            off = [t for t in self._static_list if not self._diagonal_term(*t)]
+            self._static_list = off
+            self._oplist = self._basis._compile_oplist(off)
         self._ops = {}
+
+    def _diagonal_term(self, opstr, indx, J):
+        """True for a term that changes no state (its flips cancel), which
+        the reference moves into ``_diagonal`` (quantum_LinearOperator_core.py:146-156)."""
+        opstr, indx, _ = self._basis._opstr_prepare(opstr, indx, J)
+        flip = 0
+        for ch, i in zip(opstr, indx):
+            if ch in "+-xy":
+                flip ^= 1 << int(i)
+        return flip == 0
 
     # ------------------------------------------------------------ kernels
     def _op_for(self, transposed, conjugated):
