@@ -260,6 +260,23 @@ int qsx_expm_multiply_csr(const void* indptr, const void* indices, const void* d
                           double a_re, double a_im, void* v, void* work, int vec_dtype, int64_t nvec, int group,
                           void* stream);
 
+/* Allocation-free form of qsx_expm_multiply_csr (same arguments, same
+ * kernels, same results).  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory -- code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * Taylor loop's control block (truncation flag, per-term norms, per-block
+ * partials) lives in a library-owned device array of
+ * qsx_expm_multiply_csr_noalloc_max_bytes() bytes (4 MiB) per GPU, held until
+ * the process ends; a partition whose block does not fit is rejected with
+ * QSX_ERR_ARG.  Every call on one GPU shares that array: calls that overlap
+ * on one GPU (other streams, other threads) race.  Each call clears the
+ * block itself, with a kernel. */
+int qsx_expm_multiply_csr_noalloc(const void* indptr, const void* indices, const void* data, int idx64,
+                                  int mat_dtype, int64_t Ns, int64_t nnz, int s, int m_star, double tol, double mu_re,
+                                  double mu_im, double a_re, double a_im, void* v, void* work, int vec_dtype,
+                                  int64_t nvec, int group, void* stream);
+int64_t qsx_expm_multiply_csr_noalloc_max_bytes(void);
+
 /* The same Taylor loop on the matrix-free operator of an H.v context (no
  * CSR in memory); no counterpart in the reference. */
 int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star, double tol, double mu_re, double mu_im,

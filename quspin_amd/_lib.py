@@ -82,6 +82,9 @@ SIGNATURES = {
     "qsx_csr_1_norm_diag": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _dbl, _dbl, _vp, _dbl, _dbl, _vp, _vp, _vp]),
     "qsx_expm_multiply_csr": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _i32, _i32, _dbl, _dbl, _dbl, _dbl,
                                      _dbl, _vp, _vp, _i32, _i64, _i32, _vp]),
+    "qsx_expm_multiply_csr_noalloc": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _i32, _i32, _dbl, _dbl, _dbl,
+                                             _dbl, _dbl, _vp, _vp, _i32, _i64, _i32, _vp]),
+    "qsx_expm_multiply_csr_noalloc_max_bytes": (_i64, []),
     "qsx_expm_multiply_mf": (_i32, [_vp, _i64, _i32, _i32, _dbl, _dbl, _dbl, _dbl, _dbl, _vp, _vp, _i32, _i64,
                                     _vp]),
     "qsx_vdot": (_i32, [_i32, _vp, _vp, _i64, _vp, _vp]),

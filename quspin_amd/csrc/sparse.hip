@@ -770,22 +770,22 @@ static int expm_ctl_alloc(const ExpmPlan& P, ExpmCtl** ctl, double** slots, hipS
   return QSX_OK;
 }
 
-// CSR Taylor loop: v (Ns, nvec) is F and is overwritten; work has 2*Ns*nvec.
-extern "C" int qsx_expm_multiply_csr(const void* indptr, const void* indices, const void* data, int idx64,
-                                     int mat_dtype, int64_t Ns, int64_t nnz, int s, int m_star, double tol,
-                                     double mu_re, double mu_im, double a_re, double a_im, void* v, void* work,
-                                     int vec_dtype, int64_t nvec, int group, void* stream) {
-  hipStream_t st = (hipStream_t)stream;
-  if (Ns <= 0) return QSX_OK;
-  if (s < 1 || m_star < 0) { qsx_set_error("qsx_expm_multiply_csr: bad partition"); return QSX_ERR_ARG; }
-  const bool real = (vec_dtype == QSX_F32 || vec_dtype == QSX_F64);
-  if (real && (a_im != 0.0 || mu_im != 0.0)) { qsx_set_error("real vectors with complex a or mu"); return QSX_ERR_COMPLEX; }
-  ExpmPlan P{s, m_star, tol, make_double2(a_re, a_im), make_double2(mu_re, mu_im), Ns, nvec, vec_dtype};
-  ExpmCtl* ctl = nullptr;
-  double* slots = nullptr;
+// CSR Taylor loop on a caller-provided control block: ctl (ExpmCtl, then
+// slots and per-block partials) initialised in-kernel, so the block may be
+// reused by every call.
+static int expm_csr_loop(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
+                         int64_t Ns, int64_t nnz, const ExpmPlan& P, void* v, void* work, int group, ExpmCtl* ctl,
+                         hipStream_t st) {
+  const int s = P.s, m_star = P.m;
+  const double a_re = P.a.x, a_im = P.a.y, mu_re = P.mu.x, mu_im = P.mu.y;
+  const int vec_dtype = P.vdt;
+  const int64_t nvec = P.nvec;
   const int nsp = CSR_NSP;
-  int rc = expm_ctl_alloc(P, &ctl, &slots, st, nsp);
-  if (rc) return rc;
+  double* slots = (double*)((char*)ctl + sizeof(ExpmCtl));
+  const int64_t nsl = (int64_t)s * (m_star + 1) * 2 * nsp;
+  hipLaunchKernelGGL(k_expm_init_ctl, dim3(qsx_grid(nsl, 256) > 0 ? (unsigned)qsx_grid(nsl, 256) : 1u), dim3(256), 0, st,
+                     ctl, P.tol, slots, nsl);
+  QSX_LAUNCH_CHECK();
   const int64_t stride = (int64_t)(m_star + 1) * nsp;  // slot (i, j) at normB + i * stride + j * nsp
   double* normB = slots;
   double* normF = slots + (int64_t)s * stride;
@@ -821,7 +821,7 @@ extern "C" int qsx_expm_multiply_csr(const void* indptr, const void* indices, co
         A.chk_c2 = normB + i * stride + j * nsp;
         A.chk_F = normF + i * stride + j * nsp;
       }
-      rc = launch_csr<true>(A, mat_dtype, vec_dtype, G, st);
+      const int rc = launch_csr<true>(A, mat_dtype, vec_dtype, G, st);
       if (rc) return rc;
     }
     double* nb0 = (i + 1 < s) ? normB + (i + 1) * stride : nullptr;
@@ -829,8 +829,96 @@ extern "C" int qsx_expm_multiply_csr(const void* indptr, const void* indices, co
                                                 nvec, eta, nb0, ctl, nsp));
     QSX_LAUNCH_CHECK();
   }
-  (void)hipFreeAsync(ctl, st);
   return QSX_OK;
+}
+
+// bytes of the control block of an (s, m_star) loop: ExpmCtl, the norm
+// slots, 2 spare doubles and the per-block norm partials of the term kernel
+static int64_t expm_csr_ctl_bytes(int s, int m_star) {
+  return (int64_t)sizeof(ExpmCtl) +
+         (int64_t)sizeof(double) * ((int64_t)s * (m_star + 1) * 2 * CSR_NSP + 2 + 2 * EPI_MAX_BLOCKS);
+}
+
+static int expm_csr_check(int64_t Ns, int s, int m_star, int vec_dtype, double a_im, double mu_im, const char* who) {
+  if (s < 1 || m_star < 0) { qsx_set_error(std::string(who) + ": bad partition"); return QSX_ERR_ARG; }
+  const bool real = (vec_dtype == QSX_F32 || vec_dtype == QSX_F64);
+  if (real && (a_im != 0.0 || mu_im != 0.0)) { qsx_set_error("real vectors with complex a or mu"); return QSX_ERR_COMPLEX; }
+  (void)Ns;
+  return QSX_OK;
+}
+
+// CSR Taylor loop: v (Ns, nvec) is F and is overwritten; work has 2*Ns*nvec.
+extern "C" int qsx_expm_multiply_csr(const void* indptr, const void* indices, const void* data, int idx64,
+                                     int mat_dtype, int64_t Ns, int64_t nnz, int s, int m_star, double tol,
+                                     double mu_re, double mu_im, double a_re, double a_im, void* v, void* work,
+                                     int vec_dtype, int64_t nvec, int group, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (Ns <= 0) return QSX_OK;
+  int rc = expm_csr_check(Ns, s, m_star, vec_dtype, a_im, mu_im, "qsx_expm_multiply_csr");
+  if (rc) return rc;
+  ExpmPlan P{s, m_star, tol, make_double2(a_re, a_im), make_double2(mu_re, mu_im), Ns, nvec, vec_dtype};
+  ExpmCtl* ctl = nullptr;
+  QSX_CHECK_HIP(hipMallocAsync((void**)&ctl, (size_t)expm_csr_ctl_bytes(s, m_star), st));
+  rc = expm_csr_loop(indptr, indices, data, idx64, mat_dtype, Ns, nnz, P, v, work, group, ctl, st);
+  (void)hipFreeAsync(ctl, st);
+  return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free form of qsx_expm_multiply_csr.  Its control block lives in
+// the file-scope device array `scratch` (one copy per GPU, kept until the
+// process ends); the host addresses it only through the pointer that
+// hipGetSymbolAddress gives for the current device, fetched once per device.
+// Every call on one GPU shares that GPU's array: calls that overlap on one
+// GPU race.  The loop's init kernel clears what must start at zero on every
+// call (no memset node, so the sequence can be captured).
+// ---------------------------------------------------------------------------
+constexpr int64_t QSX_NOALLOC_SCRATCH_BYTES = 4ll << 20;  // 4 MiB
+static __device__ __align__(256) unsigned char scratch[QSX_NOALLOC_SCRATCH_BYTES];
+
+static int noalloc_scratch(void** out) {
+  constexpr int MAX_DEV = 64;
+  static std::mutex mu;
+  static void* ptrs[MAX_DEV] = {};
+  int dev = 0;
+  QSX_CHECK_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= MAX_DEV) { qsx_set_error("device index out of range"); return QSX_ERR_UNSUPPORTED; }
+  std::lock_guard<std::mutex> lk(mu);
+  if (!ptrs[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      qsx_set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return QSX_ERR_HIP;
+    }
+    ptrs[dev] = p;
+  }
+  *out = ptrs[dev];
+  return QSX_OK;
+}
+
+extern "C" int64_t qsx_expm_multiply_csr_noalloc_max_bytes(void) { return QSX_NOALLOC_SCRATCH_BYTES; }
+
+extern "C" int qsx_expm_multiply_csr_noalloc(const void* indptr, const void* indices, const void* data, int idx64,
+                                             int mat_dtype, int64_t Ns, int64_t nnz, int s, int m_star, double tol,
+                                             double mu_re, double mu_im, double a_re, double a_im, void* v,
+                                             void* work, int vec_dtype, int64_t nvec, int group, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  if (Ns <= 0) return QSX_OK;
+  int rc = expm_csr_check(Ns, s, m_star, vec_dtype, a_im, mu_im, "qsx_expm_multiply_csr_noalloc");
+  if (rc) return rc;
+  const int64_t need = expm_csr_ctl_bytes(s, m_star);
+  if (need > QSX_NOALLOC_SCRATCH_BYTES) {
+    qsx_set_error("qsx_expm_multiply_csr_noalloc: partition (s = " + std::to_string(s) + ", m = " +
+                  std::to_string(m_star) + ") needs " + std::to_string(need) + " bytes of control block, above the " +
+                  std::to_string(QSX_NOALLOC_SCRATCH_BYTES) + "-byte scratch");
+    return QSX_ERR_ARG;
+  }
+  void* p = nullptr;
+  rc = noalloc_scratch(&p);
+  if (rc) return rc;
+  ExpmPlan P{s, m_star, tol, make_double2(a_re, a_im), make_double2(mu_re, mu_im), Ns, nvec, vec_dtype};
+  return expm_csr_loop(indptr, indices, data, idx64, mat_dtype, Ns, nnz, P, v, work, group, (ExpmCtl*)p, st);
 }
 
 // Matrix-free Taylor loop: same recurrence with H.v from the fused kernel.
