@@ -710,7 +710,172 @@ struct qsx_hv_ctx {
   const qsx_oplist_full* L;
   int dtype, transposed, conjugated;
   uint64_t owner;  // id of the captures that hold A (renewed when A changes)
+  // split-chain plan of schedule 7 (device allocations owned here)
+  std::vector<void*> split_bufs;
+  SplitDev* split_d = nullptr;
 };
+
+static void split_free(qsx_hv_ctx* X) {
+  for (void* p : X->split_bufs) (void)hipFree(p);
+  X->split_bufs.clear();
+  X->split_d = nullptr;
+  X->A.split = nullptr;
+  X->A.split_lds = 0;
+  X->A.split_nblk = 0;
+}
+
+// Build the split-chain plan of a context (hard-core-boson sector of N <= 32
+// bits, no symmetry maps, every off-diagonal term in the bit-parallel form).
+static int split_build(qsx_hv_ctx* X) {
+  const qsx_core* c = X->c;
+  const qsx_oplist_full* L = X->L;
+  const int N = c->h.N;
+  if (c->h.nmaps != 0 || c->h.kind != KIND_HCB || N > 32 || !c->rank_ok || c->rank_h.mode != 0 ||
+      c->rank_h.Ns != X->A.Ns || !L->all_fast || X->A.nvec != 1) {
+    qsx_set_error("split-chain schedule: needs one hard-core-boson particle sector without symmetry maps "
+                  "(N <= 32), bit-parallel terms and one vector");
+    return QSX_ERR_UNSUPPORTED;
+  }
+  const int Np = c->rank_h.Np;
+  const int esz = (X->dtype == QSX_F32) ? 4 : (X->dtype == QSX_F64 || X->dtype == QSX_C64) ? 8 : 16;
+  // lower half as wide as the LDS budget allows (largest block C(K, c) elements)
+  int K = N / 2;
+  auto binom = [](int n, int k) -> int64_t {
+    if (k < 0 || k > n) return 0;
+    int64_t v = 1;
+    for (int i = 1; i <= k; ++i) v = v * (n - k + i) / i;
+    return v;
+  };
+  auto maxblk = [&](int k) { int64_t m = 0; for (int q = 0; q <= k; ++q) m = std::max(m, binom(k, q)); return m; };
+  while (K > 1 && (maxblk(K) * esz > 64 * 1024 || K > 16)) --K;
+  const int U = N - K;
+  if (K < 1 || U < 1 || U > 20) { qsx_set_error("split-chain schedule: sector too small or too wide"); return QSX_ERR_UNSUPPORTED; }
+  // d grouped by popcount, descending inside a group; position of each d
+  std::vector<int32_t> cls_off(K + 2, 0);
+  for (int q = 0; q <= K; ++q) cls_off[q + 1] = cls_off[q] + (int32_t)binom(K, q);
+  std::vector<uint16_t> dlist((size_t)1 << K), dpos((size_t)1 << K);
+  {
+    std::vector<int32_t> fill(cls_off.begin(), cls_off.end() - 1);
+    for (int64_t d = ((int64_t)1 << K) - 1; d >= 0; --d) {
+      const int q = __builtin_popcountll((uint64_t)d);
+      dpos[d] = (uint16_t)(fill[q] - cls_off[q]);
+      dlist[fill[q]++] = (uint16_t)d;
+    }
+  }
+  // first row of every block, u descending
+  std::vector<int64_t> ublk((size_t)1 << U, -1);
+  std::vector<int32_t> order;
+  int64_t row = 0;
+  for (int64_t u = ((int64_t)1 << U) - 1; u >= 0; --u) {
+    const int q = Np - __builtin_popcountll((uint64_t)u);
+    if (q < 0 || q > K) continue;
+    ublk[u] = row;
+    row += binom(K, q);
+    order.push_back((int32_t)u);
+  }
+  if (row != X->A.Ns) { qsx_set_error("split-chain schedule: layout does not match the basis"); return QSX_ERR_UNSUPPORTED; }
+  // largest blocks first, grouped by the popcount of u
+  std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+    const int64_t la = binom(K, Np - __builtin_popcount((uint32_t)a)), lb = binom(K, Np - __builtin_popcount((uint32_t)b));
+    if (la != lb) return la > lb;
+    return __builtin_popcount((uint32_t)a) < __builtin_popcount((uint32_t)b);
+  });
+  // terms of the gather form this context uses, by the bits they flip and read
+  const std::vector<TermDev>& terms = X->transposed ? L->base.off_h : L->off_hc_h;
+  std::vector<TermDev> ell, low, up, upu, cross;
+  const uint64_t lmask = ((uint64_t)1 << K) - 1;
+  for (const TermDev& T : terms) {
+    const uint64_t rd = T.req_mask | T.smask;
+    if ((T.flip & ~lmask) == 0) ((rd & ~lmask) == 0 ? ell : low).push_back(T);
+    else if ((T.flip & lmask) == 0) ((rd & lmask) == 0 ? upu : up).push_back(T);
+    else cross.push_back(T);
+  }
+  if (upu.size() > 64) { qsx_set_error("split-chain schedule: more than 64 upper-half terms"); return QSX_ERR_UNSUPPORTED; }
+  // the lower-half terms that read lower bits only: sliced ELL per class,
+  // coefficients through a table of distinct values
+  const bool oconj = (X->transposed == X->conjugated);
+  std::vector<double2> lut(1, make_double2(0.0, 0.0));
+  auto lut_id = [&](double2 v) -> int {
+    for (size_t i = 1; i < lut.size(); ++i)
+      if (lut[i].x == v.x && lut[i].y == v.y) return (int)i;
+    lut.push_back(v);
+    return (int)lut.size() - 1;
+  };
+  std::vector<int64_t> ell_off(K + 2, 0);
+  std::vector<int32_t> ell_k(K + 1, 0);
+  std::vector<std::vector<uint32_t>> rows_e((size_t)1 << K);
+  for (int q = 0; q <= K; ++q) {
+    const int nq = cls_off[q + 1] - cls_off[q];
+    int km = 0;
+    for (int pp = 0; pp < nq; ++pp) {
+      const uint32_t d = dlist[cls_off[q] + pp];
+      auto& row_e = rows_e[cls_off[q] + pp];
+      for (const TermDev& T : ell) {
+        const uint64_t v = d & T.req_mask;
+        if (v != T.req_val && !(T.has2 && v == T.req_val2)) continue;
+        const uint32_t d2 = d ^ (uint32_t)T.flip;
+        if (__builtin_popcount(d2) != q) continue;
+        const double w = (__builtin_popcountll(d & T.smask) & 1) ? -1.0 : 1.0;
+        double2 cf = make_double2(T.cr * w, T.ci * w);
+        if (oconj) cf.y = -cf.y;
+        row_e.push_back((uint32_t)dpos[d2] | ((uint32_t)lut_id(cf) << 16));
+      }
+      km = std::max<int>(km, (int)row_e.size());
+    }
+    ell_k[q] = km;
+    ell_off[q + 1] = ell_off[q] + (int64_t)km * nq;
+  }
+  if (lut.size() > 1024) { qsx_set_error("split-chain schedule: more than 1024 distinct coefficients"); return QSX_ERR_UNSUPPORTED; }
+  std::vector<uint32_t> ellv((size_t)std::max<int64_t>(ell_off[K + 1], 1), 0u);
+  for (int q = 0; q <= K; ++q) {
+    const int nq = cls_off[q + 1] - cls_off[q];
+    for (int pp = 0; pp < nq; ++pp) {
+      const auto& row_e = rows_e[cls_off[q] + pp];
+      for (int k = 0; k < ell_k[q]; ++k)
+        ellv[ell_off[q] + (int64_t)k * nq + pp] = k < (int)row_e.size() ? row_e[k] : (uint32_t)pp;
+    }
+  }
+  split_free(X);
+  auto up_copy = [&](const void* src, size_t bytes, void** dst) -> hipError_t {
+    *dst = nullptr;
+    if (bytes == 0) return hipSuccess;
+    hipError_t e = hipMalloc(dst, bytes);
+    if (e != hipSuccess) return e;
+    X->split_bufs.push_back(*dst);
+    return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  };
+  SplitDev H;
+  std::memset(&H, 0, sizeof(H));
+  H.K = K;
+  H.nblk = (int)order.size();
+  H.n_low = (int)low.size(); H.n_up = (int)up.size(); H.n_cross = (int)cross.size(); H.n_upu = (int)upu.size();
+  H.n_lut = (int)lut.size();
+  void* p = nullptr;
+  hipError_t e = hipSuccess;
+  if (e == hipSuccess) { e = up_copy(low.data(), sizeof(TermDev) * low.size(), &p); H.low = (const TermDev*)p; }
+  if (e == hipSuccess) { e = up_copy(up.data(), sizeof(TermDev) * up.size(), &p); H.up = (const TermDev*)p; }
+  if (e == hipSuccess) { e = up_copy(cross.data(), sizeof(TermDev) * cross.size(), &p); H.cross = (const TermDev*)p; }
+  if (e == hipSuccess) { e = up_copy(upu.data(), sizeof(TermDev) * upu.size(), &p); H.upu = (const TermDev*)p; }
+  if (e == hipSuccess) { e = up_copy(ell_off.data(), sizeof(int64_t) * ell_off.size(), &p); H.ell_off = (const int64_t*)p; }
+  if (e == hipSuccess) { e = up_copy(ell_k.data(), sizeof(int32_t) * ell_k.size(), &p); H.ell_k = (const int32_t*)p; }
+  if (e == hipSuccess) { e = up_copy(ellv.data(), sizeof(uint32_t) * ellv.size(), &p); H.ell = (const uint32_t*)p; }
+  if (e == hipSuccess) { e = up_copy(lut.data(), sizeof(double2) * lut.size(), &p); H.lut = (const double2*)p; }
+  if (e == hipSuccess) { e = up_copy(dpos.data(), sizeof(uint16_t) * dpos.size(), &p); H.dpos = (const uint16_t*)p; }
+  if (e == hipSuccess) { e = up_copy(dlist.data(), sizeof(uint16_t) * dlist.size(), &p); H.dlist = (const uint16_t*)p; }
+  if (e == hipSuccess) { e = up_copy(cls_off.data(), sizeof(int32_t) * cls_off.size(), &p); H.cls_off = (const int32_t*)p; }
+  if (e == hipSuccess) { e = up_copy(ublk.data(), sizeof(int64_t) * ublk.size(), &p); H.ublk = (const int64_t*)p; }
+  if (e == hipSuccess) { e = up_copy(order.data(), sizeof(int32_t) * order.size(), &p); H.order = (const int32_t*)p; }
+  if (e == hipSuccess) { e = up_copy(&H, sizeof(H), &p); X->split_d = (SplitDev*)p; }
+  if (e != hipSuccess) {
+    split_free(X);
+    qsx_set_error(std::string("split-chain schedule: ") + hipGetErrorString(e));
+    return QSX_ERR_HIP;
+  }
+  X->A.split = X->split_d;
+  X->A.split_lds = (int)(maxblk(K) * esz);
+  X->A.split_nblk = H.nblk;
+  return QSX_OK;
+}
 
 uint64_t qsx_hv_ctx_owner(const qsx_hv_ctx* X) { return X ? X->owner : 0; }
 
@@ -737,10 +902,16 @@ extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_
 }
 
 // 0: one lane per row; 1: compacted work queue; 2/3/4: 2/4/8 lanes per row;
-// 5/6: batches of 4/8 terms with rank-based lookups (table or no symmetry).
+// 5/6: batches of 4/8 terms with rank-based lookups (table or no symmetry);
+// 7: split-chain blocks (one hard-core-boson sector without symmetry maps;
+// the plan is built on the first request and owned by the context).
 // Single vector; several vectors always use one lane per row.
 extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
-  if (!X || sched < 0 || sched > 6) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
+  if (!X || sched < 0 || sched > 7) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
+  if (sched == 7 && !X->split_d) {
+    const int rc = split_build(X);
+    if (rc) return rc;
+  }
   X->A.sched = sched;
   ctx_changed(X);
   return QSX_OK;
@@ -800,6 +971,7 @@ extern "C" int qsx_hv_ctx_set_diag(qsx_hv_ctx* X, const void* diag_arr, int diag
 extern "C" void qsx_hv_ctx_destroy(qsx_hv_ctx* X) {
   if (!X) return;
   qsx_graph_drop_owner(X->owner);
+  split_free(X);
   delete X;
 }
 

@@ -119,6 +119,38 @@ struct HvArgs {
   int n_sb;           // binomial entries staged by k_hv_rows_fast_batch (even)
   int sched;          // 0: one lane per row, 1: compacted queue (one vector)
   int fast_rows;      // every off-diagonal term bit-parallel, no diagonal terms, single hcb sector
+  const struct SplitDev* split;  // split-chain plan (schedule 7), device memory
+  int split_lds;      // bytes of LDS the split kernel stages (largest block)
+  int split_nblk;     // its workgroups (blocks of the layout)
+};
+
+// Split-chain layout of one hard-core-boson particle sector of N <= 32 bits
+// without symmetry maps: a state is s = (u << K) | d, u the upper N - K
+// bits and d the lower K bits.  The descending basis lists every u in
+// descending order and, for each, the d of popcount Np - popc(u) in
+// descending order: the rows of one u form a contiguous block, a term that
+// flips only lower bits maps the block onto itself, and one that flips only
+// upper bits maps block u onto block u ^ flip at the same position.
+struct SplitDev {
+  int K, nblk;
+  int n_low, n_up, n_cross, n_upu, n_lut;
+  const TermDev* low;     // terms flipping lower bits only, reading some upper bit
+  const TermDev* up;      // terms flipping upper bits only, reading some lower bit
+  const TermDev* cross;   // terms flipping bits of both halves
+  const TermDev* upu;     // terms flipping and reading upper bits only (uniform over a block)
+  // the lower-bit terms that read lower bits only, as one sparse matrix per
+  // popcount class c of d in sliced-ELL form: entry k of position p at
+  // ell_off[c] + k * n_c + p (n_c = C(K, c)), ell_k[c] entries per position,
+  // each (partner position) | (coefficient id << 16); padding (p, id 0 = 0)
+  const int64_t* ell_off;
+  const int32_t* ell_k;
+  const uint32_t* ell;
+  const double2* lut;     // n_lut coefficients (lut[0] = 0)
+  const uint16_t* dpos;   // [2^K] position of d among the d of its popcount, descending
+  const uint16_t* dlist;  // [2^K] the d grouped by popcount, descending inside a group
+  const int32_t* cls_off; // [K + 2] group offsets into dlist
+  const int64_t* ublk;    // [2^(N-K)] first row of the block of u (-1: no such block)
+  const int32_t* order;   // [nblk] u of each workgroup (largest blocks first)
 };
 
 __device__ __forceinline__ double load_n(const void* p, int nb, int64_t i) {
@@ -1111,6 +1143,176 @@ void launch_hv_one(const HvArgs& A, hipStream_t st) {
     hipLaunchKernelGGL((k_hv_rows<I, VT, M, FERM, LK, 0>), grid, dim3(blk), 0, st, A);
 }
 
+// Split-chain H.v (schedule 7; sector and terms as k_hv_rows_fast requires,
+// no symmetry maps, N <= 32, one vector).  One workgroup per block u: the
+// block's x is staged in LDS for the lower-bit terms, an upper-bit term
+// reads the partner block's x at the same positions (consecutive lanes,
+// consecutive addresses), the few terms across the halves gather from their
+// target block.  Blocks are launched largest first and grouped by the
+// popcount of u, so the partner blocks of the ones in flight belong to one
+// popcount class (94 MB of complex64 at N = 28) and stay in the last-level
+// cache.
+constexpr int SPLIT_THREADS = 512;
+constexpr int SPLIT_MAX_UPU = 64;
+constexpr int SPLIT_MAX_K = 16;
+constexpr int SPLIT_MAX_LUT = 1024;
+
+template <typename VT>
+__global__ void __launch_bounds__(SPLIT_THREADS) k_hv_split(HvArgs A) {
+  extern __shared__ unsigned char split_smem[];
+  __shared__ int64_t ub[SPLIT_MAX_UPU];
+  __shared__ double2 uc[SPLIT_MAX_UPU];
+  __shared__ int nu;
+  __shared__ int64_t sbk[SPLIT_MAX_K * 33];  // C(n, k) for n < K (the ranks of d)
+  __shared__ double2 lut[SPLIT_MAX_LUT];
+  if (A.skip && *A.skip) return;
+  const SplitDev S = *A.split;  // by value: the kernel's stores cannot alias it
+  const uint32_t u = (uint32_t)S.order[blockIdx.x];
+  const int K = S.K;
+  const int Np = A.R->Np;
+  const int c = Np - __builtin_popcount(u);
+  const int off = S.cls_off[c];
+  const int n = S.cls_off[c + 1] - off;
+  const int64_t base = S.ublk[u];
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ vout = (VT*)A.v_out;
+  VT* xs = (VT*)split_smem;
+  constexpr bool CPLX = VTr<VT>::cplx;
+  for (int p = threadIdx.x; p < n; p += blockDim.x) xs[p] = vin[base + p];
+  for (int i = threadIdx.x; i < K * 33; i += blockDim.x) sbk[i] = (&A.R->binom[0][0])[i];
+  for (int i = threadIdx.x; i < S.n_lut; i += blockDim.x) lut[i] = S.lut[i];
+  // upper-bit terms that read upper bits only: the same partner block and
+  // coefficient for every row of the block
+  if (threadIdx.x == 0) {
+    int k = 0;
+    const uint32_t su = u << K;
+    for (int t = 0; t < S.n_upu && k < SPLIT_MAX_UPU; ++t) {
+      const TermDev* __restrict__ T = S.upu + t;
+      const uint32_t v = su & (uint32_t)T->req_mask;
+      if (v != (uint32_t)T->req_val && !(T->has2 && v == (uint32_t)T->req_val2)) continue;
+      const uint32_t u2 = u ^ (uint32_t)(T->flip >> K);
+      if (__builtin_popcount(u2) != Np - c) continue;
+      const double w = (__builtin_popcount(su & (uint32_t)T->smask) & 1) ? -1.0 : 1.0;
+      double2 coef = make_double2(T->cr * w, T->ci * w);
+      if (A.off_conj) coef = cconj(coef);
+      ub[k] = S.ublk[u2];
+      uc[k] = coef;
+      ++k;
+    }
+    nu = k;
+  }
+  __syncthreads();
+  const uint32_t dmask = (1u << K) - 1u;
+  const uint16_t* __restrict__ dl = S.dlist + off;
+  const int nup = nu;
+  const int ke = S.ell_k[c];
+  const uint32_t* __restrict__ el = S.ell + S.ell_off[c];
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    const int64_t r = base + p;
+    // the row's own reads first (diagonal), then the partner blocks' values
+    // in batches of 8, all requested before any is used
+    double2 dg = make_double2(0.0, 0.0);
+    if (A.diag_arr) {
+      if (A.diag_arr_cplx) dg = ((const double2*)A.diag_arr)[r];
+      else dg.x = ((const double*)A.diag_arr)[r];
+    }
+    for (int j0 = 0; j0 < nup; j0 += 8) {
+      double2 xv[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        xv[q] = (j0 + q < nup) ? VTr<VT>::ld(vin + ub[j0 + q] + p) : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if (j0 + q >= nup) break;
+        if constexpr (CPLX) acc = cadd(acc, cmul(uc[j0 + q], xv[q]));
+        else acc.x += uc[j0 + q].x * xv[q].x;
+      }
+    }
+    // lower half: the class's sparse matrix (entries read coalesced, 8 at a
+    // time) against the staged block
+    for (int k0 = 0; k0 < ke; k0 += 8) {
+      uint32_t ev[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) ev[q] = (k0 + q < ke) ? el[(int64_t)(k0 + q) * n + p] : (uint32_t)p;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const double2 x = VTr<VT>::ld(xs + (ev[q] & 0xFFFFu));
+        const double2 cf = lut[ev[q] >> 16];
+        if constexpr (CPLX) acc = cadd(acc, cmul(cf, x));
+        else acc.x += cf.x * x.x;
+      }
+    }
+    const uint32_t d = dl[p];
+    const uint32_t s = (u << K) | d;
+    const int64_t ra = (int64_t)n - 1 - p;  // ascending rank of d in its class
+    auto add = [&](const TermDev* __restrict__ T, const double2 x) {
+      const double w = (__builtin_popcount(s & (uint32_t)T->smask) & 1) ? -1.0 : 1.0;
+      if constexpr (CPLX) {
+        double2 coef = make_double2(T->cr * w, T->ci * w);
+        if (A.off_conj) coef = cconj(coef);
+        acc = cadd(acc, cmul(coef, x));
+      } else {
+        acc.x += T->cr * w * x.x;
+      }
+    };
+    auto applies = [&](const TermDev* __restrict__ T) -> bool {
+      const uint32_t v = s & (uint32_t)T->req_mask;
+      return v == (uint32_t)T->req_val || (T->has2 && v == (uint32_t)T->req_val2);
+    };
+    // lower half: the partner position from the rank of d, the value from LDS
+    for (int t = 0; t < S.n_low; ++t) {
+      const TermDev* __restrict__ T = S.low + t;
+      if (!applies(T)) continue;
+      const uint32_t d2 = d ^ (uint32_t)T->flip;
+      if (__builtin_popcount(d2) != c) continue;
+      const int lo = T->lo, hi = T->hi;
+      int64_t rk;
+      if (hi == lo + 1) {
+        const int64_t dl1 = sbk[lo * 33 + __builtin_popcount(d & ((1u << lo) - 1u))];
+        rk = ((d >> lo) & 1u) ? ra + dl1 : ra - dl1;
+      } else {
+        rk = ra - window_rank<uint32_t>(sbk, d, lo, hi) + window_rank<uint32_t>(sbk, d2, lo, hi);
+      }
+      add(T, VTr<VT>::ld(xs + (n - 1 - rk)));
+    }
+    for (int t = 0; t < S.n_up; ++t) {
+      const TermDev* __restrict__ T = S.up + t;
+      if (!applies(T)) continue;
+      const uint32_t u2 = u ^ (uint32_t)(T->flip >> K);
+      if (__builtin_popcount(u2) != Np - c) continue;
+      add(T, VTr<VT>::ld(vin + S.ublk[u2] + p));
+    }
+    for (int t = 0; t < S.n_cross; ++t) {
+      const TermDev* __restrict__ T = S.cross + t;
+      if (!applies(T)) continue;
+      const uint32_t s2 = s ^ (uint32_t)T->flip;
+      if (__builtin_popcount(s2) != Np) continue;
+      const int64_t b2 = S.ublk[s2 >> K];
+      if (b2 < 0) continue;
+      add(T, VTr<VT>::ld(vin + b2 + S.dpos[s2 & dmask]));
+    }
+    const double2 dsum = cadd(cmul(dg, A.alpha_diag), A.shift);
+    const double2 xr = VTr<VT>::ld(xs + p);
+    double2 res;
+    if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc), cmul(dsum, xr));
+    else res = make_double2(A.alpha_off.x * acc.x + dsum.x * xr.x, 0.0);
+    if (A.beta) res = cadd(res, VTr<VT>::ld(vout + r));
+    VTr<VT>::st(vout + r, res);
+  }
+}
+
+template <typename VT>
+int launch_hv_split(const HvArgs& A, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_hv_split<VT>, dim3((unsigned)nblk), dim3(SPLIT_THREADS), (size_t)A.split_lds, st, A);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    qsx_set_error(std::string("split-chain launch failed: ") + hipGetErrorString(e));
+    return QSX_ERR_HIP;
+  }
+  return QSX_OK;
+}
+
 template <typename I, typename VT>
 int launch_hv_rows(const HvArgs& A, int nmaps, bool fermion_maps, int lookup, hipStream_t st) {
   if (lookup == LOOKUP_TABLE) {
@@ -1166,6 +1368,8 @@ int qsx_hv_launch_c128(const HvArgs& A, int N, int nmaps, bool fm, int lookup, i
 
 #define QSX_DEFINE_HV_LAUNCH(NAME, VT)                                                         \
   int NAME(const HvArgs& A, int N, int nmaps, bool fm, int lookup, int scatter, hipStream_t st) { \
+    if (!scatter && A.split && A.sched == 7 && A.fast_rows && A.nvec == 1)                     \
+      return launch_hv_split<VT>(A, A.split_nblk, st);                                        \
     if (scatter) {                                                                             \
       if (N <= 32) return launch_hv_scatter<uint32_t, VT>(A, nmaps, fm, lookup, st);           \
       return launch_hv_scatter<uint64_t, VT>(A, nmaps, fm, lookup, st);                        \

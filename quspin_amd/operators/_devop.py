@@ -423,8 +423,14 @@ class _Ctx:
                         "qsx_hv_ctx_set_table")
         else:
             L.check(lib.qsx_hv_ctx_set_table(self.h, None, 0, 0, None, 0), "qsx_hv_ctx_set_table")
-        L.check(lib.qsx_hv_ctx_set_schedule(self.h, tuning.SCHEDULES.get(cfg.get("schedule"), 0)),
-                "qsx_hv_ctx_set_schedule")
+        code = tuning.SCHEDULES.get(cfg.get("schedule"), 0)
+        rc = lib.qsx_hv_ctx_set_schedule(self.h, code)
+        if rc == L.QSX_ERR_UNSUPPORTED and code == tuning.SCHEDULES["split"]:
+            # the split-chain layout needs one hcb sector without symmetry
+            # maps and bit-parallel terms: other operators keep one lane per row
+            self.cfg["schedule"] = "rows"
+            rc = lib.qsx_hv_ctx_set_schedule(self.h, 0)
+        L.check(rc, "qsx_hv_ctx_set_schedule")
 
     def __del__(self):
         try:

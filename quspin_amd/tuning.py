@@ -5,7 +5,9 @@ For a matrix-free operator the tuner chooses the representative lookup
 table in HBM) and the schedule ("rows": one lane per row; "rows2/4/8": that
 many lanes per row with shuffle reduction; "queue": compacted LDS work queue;
 "batch4/8": terms in batches with all lookups of a batch in flight, for
-rank-based lookups; all but "rows" for one vector only).  Candidates are timed on the operator's own
+rank-based lookups; "split": blocks of the split-chain layout of one
+hard-core-boson sector without symmetry maps; all but "rows" for one vector
+only).  Candidates are timed on the operator's own
 basis with HIP events (a few launches each) and the winner is cached per
 (basis signature, dtype, nvec) in memory and in a JSON file
 (``QSX_TUNE_CACHE``, default ``~/.cache/quspin_amd/tune.json``).
@@ -23,7 +25,7 @@ from . import _lib as L
 _mem = {}
 
 # schedule name -> launch code of qsx_hv_ctx_set_schedule
-SCHEDULES = {"rows": 0, "queue": 1, "rows2": 2, "rows4": 3, "rows8": 4, "batch4": 5, "batch8": 6}
+SCHEDULES = {"rows": 0, "queue": 1, "rows2": 2, "rows4": 3, "rows8": 4, "batch4": 5, "batch8": 6, "split": 7}
 
 
 def _cache_path():
@@ -88,8 +90,20 @@ def candidates(op, vdt, nvec):
         scheds = ["rows", "rows2", "rows4", "queue"]
         if lk == "table" or _rank_lookup(op):
             scheds += ["batch4", "batch8"]
+        if lk == "search" and _split_layout(op):
+            scheds.append("split")
         out += [{"lookup": lk, "schedule": sc} for sc in scheds]
     return out
+
+
+def _split_layout(op):
+    """One hard-core-boson particle sector without symmetry maps on at most
+    32 bits (the library also needs every term in the bit-parallel form and
+    falls back to "rows" otherwise)."""
+    from .basis.base_general import hcb_basis_general
+    b = op.basis
+    return (isinstance(b, hcb_basis_general) and not len(b._pers) and b.N <= 32 and b._Np is not None
+            and not isinstance(b._Np, (list, tuple)))
 
 
 def _rank_lookup(op):
@@ -134,7 +148,7 @@ def time_candidates(op, ctx, vdt, nvec, reps=5):
 
     def timed(cfg, n):
         ctx.configure(cfg, op.basis)
-        if ctx.cfg["lookup"] != cfg["lookup"]:
+        if ctx.cfg["lookup"] != cfg["lookup"] or ctx.cfg["schedule"] != cfg["schedule"]:
             return None
         for _ in range(2):
             lib.qsx_hv_ctx_apply(ctx.h, L.ptr(x), L.ptr(y), 1.0, 0.0, 1.0, 0.0, 0, L.stream_handle())
