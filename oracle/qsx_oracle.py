@@ -25,6 +25,12 @@ Pins (checked in ``tests/test_oracle_*.py``):
 * ``op_shift_sector``/``get_amp`` — built on ``projector`` (pinned above);
   P^H P = 1 and the shifted product of a symmetric operator equals the
   reduced operator (tests/test_oracle_pins.py).
+* ``ftlm_static``/``ltlm_static`` — with Krylov data spanning the whole space
+  they equal <r|e^{-bH} O|r> and <r|e^{-bH/2} O e^{-bH/2}|r> for a
+  non-Hermitian O (dense ``expm``), and <r|O|r> at b = 0 for any Krylov size
+  (tests/test_oracle_pins.py).  LTLM's truncation of Krylov vectors whose
+  weight falls below machine epsilon restates _LTLM.py:95-101 and is pinned
+  only where it removes nothing.
 """
 
 import itertools
@@ -40,6 +46,7 @@ __all__ = [
     "op_coo", "make_csr", "inplace_op", "index_of",
     "expm_multiply", "fragment_3_1_partition", "lanczos_full",
     "csr_trace", "csr_1_norm", "mlp_forward", "mlp_backward", "op_shift_sector", "get_amp",
+    "ftlm_static", "ltlm_static",
 ]
 
 
@@ -575,6 +582,56 @@ def lanczos_full(matvec, v0, m, eps=None):
             break
     E, V = eigh_tridiagonal(a[:mm], b[:mm - 1])
     return E, V, Q[:mm], a[:mm], b[:mm]
+
+
+# ----------------------------------------------------------------------------
+# FTLM / LTLM (src/quspin/tools/lanczos/_FTLM.py:10-105, _LTLM.py:9-139)
+# ----------------------------------------------------------------------------
+def ftlm_static(O_mats, E, V, Q, beta=0):
+    """One FTLM sample (_FTLM.py:76-103): with w_j(b) = exp(-b E_j) and
+    c_i(b) = sum_j V[i,j] V[0,j] w_j(b), <O>_r(b) = sum_i c_i(b) <q_i|O q_0>.
+    ``O_mats`` maps names to dense or scipy matrices; ``Q`` holds the Krylov
+    vectors as rows.  Returns ({name: <O>_r}, c_0)."""
+    beta = np.atleast_1d(np.asarray(beta, dtype=np.float64))
+    E = np.asarray(E)
+    V = np.asarray(V)
+    Q = np.asarray(Q)
+    w = np.exp(-E[:, None] * beta[None, :])
+    c = (V * V[0][None, :]) @ w
+    out = {}
+    for name, O in O_mats.items():
+        Or = O @ Q[0]
+        ov = np.conj(Q) @ Or
+        out[name] = np.squeeze((c * ov[:, None]).sum(axis=0))
+    return out, np.squeeze(c[0])
+
+
+def ltlm_static(O_mats, E, V, Q, beta=0):
+    """One LTLM sample (_LTLM.py:86-137): keep the first nv Krylov vectors,
+    nv = first j with exp(-min(b) E_j) max|V[0]| < eps (all when none);
+    M[i,j] = <q_j|O q_i>, A = V^T M V on the kept block, c_j = V[0,j]
+    exp(-b E_j / 2), <O>_r = sum_{jl} c_j c_l A_{jl}; also returns
+    <I>_r = sum_j V[0,j]^2 exp(-b E_j)."""
+    beta = np.atleast_1d(np.asarray(beta, dtype=np.float64))
+    E = np.asarray(E)
+    V = np.asarray(V)
+    Q = np.asarray(Q)
+    nv = E.size
+    weight = np.exp(-E * beta.min()) * np.abs(V[0]).max()
+    small = weight < np.finfo(weight.dtype).eps
+    if small.any():
+        nv = int(np.nonzero(small)[0][0])
+    Qk = Q[:nv]
+    Vk = V[:nv, :nv]
+    c = Vk[0][:, None] * np.exp(-E[:nv, None] * beta[None, :] / 2.0)
+    out = {}
+    for name, O in O_mats.items():
+        OQ = np.asarray(O @ Qk.T)
+        M = (np.conj(Qk) @ OQ).T
+        A = Vk.T @ M @ Vk
+        out[name] = np.squeeze(np.einsum("jb,lb,jl->b", c, c, A))
+    Id = (Vk[0] ** 2) @ np.exp(-E[:nv, None] * beta[None, :])
+    return out, np.squeeze(Id)
 
 
 # ----------------------------------------------------------------------------

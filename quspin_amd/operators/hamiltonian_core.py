@@ -316,6 +316,55 @@ class hamiltonian:
             return out
         return res
 
+    # ------------------------------------------------------------ evolution
+    def evolve(self, v0, t0, times, eom="SE", solver_name="dop853", stack_state=False, verbose=False,
+               iterate=False, imag_time=False, **solver_args):
+        """Solve the Schroedinger (``eom="SE"``, real or imaginary time) or
+        Liouville-von Neumann (``"LvNE"``) equation with scipy's ``ode``
+        integrators (hamiltonian_core.py:1494-1685); every right-hand side
+        evaluation applies H(t) on the device (``_DeviceEOM``)."""
+        from ..tools.evolution import evolve
+        if L.is_device_tensor(v0):
+            v0 = L.to_host(v0)
+        v0 = np.asanyarray(v0)
+        if np.iscomplexobj(times):
+            raise ValueError("times must be real number(s).")
+        kw = dict(solver_args, solver_name=solver_name, stack_state=stack_state, verbose=verbose,
+                  iterate=iterate, imag_time=imag_time)
+        if eom == "SE":
+            if v0.ndim > 2:
+                raise ValueError("v0 must have ndim <= 2")
+            if v0.shape[0] != self.Ns:
+                raise ValueError("v0 must have {0} elements".format(self.Ns))
+            if imag_time:
+                if stack_state:
+                    raise NotImplementedError("stack state is not compatible with imaginary time evolution.")
+                dt = np.result_type(v0.dtype, self._dtype, np.float64)
+                f = _DeviceEOM(self, "ISO", v0.shape, dt)
+                kw["real"] = dt.kind != "c"
+            elif stack_state:
+                if self._dtype.kind == "c":
+                    raise ValueError("stack_state option cannot be used with complex-valued Hamiltonians")
+                f = _DeviceEOM(self, "SO_real", (2 * v0.shape[0],) + v0.shape[1:], np.float64)
+                kw["real"] = False
+            else:
+                f = _DeviceEOM(self, "SO", v0.shape, np.complex128)
+                kw["real"] = False
+        elif eom == "LvNE":
+            if v0.ndim != 2:
+                raise ValueError("v0 must have ndim = 2")
+            if v0.shape != self._shape:
+                raise ValueError("v0 must be same shape as Hamiltonian")
+            if imag_time:
+                raise NotImplementedError("imaginary time not implemented for Liouville-von Neumann dynamics")
+            if stack_state:
+                raise NotImplementedError("stack_state not implemented for Liouville-von Neumann dynamics")
+            f = _DeviceEOM(self, "LvNE", v0.shape, np.complex128)
+            kw["real"] = False
+        else:
+            raise ValueError("'{} equation' not recognized, must be 'SE' or 'LvNE'".format(eom))
+        return evolve(v0, t0, times, f, **kw)
+
     def rdot(self, V, time=0, check=True, out=None, overwrite_out=True, a=1.0):
         return self.transpose().dot(V.T if not L.is_device_tensor(V) else V.T, time=time, check=check, a=a).T
 
@@ -358,3 +407,54 @@ class hamiltonian:
 
     def __repr__(self):
         return "<{0}x{1} qspin dtype:{2}>".format(*(self._shape + (self._dtype,)))
+
+
+class _DeviceEOM:
+    """Right-hand sides of the equations of motion (hamiltonian_core.py:1390-1492)
+    as scipy ``ode`` callables: the state arrives as a host array, H(t) acts
+    on it on the device through the operator's CSR kernels (static part with
+    overwrite, each drive term f(t)*H_f accumulated), and the derivative
+    returns as a host array.
+
+    kind "SO":      -i H(t) V                         (complex V)
+    kind "ISO":     -H(t) V                           (imaginary time)
+    kind "SO_real": [H(t) v; -H(t) u] for V = [u; v]  (real H, stacked state)
+    kind "LvNE":    -i (H(t) rho - rho H(t)), rho H = (H^T rho^T)^T
+    """
+
+    def __init__(self, H, kind, shape, dtype):
+        self.H = H
+        self.kind = kind
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.x = L.empty(self.shape, self.dtype)
+        self.y = L.empty(self.shape, self.dtype)
+        if kind == "LvNE":
+            self.HT = H.transpose()
+            self.yt = L.empty(self.shape, self.dtype)
+
+    @staticmethod
+    def _apply(Hop, x, y, time, a, beta):
+        Hop._static.apply(x, y, a=a, beta=beta)
+        for f, M in Hop._dynamic.items():
+            M.apply(x, y, a=a * f(time), beta=1)
+
+    def __call__(self, time, V):
+        torch = L.torch()
+        self.x.copy_(torch.from_numpy(np.ascontiguousarray(np.asarray(V).reshape(self.shape), dtype=self.dtype)))
+        x, y = self.x, self.y
+        if self.kind == "SO":
+            self._apply(self.H, x, y, time, -1j, 0)
+        elif self.kind == "ISO":
+            self._apply(self.H, x, y, time, -1.0, 0)
+        elif self.kind == "SO_real":
+            Ns = self.shape[0] // 2
+            self._apply(self.H, x[Ns:], y[:Ns], time, 1.0, 0)
+            self._apply(self.H, x[:Ns], y[Ns:], time, -1.0, 0)
+        else:
+            self._apply(self.H, x, y, time, 1.0, 0)
+            xt = x.transpose(0, 1).contiguous()
+            self._apply(self.HT, xt, self.yt, time, 1.0, 0)
+            y.sub_(self.yt.transpose(0, 1))
+            y.mul_(-1j)
+        return L.to_host(y).ravel()

@@ -15,8 +15,10 @@ from scipy.linalg import eigh_tridiagonal
 from .. import _lib as L
 from .._lib import ptr, stream_handle
 from ..operators._devop import as_device_operator, nrm2_dev
+from ._ftlm import FTLM_static_iteration, LTLM_static_iteration  # noqa: F401
 
-__all__ = ["lanczos_full", "lanczos_iter", "lin_comb_Q_T", "expm_lanczos"]
+__all__ = ["lanczos_full", "lanczos_iter", "lin_comb_Q_T", "expm_lanczos", "FTLM_static_iteration",
+           "LTLM_static_iteration"]
 
 
 def _dtype_of(A, v0):

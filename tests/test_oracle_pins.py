@@ -292,3 +292,62 @@ def test_shift_sector_and_get_amp_identities():
         assert np.abs(psi_full[~ok]).max(initial=0) < 1e-12
         for c in range(len(b)):
             assert abs(np.sum(np.abs(C[cols == c]) ** 2) - 1.0) < 1e-12
+
+
+def _full_krylov_data(n, seed):
+    """Real symmetric H, real start vector r and an orthonormal basis Q (rows)
+    of the whole space with q_0 = r; T = Q H Q^T and its eigenpairs."""
+    rng = np.random.default_rng(seed)
+    X = rng.normal(size=(n, n))
+    H = (X + X.T) / 2
+    r = rng.normal(size=n)
+    r /= np.linalg.norm(r)
+    B, _ = np.linalg.qr(np.column_stack([r, rng.normal(size=(n, n - 1))]))
+    B[:, 0] *= np.sign(B[:, 0] @ r)
+    Q = B.T
+    E, V = np.linalg.eigh(Q @ H @ Q.T)
+    return H, r, Q, E, V
+
+
+def test_ftlm_ltlm_closed_forms():
+    """Whole-space Krylov data: FTLM = <r|e^{-bH} O|r>, LTLM = <r|e^{-bH/2} O
+    e^{-bH/2}|r>, c_0 = <I>_r = <r|e^{-bH}|r>, with a non-Hermitian complex O
+    so that a transposed or conjugated overlap fails."""
+    from scipy.linalg import expm
+    n = 9
+    H, r, Q, E, V = _full_krylov_data(n, 3)
+    rng = np.random.default_rng(4)
+    Op = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    betas = np.array([0.0, 0.4, 1.3])
+    f_res, c0 = O.ftlm_static({"O": Op, "Os": sp.csr_matrix(Op)}, E, V, Q, beta=betas)
+    l_res, Id = O.ltlm_static({"O": Op}, E, V, Q, beta=betas)
+    for k, b in enumerate(betas):
+        eb = expm(-b * H)
+        eh = expm(-b * H / 2)
+        np.testing.assert_allclose(f_res["O"][k], r @ eb @ Op @ r, rtol=1e-11, atol=1e-12)
+        np.testing.assert_allclose(f_res["Os"][k], r @ eb @ Op @ r, rtol=1e-11, atol=1e-12)
+        np.testing.assert_allclose(l_res["O"][k], r @ eh @ Op @ eh @ r, rtol=1e-11, atol=1e-12)
+        np.testing.assert_allclose(c0[k], r @ eb @ r, rtol=1e-12)
+        np.testing.assert_allclose(Id[k], r @ eb @ r, rtol=1e-12)
+    # the two differ for a non-commuting O, so neither formula passes for the other
+    assert abs(f_res["O"][2] - l_res["O"][2]) > 1e-3
+
+
+def test_ftlm_ltlm_beta_zero_any_krylov_size():
+    """At b = 0 both reduce to <r|O|r> whatever the Krylov size (V is
+    orthogonal); complex Hermitian H and complex r, so a missing conjugate fails."""
+    n, m = 40, 12
+    rng = np.random.default_rng(5)
+    X = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    H = (X + X.conj().T) / 2
+    r = rng.normal(size=n) + 1j * rng.normal(size=n)
+    r /= np.linalg.norm(r)
+    Op = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    E, V, Q, _, _ = O.lanczos_full(lambda v: H @ v, r, m)
+    f_res, c0 = O.ftlm_static({"O": Op}, E, V, Q, beta=0.0)
+    l_res, Id = O.ltlm_static({"O": Op}, E, V, Q, beta=0.0)
+    want = np.vdot(r, Op @ r)
+    np.testing.assert_allclose(f_res["O"], want, rtol=1e-10)
+    np.testing.assert_allclose(l_res["O"], want, rtol=1e-10)
+    np.testing.assert_allclose(c0, 1.0, rtol=1e-12)
+    np.testing.assert_allclose(Id, 1.0, rtol=1e-12)
