@@ -678,14 +678,17 @@ __device__ __forceinline__ uint32_t table_n(const void* p, int nb, int64_t i) {
   return nb == 1 ? ((const uint8_t*)p)[i] : nb == 2 ? ((const uint16_t*)p)[i] : ((const uint32_t*)p)[i];
 }
 
-// ncls > 0: the code field holds code + G * class(n_c) (an id into the
-// coefficient table chi(code) * sqrt(n_c)); ncls = 0: the group element code.
+// ncls > 0: the code field holds chi_cls[code] + nchi * class(n_c), an id
+// into the coefficient table chi * sqrt(n_c) whose rows are the distinct
+// characters (chi_cls maps each group element to its character's row);
+// ncls = 0: the group element code.
 template <typename I, int M, bool FERM>
 __global__ void __launch_bounds__(256) k_table(const CoreDev* __restrict__ C, const RankDev* __restrict__ R,
                                                const I* __restrict__ basis, int64_t Ns, const int64_t* __restrict__ begin,
                                                const int64_t* __restrict__ end, int N_p, int64_t total, int64_t chunk,
                                                void* __restrict__ table, int u64, int idx_bits, const void* nrm,
-                                               int n_bytes, const uint32_t* __restrict__ nvals, int ncls) {
+                                               int n_bytes, const uint32_t* __restrict__ nvals, int ncls,
+                                               const int32_t* __restrict__ chi_cls, int nchi) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t a = t * chunk;
   if (a >= total) return;
@@ -706,7 +709,7 @@ __global__ void __launch_bounds__(256) k_table(const CoreDev* __restrict__ C, co
       const uint32_t n = table_n(nrm, n_bytes, idx);
       int k = 0;
       while (k + 1 < ncls && nvals[k] != n) ++k;
-      code += (uint32_t)C->G * (uint32_t)k;
+      code = (uint32_t)chi_cls[O.code] + (uint32_t)nchi * (uint32_t)k;
     }
     if (u64) {
       uint64_t e = ~0ull;
@@ -744,6 +747,23 @@ static int table_classes(const void* nrm, int n_bytes, int64_t Ns, std::vector<u
   return QSX_OK;
 }
 
+// Distinct characters chi(g) of the group (bitwise distinct, in order of
+// first appearance) and the row of each group element among them.  Elements
+// with equal characters give equal matrix elements, so the coefficient table
+// needs one row per distinct character, not per element: for k = 0 with
+// +1 parity and inversion eigenvalues that is a single row.
+static void chi_classes(const qsx_core* c, std::vector<int32_t>& cls, std::vector<double2>& vals) {
+  cls.assign(c->h.G, 0);
+  vals.clear();
+  for (int g = 0; g < c->h.G; ++g) {
+    const double2 p = c->phase_h[g];
+    int k = 0;
+    while (k < (int)vals.size() && std::memcmp(&vals[k], &p, sizeof(double2)) != 0) ++k;
+    if (k == (int)vals.size()) vals.push_back(p);
+    cls[g] = k;
+  }
+}
+
 // Size and layout of the lookup table for the sector of the last make_basis:
 // entries, bytes per entry (4 or 8), index bits of the 4-byte layout, and the
 // length of the coefficient table (0: entries hold the group element; the
@@ -767,7 +787,10 @@ extern "C" int qsx_table_info(const qsx_core* c, int64_t Ns, const void* nrm, in
   bool fits = false;
   const int rc = table_classes(nrm, n_bytes, Ns, vals, fits);
   if (rc) return rc;
-  const int64_t nl = fits ? (int64_t)c->h.G * (int64_t)vals.size() : 0;
+  std::vector<int32_t> gcls;
+  std::vector<double2> chis;
+  chi_classes(c, gcls, chis);
+  const int64_t nl = fits ? (int64_t)chis.size() * (int64_t)vals.size() : 0;
   int lb = 0;
   while (nl > 0 && ((int64_t)1 << lb) < nl) ++lb;
   *entries = c->sector_total;
@@ -790,8 +813,9 @@ extern "C" int qsx_table_info(const qsx_core* c, int64_t Ns, const void* nrm, in
 }
 
 // Fills table (entries as described by qsx_table_info) and, when lut_len > 0,
-// the coefficient table lut[code + G * class] = chi(code) * sqrt(n_class)
-// (lut_len double2 of device memory).  Synchronous.
+// the coefficient table lut[r + nchi * class] = chi_r * sqrt(n_class) over
+// the distinct characters chi_r (lut_len double2 of device memory).
+// Synchronous.
 extern "C" int qsx_table_build(qsx_core* c, const void* basis, int64_t Ns, const int64_t* begin, const int64_t* end,
                                int N_p, const void* nrm, int n_bytes, void* table, int entry_bytes, int idx_bits,
                                void* lut, int64_t lut_len, void* stream) {
@@ -802,12 +826,17 @@ extern "C" int qsx_table_build(qsx_core* c, const void* basis, int64_t Ns, const
   hipStream_t st = (hipStream_t)stream;
   std::vector<uint32_t> vals;
   uint32_t* nv_d = nullptr;
-  int ncls = 0;
+  int32_t* cls_d = nullptr;
+  int ncls = 0, nchi = 0;
   if (lut_len > 0) {
     bool fits = false;
     const int rc = table_classes(nrm, n_bytes, Ns, vals, fits);
     if (rc) return rc;
-    if (!fits || (int64_t)c->h.G * (int64_t)vals.size() != lut_len) {
+    std::vector<int32_t> gcls;
+    std::vector<double2> chis;
+    chi_classes(c, gcls, chis);
+    nchi = (int)chis.size();
+    if (!fits || (int64_t)nchi * (int64_t)vals.size() != lut_len) {
       qsx_set_error("qsx_table_build: coefficient table length does not match the basis");
       return QSX_ERR_ARG;
     }
@@ -815,14 +844,19 @@ extern "C" int qsx_table_build(qsx_core* c, const void* basis, int64_t Ns, const
     std::vector<double2> lh((size_t)lut_len);
     for (int k = 0; k < ncls; ++k) {
       const double r = std::sqrt((double)vals[k]);
-      for (int g = 0; g < c->h.G; ++g) {
-        const double2 p = c->phase_h[g];
-        lh[(size_t)g + (size_t)c->h.G * k] = make_double2(r * p.x, r * p.y);
-      }
+      for (int j = 0; j < nchi; ++j)
+        lh[(size_t)j + (size_t)nchi * k] = make_double2(r * chis[j].x, r * chis[j].y);
     }
     QSX_CHECK_HIP(hipMemcpy(lut, lh.data(), sizeof(double2) * lut_len, hipMemcpyHostToDevice));
     QSX_CHECK_HIP(hipMalloc(&nv_d, sizeof(uint32_t) * ncls));
     QSX_CHECK_HIP(hipMemcpy(nv_d, vals.data(), sizeof(uint32_t) * ncls, hipMemcpyHostToDevice));
+    const hipError_t ec = hipMalloc(&cls_d, sizeof(int32_t) * gcls.size());
+    if (ec != hipSuccess) {
+      (void)hipFree(nv_d);
+      qsx_set_error(std::string("hipMalloc: ") + hipGetErrorString(ec));
+      return QSX_ERR_HIP;
+    }
+    QSX_CHECK_HIP(hipMemcpy(cls_d, gcls.data(), sizeof(int32_t) * gcls.size(), hipMemcpyHostToDevice));
   }
   const int64_t total = c->sector_total;
   const int64_t chunk = 16;
@@ -831,16 +865,17 @@ extern "C" int qsx_table_build(qsx_core* c, const void* basis, int64_t Ns, const
   const int u64 = entry_bytes == 8;
   const bool ferm = c->h.fermion != 0;
   if (c->h.N <= 32) {
-    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls)); }
-    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls)); }
+    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls, cls_d, nchi)); }
+    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint32_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint32_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls, cls_d, nchi)); }
   } else {
-    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls)); }
-    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls)); }
+    if (ferm) { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, true>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls, cls_d, nchi)); }
+    else { QSX_M_SWITCH(c->h.nmaps, hipLaunchKernelGGL((k_table<uint64_t, MM, false>), grid, blk, 0, st, c->d, c->rank_d, (const uint64_t*)basis, Ns, begin, end, N_p, total, chunk, table, u64, idx_bits, nrm, n_bytes, nv_d, ncls, cls_d, nchi)); }
   }
   const hipError_t le = hipGetLastError();
   if (nv_d) {
     (void)hipStreamSynchronize(st);
     (void)hipFree(nv_d);
+    (void)hipFree(cls_d);
   }
   if (le != hipSuccess) {
     qsx_set_error(std::string("kernel launch: ") + hipGetErrorString(le));
