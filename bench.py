@@ -4,14 +4,20 @@ Workload (N=1): spin-1/2 Heisenberg chain L=28, periodic, Sz=0 + momentum
 k=0 + parity + spin inversion (361,270 reduced states), float64, seeded
 Gaussian vector; one step = one fused H.v launch (quantum_LinearOperator.dot).
 Also timed in the same run: 200-step Lanczos on the same operator (a warm
-call; the first call, which captures the recurrence, is reported beside it).
+call; the first call, which captures the recurrence, is reported beside it),
+the same chain at L=34 (17,179,435 states, 64-bit state words: the
+HBM-bound size), and the expm workloads (precision, batched sectors,
+learned step).
 
 Prints ONE JSON line.  ``roofline`` uses the algorithmic bytes of one launch
-B = Ns*(4 + 1) + 2*Ns*8 (basis word, normalisation byte, v_in, v_out; the
-off-diagonal gathers of v_in count as cache hits) over the average launch
-time measured with HIP events on the launch stream.  ``cpu_baseline`` is
-the C/OpenMP restatement in oracle/ (fused all-terms gather) timed on the
-host cores on a bounded number of matvecs.
+B = Ns*(s_i + s_n) + 2*Ns*s_v (state word, normalisation byte, v_in, v_out:
+21 B per state at L=28; the off-diagonal gathers of v_in count as cache
+hits) over the average launch time measured with HIP events on the launch
+stream; the figure with the folded diagonal (8 B per state more) is given
+beside it.  ``cpu_baseline`` is the C/OpenMP restatement in oracle/ running
+the same algorithm (fused rows through a sector lookup table) on the host
+cores, timed on a bounded number of matvecs; the reference's own per-element
+orbit search is timed beside it.
 """
 
 import argparse
@@ -30,11 +36,18 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.0, write
 # x1.0); keyed by (L, lookup, schedule, dtype).  Source and commands:
-# profiles/k_hv_rows_heisenberg_L28_kpz.md
-# For this operator the "rows", "batch4" and "batch8" schedules all launch
-# k_hv_rows_fast_batch (batches of 7, 4 and 8 terms).
+# profiles/k_hv_rows_heisenberg_L28_kpz.md, profiles/k_hv_rows_heisenberg_L34_kpz.md
+# For these operators the "rows", "batch4" and "batch8" schedules all launch
+# k_hv_rows_fast_batch (batches of 7 or 6, 4 and 8 terms).
 MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 88.4e6, (28, "table", "batch4", "float64"): 87.0e6,
-                    (28, "table", "batch8", "float64"): 89.1e6}
+                    (28, "table", "batch8", "float64"): 89.1e6, (34, "table", "rows", "float64"): 9.24e9,
+                    ("csr", 34, "float64"): 14.68e9}
+# The same for the precision workload's term kernels (full launches, per
+# launch): profiles/k_csr_spmv_expm_hubbard_L14_xxz_L28.md
+C3_TRAFFIC = {("mf", "complex64"): 3.76e9, ("mf", "complex128"): 7.44e9}
+# ... and of one batched H.V of the 26-sector workload (CSR form, all sectors):
+# profiles/k_csr_wave_batched_heisenberg_L26_64vec.md
+C4_TRAFFIC = {("csr", "fp32"): 70.1e9, ("csr", "fp64"): 161.6e9}
 
 
 def chain_static(L, J=1.0, Delta=1.0):
@@ -55,6 +68,8 @@ def main():
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run the plain copy kernel on this many MiB (HBM counter calibration)")
     ap.add_argument("--no-expm", action="store_true", help="skip the expm workloads (precision, batched, learned)")
+    ap.add_argument("--no-scaled", action="store_true", help="skip the L=34 form of the headline workload")
+    ap.add_argument("--scaled-steps", type=int, default=20)
     ap.add_argument("--expm-only", default="", help="comma list of c3,c4,c5 to run (default all)")
     ap.add_argument("--c3-steps", type=int, default=100)
     ap.add_argument("--c5-steps", type=int, default=500)
@@ -114,9 +129,12 @@ def main():
     ms_per = ms_total / args.steps
     value = 1000.0 / ms_per
 
-    # per reduced state: 4 B state word, 1 B n, 8 B folded diagonal, 8 B x read, 8 B y written
-    nbytes = Ns * (4 + 1 + 8) + 2 * Ns * 8
+    # per reduced state: state word (4 B up to 32 sites, else 8 B), 1 B n, 8 B x read, 8 B y written;
+    # the kernel also reads the folded diagonal (8 B), reported beside it
+    s_i = 4 if L <= 32 else 8
+    nbytes = Ns * (s_i + 1) + 2 * Ns * 8
     achieved = nbytes / (ms_per * 1e-3) / 1e9
+    achieved_fd = (nbytes + 8 * Ns) / (ms_per * 1e-3) / 1e9
 
     # 200-step Lanczos on the same operator (device recurrence)
     lz = {}
@@ -137,40 +155,56 @@ def main():
         del Qm
 
     cpu = None
-    if not args.no_cpu:
+    if not args.no_cpu and L <= 30:
         from oracle import c_oracle
         g = c_oracle.Group(L, [P, Z, T], [2, 2, L], [0, 0, 0])
-        tc0 = time.perf_counter()
         cb, cn = g.make_basis(L // 2)
-        t_cb = time.perf_counter() - tc0
         if not np.array_equal(cb, B.states.astype(np.uint64)):
             raise SystemExit("CPU restatement basis differs from the device basis")
+        tc0 = time.perf_counter()
+        ctab = c_oracle.table_build(g, L // 2, cb)
+        t_tab = time.perf_counter() - tc0
         terms = [(op_, tuple(b[1:]), b[0]) for op_, bonds in static for b in bonds]
         vh = v.cpu().numpy()
-        ref = c_oracle.hv_real(g, cb, cn, terms, vh, fused=True)  # warm-up + parity
+        ref = c_oracle.hv_table_real(g, cb, cn, ctab, terms, vh)  # warm-up + parity
         err = float(np.abs(ref - out.cpu().numpy()).max())
-        nrep, tsum = 0, 0.0
-        while tsum < args.cpu_seconds and nrep < 1000:
-            tc = time.perf_counter()
-            c_oracle.hv_real(g, cb, cn, terms, vh, fused=True)
-            tsum += time.perf_counter() - tc
-            nrep += 1
+
+        def timed_calls(fn, budget, cap):
+            nrep, tsum = 0, 0.0
+            while tsum < budget and nrep < cap:
+                tc = time.perf_counter()
+                fn()
+                tsum += time.perf_counter() - tc
+                nrep += 1
+            return nrep, tsum
+
+        nrep, tsum = timed_calls(lambda: c_oracle.hv_table_real(g, cb, cn, ctab, terms, vh), args.cpu_seconds, 2000)
+        nref, tref = timed_calls(lambda: c_oracle.hv_real(g, cb, cn, terms, vh, fused=True), args.cpu_seconds / 3, 50)
         cpu = {"value": round(nrep / tsum, 3), "unit": "matvecs/s", "cores": c_oracle.max_threads(),
                "kind": "port",
-               "sample": f"{nrep} fused H.v calls on the same L={L} k=0,p=0,z=0 sector ({Ns} states), "
-                         f"C/OpenMP restatement (oracle/csrc/qsx_oracle.c); max|GPU-CPU| = {err:.2e}"}
+               "sample": f"{nrep} fused H.v calls on the same L={L} k=0,p=+1,z=+1 sector ({Ns} states), "
+                         f"C/OpenMP restatement of the same algorithm (rows through a sector lookup table, "
+                         f"oracle/csrc/qsx_oracle.c qo_hv_table_real; table built once in {t_tab:.1f} s, not timed); "
+                         f"max|GPU-CPU| = {err:.2e}",
+               "reference_algorithm": {"value": round(nref / tref, 3), "unit": "matvecs/s",
+                                       "sample": f"{nref} calls of the per-element orbit search + binary search "
+                                                 "form (the reference's loop, fused over terms; qo_hv_real)"}}
 
     cfg = next(iter(op.config.values()), {})
     traffic = MEASURED_TRAFFIC.get((L, cfg.get("lookup"), cfg.get("schedule"), "float64"))
+    scaled = None
+    if not args.no_scaled and L == 28:
+        scaled = run_c2_scaled(args.scaled_steps)
     expm = {}
     if not args.no_expm:
         which = set(args.expm_only.split(",")) if args.expm_only else {"c3", "c4", "c5"}
         if "c3" in which:
-            expm["precision_L28"] = run_precision(args.c3_steps)
+            expm["precision_L28"] = run_precision(args.c3_steps, cpu=not args.no_cpu)
         if "c4" in which:
             expm["batched_L26"] = run_batched()
         if "c5" in which:
-            expm["learned_hubbard_L14"] = run_learned(args.c5_steps, runs=args.c5_runs, graph=not args.no_graph)
+            expm["learned_hubbard_L14"] = run_learned(args.c5_steps, runs=args.c5_runs, graph=not args.no_graph,
+                                                      cpu=not args.no_cpu)
 
     line = {
         "metric": "H.v matvecs/s (symmetry-reduced matrix-free, fp64)",
@@ -189,13 +223,99 @@ def main():
                                "quantum_LinearOperator.dot on a seeded Gaussian vector",
                    "Ns": Ns, "basis_build_s": round(t_basis, 3), "wall_s": round(t_wall, 4)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic},
+                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
+                     "alg_bytes_per_state": s_i + 1 + 16,
+                     "achieved_with_folded_diag": round(achieved_fd, 2),
+                     "limit": ("cache-resident working set: latency of the L2 misses of the sector-table reads "
+                               "(profiles/k_hv_rows_heisenberg_L28_kpz.md); the HBM-bound size is "
+                               "c2_scaled_L34") if L <= 30 else "sector-table reads from HBM"},
         "cpu_baseline": cpu,
         "lanczos": lz,
+        "c2_scaled_L34": scaled,
         "expm": expm,
         "kernel_config": {str(k): v for k, v in op.config.items()},
     }
     print(json.dumps(line))
+
+
+def run_c2_scaled(steps, L=34):
+    """The headline chain at L=34: Sz=0, k=0, p=+1, z=+1 (17,179,435 states,
+    64-bit state words; 429 MB of algorithmic bytes per product, above the
+    256 MB MALL).  Matrix-free quantum_LinearOperator.dot (sector table,
+    4-byte entries) and hamiltonian.dot on the device CSR of the same
+    operator, each timed over `steps` launches with HIP events; the two
+    results are compared."""
+    import time
+    import torch
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    i = np.arange(L)
+    T, P, Z = (i + 1) % L, i[::-1].copy(), -(i + 1)
+    t0 = time.perf_counter()
+    B = spin_basis_general(L, Nup=L // 2, pauli=0, kblock=(T, 0), pblock=(P, 0), zblock=(Z, 0))
+    torch.cuda.synchronize()
+    t_basis = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    tab = B.lookup_table()
+    torch.cuda.synchronize()
+    t_tab = time.perf_counter() - t0
+    static = chain_static(L)
+    kw = dict(check_symm=False, check_herm=False, check_pcon=False)
+    Q = quantum_LinearOperator(static, basis=B, dtype=np.float64, **kw)
+    Ns = B.Ns
+    gen = torch.Generator(device="cuda").manual_seed(1234)
+    v = torch.randn(Ns, dtype=torch.float64, device="cuda", generator=gen)
+    y = torch.empty_like(v)
+    op = Q._qsx_operator()
+    op.apply(v, y)
+    torch.cuda.synchronize()
+    st, e0, e1 = _ev_timer()
+    e0.record(st)
+    for _ in range(steps):
+        op.apply(v, y)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    alg = Ns * (8 + 1) + 2 * Ns * 8
+    cfg = next(iter(op.config.values()), {})
+    traffic = MEASURED_TRAFFIC.get((L, cfg.get("lookup"), cfg.get("schedule"), "float64"))
+    t0 = time.perf_counter()
+    H = hamiltonian(static, [], basis=B, dtype=np.float64, **kw)
+    torch.cuda.synchronize()
+    t_csr = time.perf_counter() - t0
+    M = H._static
+    y2 = torch.empty_like(v)
+    M.apply(v, y2)
+    torch.cuda.synchronize()
+    e0.record(st)
+    for _ in range(steps):
+        M.apply(v, y2)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms_csr = e0.elapsed_time(e1) / steps
+    alg_csr = 4 * (Ns + 1) + M.nnz * (4 + 8) + 2 * Ns * 8
+    err = float((y - y2).abs().max())
+    out = {"workload": f"Heisenberg chain L={L} PBC, Sz=0, k=0, p=+1, z=+1 ({Ns} states, uint64 words), "
+                       "seeded Gaussian vector",
+           "Ns": Ns, "basis_build_s": round(t_basis, 3), "table_build_s": round(t_tab, 3),
+           "table_bytes": int(tab[0].numel() * tab[0].element_size()),
+           "matrix_free": {"matvecs_per_s": round(1000.0 / ms, 2), "ms_per_step": round(ms, 5),
+                           "kernel_config": cfg,
+                           "roofline": {"bound": "hbm", "achieved": round(alg / ms / 1e6, 2), "peak": HBM_PEAK_GBS,
+                                        "unit": "GB/s", "frac": round(alg / ms / 1e6 / HBM_PEAK_GBS, 5),
+                                        "traffic": traffic, "alg_bytes_per_state": 25,
+                                        "achieved_with_folded_diag": round((alg + 8 * Ns) / ms / 1e6, 2),
+                                        "limit": "L2-miss traffic of the sector-table reads (one 32 B sector per "
+                                                 "applied term; profiles/k_hv_rows_heisenberg_L34_kpz.md)"}},
+           "csr": {"matvecs_per_s": round(1000.0 / ms_csr, 2), "ms_per_step": round(ms_csr, 5), "nnz": int(M.nnz),
+                   "assembly_s": round(t_csr, 3),
+                   "roofline": {"bound": "hbm", "achieved": round(alg_csr / ms_csr / 1e6, 2), "peak": HBM_PEAK_GBS,
+                                "unit": "GB/s", "frac": round(alg_csr / ms_csr / 1e6 / HBM_PEAK_GBS, 5),
+                                "alg_bytes": int(alg_csr), "traffic": MEASURED_TRAFFIC.get(("csr", L, "float64"))}},
+           "max_abs_diff_matrix_free_vs_csr": err}
+    del H, M, Q, op, tab, B, v, y, y2
+    torch.cuda.empty_cache()
+    return out
 
 
 def _ev_timer():
@@ -205,15 +325,41 @@ def _ev_timer():
     return st, e0, e1
 
 
-def run_precision(steps, L=28, dt=0.1):
+def run_precision(steps, L=28, dt=0.1, cpu=True):
     """Both forms of the L=28 precision workload: matrix-free
-    (quantum_LinearOperator) and CSR (hamiltonian)."""
+    (quantum_LinearOperator) and CSR (hamiltonian); with ``cpu`` the CSR form
+    also times one FP64 step of the C/OpenMP Taylor loop on the host."""
     out = {"matrix_free": _run_precision(steps, "mf", L, dt)}
-    out["csr"] = _run_precision(steps, "csr", L, dt)
+    out["csr"] = _run_precision(steps, "csr", L, dt, cpu=cpu)
     return out
 
 
-def _run_precision(steps, form, L=28, dt=0.1):
+def _kernel_roofline(op, x, form, Ns, traffic):
+    """One H.v launch of the Taylor loop's operator on x, timed over 5
+    launches with HIP events, against its algorithmic bytes."""
+    import torch
+    y = torch.empty_like(x)
+    op.apply(x, y)
+    torch.cuda.synchronize()
+    st, e0, e1 = _ev_timer()
+    e0.record(st)
+    for _ in range(5):
+        op.apply(x, y)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 5
+    esz = x.element_size()
+    if form == "mf":
+        alg = Ns * 4 + 2 * Ns * esz
+    else:
+        alg = op.indptr.element_size() * (Ns + 1) + op.nnz * (op.indices.element_size() + op.data.element_size()) \
+            + 2 * Ns * esz
+    gbs = alg / ms / 1e6
+    return {"ms": round(ms, 4), "alg_bytes": int(alg), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+
+
+def _run_precision(steps, form, L=28, dt=0.1, cpu=False):
     """XXZ chain L=28, Delta=1, h_i ~ U[-3, 3] (seed 0), Sz=0 without spatial
     symmetry (40,116,600 states); Neel start; `steps` matrix-free expm steps
     of dt in complex64 (FP64 accumulation inside every kernel) and in
@@ -265,14 +411,51 @@ def _run_precision(steps, form, L=28, dt=0.1):
         szerr.append(float(np.abs(sz(p32) - sz(p64)).max()))
     sp32 = 1000.0 * steps / ms[np.complex64]
     sp64 = 1000.0 * steps / ms[np.complex128]
-    return {"workload": f"XXZ L={L} Delta=1 h_i~U[-3,3] seed 0, Sz=0 ({B.Ns} states), Neel start, {steps} steps dt={dt}",
-            "path": ("matrix-free Taylor loop (quantum_LinearOperator)" if form == "mf" else
-                     "CSR Taylor loop (hamiltonian, real FP32/FP64 values)"),
-            "partition": [int(x) for x in E[np.complex64].partition],
-            "steps_per_s_fp32": round(sp32, 3), "steps_per_s_fp64": round(sp64, 3),
-            "speedup_fp32_vs_fp64": round(sp32 / sp64, 3),
-            "state_err_max": max(errs), "state_err_final": errs[-1], "state_err_every_10": errs[9::10],
-            "sz_err_max": max(szerr), "sz_err_every_10": szerr[9::10]}
+    kern = {}
+    for dt_, name in ((np.complex64, "fp32"), (np.complex128, "fp64")):
+        kern[name] = _kernel_roofline(E[dt_]._op, psi[dt_], form, B.Ns,
+                                      C3_TRAFFIC.get((form, np.dtype(dt_).name)))
+    res = {"workload": f"XXZ L={L} Delta=1 h_i~U[-3,3] seed 0, Sz=0 ({B.Ns} states), Neel start, {steps} steps dt={dt}",
+           "path": ("matrix-free Taylor loop (quantum_LinearOperator)" if form == "mf" else
+                    "CSR Taylor loop (hamiltonian, real FP32/FP64 values)"),
+           "partition": [int(x) for x in E[np.complex64].partition],
+           "steps_per_s_fp32": round(sp32, 3), "steps_per_s_fp64": round(sp64, 3),
+           "speedup_fp32_vs_fp64": round(sp32 / sp64, 3),
+           "state_err_max": max(errs), "state_err_final": errs[-1], "state_err_every_10": errs[9::10],
+           "sz_err_max": max(szerr), "sz_err_every_10": szerr[9::10],
+           "hv_kernel": kern}
+    if cpu and form == "csr":
+        res["cpu_baseline"] = _cpu_taylor_step(E[np.complex128], B.Ns, i0)
+    return res
+
+
+def _cpu_taylor_step(Ed, Ns, i0):
+    """One FP64 step of the C/OpenMP Taylor loop on the host (real CSR
+    values, complex128 state; the same partition, shift and tolerance as the
+    device), from the Neel state; compared with the device's FP64 step."""
+    import time
+    import torch
+    from quspin_amd import _lib
+    from oracle import c_oracle
+    M = Ed._op
+    ip, ix = _lib.to_host(M.indptr), _lib.to_host(M.indices)
+    if ip.dtype != ix.dtype:
+        ip, ix = ip.astype(np.int64), ix.astype(np.int64)
+    R = c_oracle.RealCSR(ip, ix, _lib.to_host(M.data).astype(np.float64))
+    m_star, s = (int(x) for x in Ed.partition)
+    psi0 = np.zeros(Ns, np.complex128)
+    psi0[i0] = 1
+    t0 = time.perf_counter()
+    F = R.expm(psi0, s, m_star, float(Ed._tol), complex(Ed._mu), complex(Ed._a))
+    t_cpu = time.perf_counter() - t0
+    p = torch.zeros(Ns, dtype=torch.complex128, device="cuda")
+    p[i0] = 1
+    G = Ed.dot(p)
+    err = float(np.abs(_lib.to_host(G) - F).max())
+    return {"value": round(1.0 / t_cpu, 4), "unit": "steps/s", "cores": c_oracle.max_threads(), "kind": "port",
+            "sample": f"1 FP64 step (partition m*={m_star}, s={s}) of the C/OpenMP Taylor loop "
+                      "(oracle/csrc/qsx_oracle.c qo_expm_rcsr_c128) on the host copy of the CSR; "
+                      f"max|GPU-CPU| after the step = {err:.2e}"}
 
 
 def run_batched(L=26, nv=64, t=2.0, plain_sectors=2):
@@ -322,11 +505,16 @@ def run_batched(L=26, nv=64, t=2.0, plain_sectors=2):
         mv = e0.elapsed_time(e1) / 5
         esz = np.dtype(dt_).itemsize
         alg = rows * (4 + 1) + 2 * rows * nv * esz
+        traffic = C4_TRAFFIC.get((E.batch.form, name))
         out[name] = {"expm_s": round(ms / 1000.0, 4), "form": E.batch.form,
                      "schedule": [E.batch.chunk_rows, E.batch.xcd_slices],
                      "partitions": sorted({tuple(int(x) for x in p) for p in E.partitions}),
-                     "batched_hv_ms": round(mv, 4), "batched_hv_alg_GBps": round(alg / mv / 1e6, 1),
-                     "batched_hv_hbm_frac": round(alg / mv / 1e6 / HBM_PEAK_GBS, 4)}
+                     "batched_hv_ms": round(mv, 4),
+                     "batched_hv_roofline": {"bound": "hbm", "achieved": round(alg / mv / 1e6, 1),
+                                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                             "frac": round(alg / mv / 1e6 / HBM_PEAK_GBS, 4), "alg_bytes": int(alg),
+                                             "traffic": traffic,
+                                             "traffic_GBps": round(traffic / mv / 1e6, 1) if traffic else None}}
         del E, X, Y
     f32 = F["fp32"].to(torch.complex128)
     f64 = F["fp64"]
@@ -347,12 +535,47 @@ def run_batched(L=26, nv=64, t=2.0, plain_sectors=2):
         tp += e0.elapsed_time(e1) / 1000.0
     plain = tp * L / plain_sectors
     out["plain_fp64_s_est"] = round(plain, 3)
-    out["plain_fp64_sample"] = f"{plain_sectors} of {L} sectors timed, scaled by {L}/{plain_sectors}"
-    out["speedup_fp32_batched_vs_plain_fp64"] = round(plain / out["fp32"]["expm_s"], 2)
+    out["plain_fp64_sample"] = (f"estimate: {plain_sectors} of {L} sectors timed ({tp:.3f} s), scaled by "
+                                f"{L}/{plain_sectors} (the sectors hold 400,023 or 400,024 states each)")
+    out["speedup_fp32_batched_vs_plain_fp64_est"] = round(plain / out["fp32"]["expm_s"], 2)
+    out["speedup_fp32_batched_vs_batched_fp64"] = round(out["fp64"]["expm_s"] / out["fp32"]["expm_s"], 2)
     return out
 
 
-def run_learned(steps, L=14, dt=0.02, runs="", graph=True):
+def _cpu_driven_steps(hop, D, f, dt, i0, nsteps):
+    """`nsteps` FP64 steps of the driven workload with the C/OpenMP Taylor
+    loop on the host, each with the reference's per-step work: the 1-norm of
+    H(t) - mu I (one pass over the CSR of H0 plus f(t) D), the partition by
+    scipy's _fragment_3_1, then the Taylor loop.  Returns the state and the
+    seconds per step."""
+    import time
+    from quspin_amd import _lib
+    from quspin_amd.operators._devop import as_device_operator
+    from quspin_amd.tools.learned_step import exact_partition
+    from oracle import c_oracle
+    M = as_device_operator(hop).to_csr(np.float64)
+    ip, ix = _lib.to_host(M.indptr), _lib.to_host(M.indices)
+    if ip.dtype != ix.dtype:
+        ip, ix = ip.astype(np.int64), ix.astype(np.int64)
+    Dh = _lib.to_host(as_device_operator(D).diagonal(np.float64)).astype(np.float64)
+    R = c_oracle.RealCSR(ip, ix, _lib.to_host(M.data).astype(np.float64), Dh)
+    N = Dh.size
+    rows = np.repeat(np.arange(N), np.diff(ip.astype(np.int64)))
+    trH0 = float(R.data[ix == rows].sum())
+    trD = float(Dh.sum())
+    tol = 2.0 ** -53
+    F = np.zeros(N, np.complex128)
+    F[i0] = 1
+    t0 = time.perf_counter()
+    for k in range(nsteps):
+        t = (k + 0.5) * dt
+        mu = (trH0 + f(t) * trD) / N
+        m_star, s = exact_partition(R.one_norm(f(t), mu), -1j * dt, tol)
+        F = R.expm(F, s, m_star, tol, mu, -1j * dt, dscale=f(t))
+    return F, (time.perf_counter() - t0) / nsteps
+
+
+def run_learned(steps, L=14, dt=0.02, runs="", graph=True, cpu=True):
     """1D spinful Fermi-Hubbard L=14 PBC, t_hop=1, U(t)=4+2cos(3t), (7,7)
     (11,778,624 states): `steps` expm steps of dt in complex64 with the
     partition predicted by the device MLP (trained here on reference
@@ -401,28 +624,35 @@ def run_learned(steps, L=14, dt=0.02, runs="", graph=True):
     res = {}
     # fp32: complex64 matrix, Taylor terms and SpMV with the state accumulated in complex128
     # "csr": H0 streamed as one CSR per Taylor term; "kron": H0 = H_up (x) 1 + 1 (x) H_down, psi read once per term
-    runs = (("learned_fp32", np.complex64, hop32, net, True, "csr"),
-            ("exact_fp32", np.complex64, hop32, None, False, "csr"),
-            ("exact_fp64", np.complex128, hop64, None, False, "csr"),
-            ("learned_fp32_kron", np.complex64, hop32, net, True, "kron"),
-            ("exact_fp32_kron", np.complex64, hop32, None, False, "kron"),
-            ("learned_fp64_kron", np.complex128, hop64, net, True, "kron"),
-            ("exact_fp64_kron", np.complex128, hop64, None, False, "kron"))
+    # exact selection: "host" = per step on the host, each step launched on
+    # its own (the reference's structure); "device" = per block of 100 steps
+    # on the device, the block replayed like the learned blocks (the fair
+    # baseline of the network)
+    runs = (("learned_fp32", np.complex64, hop32, net, True, "csr", "host"),
+            ("exact_dev_fp32", np.complex64, hop32, None, False, "csr", "device"),
+            ("exact_fp32", np.complex64, hop32, None, False, "csr", "host"),
+            ("exact_fp64", np.complex128, hop64, None, False, "csr", "host"),
+            ("exact_dev_fp64", np.complex128, hop64, None, False, "csr", "device"),
+            ("learned_fp32_kron", np.complex64, hop32, net, True, "kron", "host"),
+            ("exact_dev_fp32_kron", np.complex64, hop32, None, False, "kron", "device"),
+            ("learned_fp64_kron", np.complex128, hop64, net, True, "kron", "host"),
+            ("exact_dev_fp64_kron", np.complex128, hop64, None, False, "kron", "device"))
     if runs_sel:
         runs = tuple(r for r in runs if r[0] in runs_sel or r[0] == "exact_fp64")
     psis = {}
-    for name, dt_, hop, pred, learned, form in runs:
+    for name, dt_, hop, pred, learned, form, exact in runs:
         ev = DrivenEvolution(hop, D, f, dt, dtype=dt_, phase=phase, predictor=pred, state_dtype=np.complex128,
                              form=form)
         psi = torch.zeros(B.Ns, dtype=torch.complex128, device="cuda")
         psi[i0] = 1
-        ev.run(psi, 0.0, min(steps, 100), learned=learned, block=100, graph=graph)  # capture / first launches
+        # capture / first launches
+        ev.run(psi, 0.0, min(steps, 100), learned=learned, block=100, graph=graph, exact=exact)
         psi.zero_()
         psi[i0] = 1
         ev.stats = {k: 0 for k in ev.stats}
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        ev.run(psi, 0.0, steps, learned=learned, block=100, graph=graph)
+        ev.run(psi, 0.0, steps, learned=learned, block=100, graph=graph, exact=exact)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
         res[name] = {"steps_per_s": round(steps / wall, 2), "ms_per_step": round(1000.0 * wall / steps, 4),
@@ -443,10 +673,31 @@ def run_learned(steps, L=14, dt=0.02, runs="", graph=True):
         if a in res and b in res:
             return round(res[a]["steps_per_s"] / res[b]["steps_per_s"], 3)
         return None
+    # same operator form, same structure of launches: what FP32 and the network each gain
+    res["speedup_learned_fp32_vs_exact_dev_fp64"] = ratio("learned_fp32", "exact_dev_fp64")
+    res["gain_learned_vs_exact_dev_fp32"] = ratio("learned_fp32", "exact_dev_fp32")
+    res["speedup_exact_dev_fp32_vs_exact_dev_fp64"] = ratio("exact_dev_fp32", "exact_dev_fp64")
+    res["speedup_learned_fp32_kron_vs_exact_dev_fp64_kron"] = ratio("learned_fp32_kron", "exact_dev_fp64_kron")
+    res["gain_learned_vs_exact_dev_fp32_kron"] = ratio("learned_fp32_kron", "exact_dev_fp32_kron")
+    # against the plain FP64 path with the reference's per-step host selection
     res["speedup_learned_fp32_vs_exact_fp64"] = ratio("learned_fp32", "exact_fp64")
     res["gain_learned_vs_exact_fp32"] = ratio("learned_fp32", "exact_fp32")
-    res["speedup_learned_fp32_kron_vs_exact_fp64"] = ratio("learned_fp32_kron", "exact_fp64")
-    res["gain_kron_vs_csr_learned_fp32"] = ratio("learned_fp32_kron", "learned_fp32")
+    res["speedup_learned_fp32_kron_vs_exact_fp64_csr"] = ratio("learned_fp32_kron", "exact_fp64")
+    if cpu:
+        nc = 2
+        F, sec = _cpu_driven_steps(hop64, D, f, dt, i0, nc)
+        ev = DrivenEvolution(hop64, D, f, dt, dtype=np.complex128, phase=phase)
+        p = torch.zeros(B.Ns, dtype=torch.complex128, device="cuda")
+        p[i0] = 1
+        ev.run(p, 0.0, nc, learned=False)
+        from quspin_amd import _lib
+        from oracle import c_oracle
+        err = float(np.abs(_lib.to_host(p) - F).max())
+        res["cpu_baseline"] = {"value": round(1.0 / sec, 4), "unit": "steps/s", "cores": c_oracle.max_threads(),
+                               "kind": "port",
+                               "sample": f"{nc} FP64 steps on the host: 1-norm pass, _fragment_3_1 and the C/OpenMP "
+                                         "Taylor loop on the real CSR of H0 plus f(t) D (oracle/csrc/qsx_oracle.c "
+                                         f"qo_rcsr_1_norm, qo_expm_rcsr_c128); max|GPU-CPU| after them = {err:.2e}"}
     return res
 
 

@@ -251,6 +251,23 @@ int qsx_csr_1_norm(const void* indptr, const void* indices, const void* data, in
 int qsx_csr_1_norm_diag(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
                         int64_t Ns, double shift_re, double shift_im, const double* xdiag, double xs_re, double xs_im,
                         double* colsum, double* out, void* stream);
+/* Column data for the norms above without a pass over A per shift:
+ * colabs[j] = sum_i |A_ij| (Ns device doubles), dg[j] = A_jj (Ns (re, im)
+ * pairs).  The pieces of _wrapper_csr_1_norm (EMP.html:200-202) that do not
+ * depend on the shift. */
+int qsx_csr_col_split(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype, int64_t Ns,
+                      double* colabs, double* dg, void* stream);
+/* The partition selection of ExpmMultiplyParallel (EMP.html:377-385, scipy
+ * _fragment_3_1's theta branch) for a block of time steps of
+ * H(t) = A + f(t) D, without leaving the device: es holds nsteps step
+ * records (a, mu, dscale = f(t) set; m and s written), the norms
+ * ||A + dscale D - mu I||_1 come from the column data (colabs, dg) and D
+ * (Ns doubles or null); theta/mth list the nth theta_m values and their m in
+ * the order scipy visits them, bound is the branch's bound on |a| * norm;
+ * fail[k] = 1 marks a step above it.  nmax: nsteps words of scratch. */
+int qsx_partition_exact(const double* colabs, const double* dg, const double* D, int64_t N, void* es, int nsteps,
+                        const double* theta, const int* mth, int nth, double bound, unsigned long long* nmax,
+                        int* fail, void* stream);
 
 /* Replaces _wrapper_expm_multiply(indptr, indices, data, s, m_star, tol, mu,
  * a, v, work) and _wrapper_expm_multiply_batch (EMP.html:347-372): v <-

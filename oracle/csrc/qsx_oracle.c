@@ -17,6 +17,7 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <limits.h>
 #include <stdlib.h>
 #include <string.h>
 #ifdef _OPENMP
@@ -359,6 +360,225 @@ void qo_expm_c128(int64_t Ns, const int64_t* indptr, const int64_t* indices, con
     for (int j = 0; j < m_star; ++j) {
       const double cr = a_r / (s * (j + 1.0)), ci = a_i / (s * (j + 1.0));
       qo_csr_spmv_c128(Ns, indptr, indices, data_ri, B, B2);
+      double c2 = 0, nf = 0;
+#pragma omp parallel for reduction(max : c2, nf)
+      for (int64_t r = 0; r < Ns; ++r) {
+        const double yr = B2[2 * r] - (mu_r * B[2 * r] - mu_i * B[2 * r + 1]);
+        const double yi = B2[2 * r + 1] - (mu_r * B[2 * r + 1] + mu_i * B[2 * r]);
+        const double br = cr * yr - ci * yi, bi = cr * yi + ci * yr;
+        B2[2 * r] = br;
+        B2[2 * r + 1] = bi;
+        F[2 * r] += br;
+        F[2 * r + 1] += bi;
+        c2 = fmax(c2, hypot(br, bi));
+        nf = fmax(nf, hypot(F[2 * r], F[2 * r + 1]));
+      }
+      double* t = B; B = B2; B2 = t;
+      if (c1 + c2 <= tol * nf) break;
+      c1 = c2;
+    }
+#pragma omp parallel for
+    for (int64_t r = 0; r < Ns; ++r) {
+      const double fr = F[2 * r], fi = F[2 * r + 1];
+      F[2 * r] = eta_r * fr - eta_i * fi;
+      F[2 * r + 1] = eta_r * fi + eta_i * fr;
+    }
+    memcpy(B, F, sizeof(double) * 2 * Ns);
+  }
+}
+
+/* Sector-table form of the fused H.v (the lookup the device kernels use,
+ * restated on the CPU so that the timed CPU path runs the same algorithm):
+ * for every state s of the Np-particle sector, indexed by its colex rank
+ * sum_j C(p_j, j + 1) over the set-bit positions p_0 < p_1 < ..., the table
+ * holds the basis index of rep(s) (-1 when absent), the group element code
+ * reaching it and the number of elements that do (the oracle's cnt).
+ * Built with qo_representative; 8 bytes per entry. */
+typedef struct {
+  int32_t idx;
+  uint16_t code, cnt;
+} qo_tab_entry;
+
+static int64_t qo_binom_i[65][65];
+
+static void qo_binom_init(void) {
+  if (qo_binom_i[1][0]) return;
+  for (int n = 0; n <= 64; ++n)
+    for (int k = 0; k <= 64; ++k) qo_binom_i[n][k] = k > n ? 0 : (k == 0 || k == n ? 1 : qo_binom_i[n - 1][k - 1] + qo_binom_i[n - 1][k]);
+}
+
+static int64_t qo_colex_rank(uint64_t s) {
+  int64_t r = 0;
+  int j = 1;
+  while (s) {
+    const int p = __builtin_ctzll(s);
+    s &= s - 1;
+    r += qo_binom_i[p][j];
+    ++j;
+  }
+  return r;
+}
+
+/* Fills tab (C(N, Np) entries); returns the entry count, -1 on bad sizes. */
+int64_t qo_table_build(const qo_group* g, int Np, const uint64_t* basis, int64_t Ns, void* tab_out) {
+  qo_binom_init();
+  const int N = g->N;
+  if (N > 62 || Ns >= INT32_MAX || g->G > 65535) return -1;
+  qo_tab_entry* tab = (qo_tab_entry*)tab_out;
+  const int64_t total = qo_binom_i[N][Np];
+  const int64_t chunk = 4096;
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t a = 0; a < total; a += chunk) {
+    const int64_t b = a + chunk < total ? a + chunk : total;
+    uint64_t s = Np == 0 ? 0 : unrank(a, Np, N);
+    for (int64_t r = a; r < b; ++r) {
+      int code = 0, cnt = 1;
+      const uint64_t rep = qo_representative(g, s, &code, &cnt);
+      const int64_t c = qo_index(basis, Ns, rep);
+      tab[r].idx = (int32_t)c;
+      tab[r].code = (uint16_t)code;
+      tab[r].cnt = (uint16_t)cnt;
+      if (r + 1 < b && Np > 0) s = next_comb(s);
+    }
+  }
+  return total;
+}
+
+/* Real fused H.v through the sector table: per row and term the image s2,
+ * its rank, the table entry, and J m chi(code) sqrt(cnt / n_r) v[idx]. */
+void qo_hv_table_real(const qo_group* g, const uint64_t* basis, const int32_t* nrm, int64_t Ns, const void* tab_in,
+                      int nterms, const char* ops, const int32_t* lens, const int32_t* sites, const double* J,
+                      const double* vin, double* vout) {
+  qo_binom_init();
+  const qo_tab_entry* tab = (const qo_tab_entry*)tab_in;
+  const int N = g->N;
+  int off[4097];
+  off[0] = 0;
+  for (int t = 0; t < nterms; ++t) off[t + 1] = off[t] + lens[t];
+  char* hops = (char*)malloc(off[nterms] + 1);
+  int* hsites = (int*)malloc(sizeof(int) * (off[nterms] + 1));
+  for (int t = 0; t < nterms; ++t)
+    for (int k = 0; k < lens[t]; ++k) {
+      char c = ops[off[t] + lens[t] - 1 - k];
+      hops[off[t] + k] = c == '+' ? '-' : (c == '-' ? '+' : c);
+      hsites[off[t] + k] = sites[off[t] + lens[t] - 1 - k];
+    }
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (int64_t r = 0; r < Ns; ++r) {
+    const uint64_t s = basis[r];
+    const double rn = 1.0 / sqrt((double)nrm[r]);
+    double acc = 0.0;
+    for (int t = 0; t < nterms; ++t) {
+      uint64_t s2 = s;
+      double m;
+      if (!qo_apply_term(N, hops + off[t], hsites + off[t], lens[t], &s2, &m)) continue;
+      const qo_tab_entry e = tab[qo_colex_rank(s2)];
+      if (e.idx < 0) continue;
+      acc += J[t] * m * g->chr[e.code] * sqrt((double)e.cnt) * rn * vin[e.idx];
+    }
+    vout[r] = acc;
+  }
+  free(hops);
+  free(hsites);
+}
+
+/* Real-valued CSR (int32 or int64 index arrays, idx64 = 1 for int64) plus
+ * an optional diagonal dscale * D, applied to complex128 vectors: the form
+ * in which the reference's OpenMP Taylor loop runs a real Hamiltonian on a
+ * complex state (EMP.html:347-372; scipy _expm_multiply.py:280-303). */
+typedef struct {
+  int64_t Ns;
+  const void* indptr;
+  const void* indices;
+  int idx64;
+  const double* data;
+  const double* D;
+  double dscale;
+} qo_rcsr;
+
+static inline int64_t rc_ip(const qo_rcsr* A, int64_t r) {
+  return A->idx64 ? ((const int64_t*)A->indptr)[r] : (int64_t)((const int32_t*)A->indptr)[r];
+}
+
+static inline int64_t rc_ix(const qo_rcsr* A, int64_t k) {
+  return A->idx64 ? ((const int64_t*)A->indices)[k] : (int64_t)((const int32_t*)A->indices)[k];
+}
+
+static void rcsr_spmv(const qo_rcsr* A, const double* x, double* y) {
+#pragma omp parallel for schedule(static, 4096)
+  for (int64_t r = 0; r < A->Ns; ++r) {
+    double sr = 0, si = 0;
+    const int64_t e = rc_ip(A, r + 1);
+    for (int64_t k = rc_ip(A, r); k < e; ++k) {
+      const int64_t c = rc_ix(A, k);
+      const double a = A->data[k];
+      sr += a * x[2 * c];
+      si += a * x[2 * c + 1];
+    }
+    if (A->D) {
+      const double d = A->dscale * A->D[r];
+      sr += d * x[2 * r];
+      si += d * x[2 * r + 1];
+    }
+    y[2 * r] = sr;
+    y[2 * r + 1] = si;
+  }
+}
+
+/* max_j sum_i |A_ij + delta_ij (dscale D_j - mu)| (the reference's
+ * _wrapper_csr_1_norm of A - mu I, EMP.html:200-202), column sums by atomic
+ * accumulation. colsum: Ns doubles of scratch. */
+double qo_rcsr_1_norm(int64_t Ns, const void* indptr, const void* indices, int idx64, const double* data,
+                      const double* D, double dscale, double mu_r, double mu_i, double* colsum) {
+  qo_rcsr A = {Ns, indptr, indices, idx64, data, D, dscale};
+  double* dg = (double*)calloc(Ns > 0 ? Ns : 1, sizeof(double));
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < Ns; ++r) {
+    colsum[r] = 0.0;
+    dg[r] = D ? dscale * D[r] : 0.0;
+  }
+#pragma omp parallel for schedule(static, 4096)
+  for (int64_t r = 0; r < Ns; ++r) {
+    const int64_t e = rc_ip(&A, r + 1);
+    for (int64_t k = rc_ip(&A, r); k < e; ++k) {
+      const int64_t c = rc_ix(&A, k);
+      if (c == r) {
+        dg[r] += data[k];
+      } else {
+        const double v = fabs(data[k]);
+#pragma omp atomic
+        colsum[c] += v;
+      }
+    }
+  }
+  double best = 0.0;
+#pragma omp parallel for reduction(max : best)
+  for (int64_t j = 0; j < Ns; ++j) {
+    const double v = colsum[j] + hypot(dg[j] - mu_r, -mu_i);
+    best = fmax(best, v);
+  }
+  free(dg);
+  return best;
+}
+
+/* Taylor loop of qo_expm_c128 on the real CSR form (plus diagonal). */
+void qo_expm_rcsr_c128(int64_t Ns, const void* indptr, const void* indices, int idx64, const double* data,
+                       const double* D, double dscale, int s, int m_star, double tol, double mu_r, double mu_i,
+                       double a_r, double a_i, double* F, double* work) {
+  qo_rcsr A = {Ns, indptr, indices, idx64, data, D, dscale};
+  double* B = work;
+  double* B2 = work + 2 * Ns;
+  const double er = a_r * mu_r - a_i * mu_i, ei = a_r * mu_i + a_i * mu_r;
+  const double mag = exp(er / s);
+  const double eta_r = mag * cos(ei / s), eta_i = mag * sin(ei / s);
+  memcpy(B, F, sizeof(double) * 2 * Ns);
+  for (int i = 0; i < s; ++i) {
+    double c1 = 0;
+#pragma omp parallel for reduction(max : c1)
+    for (int64_t r = 0; r < Ns; ++r) c1 = fmax(c1, hypot(B[2 * r], B[2 * r + 1]));
+    for (int j = 0; j < m_star; ++j) {
+      const double cr = a_r / (s * (j + 1.0)), ci = a_i / (s * (j + 1.0));
+      rcsr_spmv(&A, B, B2);
       double c2 = 0, nf = 0;
 #pragma omp parallel for reduction(max : c2, nf)
       for (int64_t r = 0; r < Ns; ++r) {

@@ -80,6 +80,8 @@ SIGNATURES = {
     "qsx_csr_trace": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _vp, _vp]),
     "qsx_csr_1_norm": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _dbl, _dbl, _vp, _vp, _vp]),
     "qsx_csr_1_norm_diag": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _dbl, _dbl, _vp, _dbl, _dbl, _vp, _vp, _vp]),
+    "qsx_csr_col_split": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _vp, _vp, _vp]),
+    "qsx_partition_exact": (_i32, [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp, _i32, _dbl, _vp, _vp, _vp]),
     "qsx_expm_multiply_csr": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _i32, _i32, _dbl, _dbl, _dbl, _dbl,
                                      _dbl, _vp, _vp, _i32, _i64, _i32, _vp]),
     "qsx_expm_multiply_csr_noalloc": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _i32, _i32, _dbl, _dbl, _dbl,

@@ -649,6 +649,148 @@ extern "C" int qsx_csr_1_norm_diag(const void* indptr, const void* indices, cons
   return QSX_OK;
 }
 
+// diagonal of a CSR (duplicates summed; 0 where a row has no diagonal entry)
+template <typename MT>
+__global__ void __launch_bounds__(256) k_csr_diag(const void* indptr, const void* indices, const MT* __restrict__ data,
+                                                  int idx64, int64_t Ns, double2* __restrict__ dg) {
+  for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < Ns; r += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t k0 = ld_idx(indptr, idx64, r), k1 = ld_idx(indptr, idx64, r + 1);
+    double2 d = make_double2(0.0, 0.0);
+    for (int64_t k = k0; k < k1; ++k)
+      if (ld_idx(indices, idx64, k) == r) d = cadd(d, MTr<MT>::ld(data + k));
+    dg[r] = d;
+  }
+}
+
+// Column data of A for 1-norms of A + shifted diagonals: colabs[j] =
+// sum_i |A_ij| and dg[j] = A_jj (Ns device doubles and Ns double2).
+extern "C" int qsx_csr_col_split(const void* indptr, const void* indices, const void* data, int idx64, int mat_dtype,
+                                 int64_t Ns, double* colabs, double* dg, void* stream) {
+  if (Ns < 0 || (Ns > 0 && (!indptr || !colabs || !dg))) { qsx_set_error("qsx_csr_col_split: bad arguments"); return QSX_ERR_ARG; }
+  if (Ns == 0) return QSX_OK;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_fill_f64, dim3(red_grid(Ns)), dim3(256), 0, st, colabs, Ns, 0.0);
+  QSX_MT_SWITCH(mat_dtype, hipLaunchKernelGGL(k_csr_colabs<MT>, dim3(red_grid(Ns) * 4), dim3(256), 0, st, indptr,
+                                              indices, (const MT*)data, idx64, Ns, make_double2(0.0, 0.0), nullptr,
+                                              make_double2(0.0, 0.0), colabs));
+  QSX_MT_SWITCH(mat_dtype, hipLaunchKernelGGL(k_csr_diag<MT>, dim3(red_grid(Ns)), dim3(256), 0, st, indptr, indices,
+                                              (const MT*)data, idx64, Ns, (double2*)dg));
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Exact Taylor partitions of a block of time steps, on the device.  For step
+// record k, ||A + dscale_k D - mu_k I||_1 = max_j (colabs_j - |dg_j| +
+// |dg_j + dscale_k D_j - mu_k|) from the column data of A, then (m*, s) by
+// the theta branch of _fragment_3_1 (scipy _expm_multiply.py:503-558 as
+// EMP.html:377-385 calls it, n0 = 1, ell = 2): s = ceil(|a| norm / theta_m),
+// the smallest m * s over the theta table, first minimum kept.  A step whose
+// |a| norm exceeds the branch bound is flagged in fail[k] (its partition
+// needs the power-norm estimates of the other branch).
+// ---------------------------------------------------------------------------
+constexpr int PX_KB = 8;  // steps per thread of the norm pass
+
+__global__ void k_part_clear(unsigned long long* __restrict__ nmax, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) nmax[k] = 0ull;
+}
+
+__global__ void __launch_bounds__(256) k_part_norm(const double* __restrict__ colabs, const double2* __restrict__ dg,
+                                                   const double* __restrict__ D, int64_t N, const ExpmSec* __restrict__ es,
+                                                   int nsteps, unsigned long long* __restrict__ nmax) {
+  const int k0 = blockIdx.y * PX_KB;
+  double2 dsc[PX_KB], mu[PX_KB];
+  double best[PX_KB];
+#pragma unroll
+  for (int b = 0; b < PX_KB; ++b) {
+    const int k = k0 + b < nsteps ? k0 + b : nsteps - 1;
+    dsc[b] = es[k].dscale;
+    mu[b] = es[k].mu;
+    best[b] = 0.0;
+  }
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < N; j += (int64_t)gridDim.x * blockDim.x) {
+    const double2 d = dg[j];
+    const double off = fmax(colabs[j] - hypot(d.x, d.y), 0.0);
+    const double dj = D ? D[j] : 0.0;
+#pragma unroll
+    for (int b = 0; b < PX_KB; ++b) {
+      const double re = d.x + dsc[b].x * dj - mu[b].x, im = d.y + dsc[b].y * dj - mu[b].y;
+      best[b] = fmax(best[b], off + hypot(re, im));
+    }
+  }
+  __shared__ double red[PX_KB][4];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int b = 0; b < PX_KB; ++b) {
+    double v = best[b];
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if (lane == 0) red[b][w] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < PX_KB && k0 + (int)threadIdx.x < nsteps) {
+    const int b = threadIdx.x;
+    const double v = fmax(fmax(red[b][0], red[b][1]), fmax(red[b][2], red[b][3]));
+    // non-negative doubles order as their bit patterns
+    atomicMax(nmax + k0 + b, (unsigned long long)__double_as_longlong(v));
+  }
+}
+
+__global__ void k_part_theta(ExpmSec* __restrict__ es, int nsteps, const unsigned long long* __restrict__ nmax,
+                             const double* __restrict__ theta, const int* __restrict__ mth, int nth, double bound,
+                             int* __restrict__ fail) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nsteps) return;
+  const double nrm = __longlong_as_double((long long)nmax[k]);
+  const double one = hypot(es[k].a.x, es[k].a.y) * nrm;
+  int bm = 0, bs = 1, f = 0;
+  if (one > 0.0) {
+    if (one <= bound) {
+      long long best_m = -1, best_s = 0;
+      for (int i = 0; i < nth; ++i) {
+        const long long sv = (long long)ceil(one / theta[i]);
+        if (best_m < 0 || (long long)mth[i] * sv < best_m * best_s) {
+          best_m = mth[i];
+          best_s = sv;
+        }
+      }
+      bm = (int)best_m;
+      bs = (int)best_s;
+    } else {
+      bm = 1;
+      f = 1;
+    }
+  }
+  es[k].m = bm;
+  es[k].s = bs;
+  fail[k] = f;
+}
+
+// es: nsteps ExpmSec records (a, mu, dscale set; m and s written); colabs/dg
+// from qsx_csr_col_split (or the matrix-free column pass), D the dynamic
+// diagonal (or null); theta/mth: the nth theta_m values and their m, in the
+// order the selection visits them; bound: the branch bound on |a| norm;
+// nmax: nsteps device words of scratch; fail: nsteps ints.
+extern "C" int qsx_partition_exact(const double* colabs, const double* dg, const double* D, int64_t N, void* es,
+                                   int nsteps, const double* theta, const int* mth, int nth, double bound,
+                                   unsigned long long* nmax, int* fail, void* stream) {
+  if (!es || nsteps < 1 || !theta || !mth || nth < 1 || !nmax || !fail || N < 0 || (N > 0 && (!colabs || !dg))) {
+    qsx_set_error("qsx_partition_exact: bad arguments");
+    return QSX_ERR_ARG;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_part_clear, dim3((nsteps + 255) / 256), dim3(256), 0, st, nmax, nsteps);
+  if (N > 0) {
+    const dim3 grid((unsigned)std::min<int64_t>(red_grid(N), 512), (unsigned)((nsteps + PX_KB - 1) / PX_KB));
+    hipLaunchKernelGGL(k_part_norm, grid, dim3(256), 0, st, colabs, (const double2*)dg, D, N, (const ExpmSec*)es,
+                       nsteps, nmax);
+  }
+  hipLaunchKernelGGL(k_part_theta, dim3((nsteps + 255) / 256), dim3(256), 0, st, (ExpmSec*)es, nsteps, nmax, theta,
+                     mth, nth, bound, fail);
+  QSX_LAUNCH_CHECK();
+  return QSX_OK;
+}
+
 // ---------------------------------------------------------------------------
 // Taylor-series expm: F <- e^{a(A - mu I)} ... e^{a mu} F, in place
 // ---------------------------------------------------------------------------

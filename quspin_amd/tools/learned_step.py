@@ -19,6 +19,12 @@ and the partition from scipy's ``_fragment_3_1`` on the host
   passing the truncation test flags its step.  The block is then restored
   and redone with the exact selection (``learned=False`` runs that exact
   method throughout).
+* The exact selection runs on the host step by step, as the reference does
+  (``exact="host"``), or for a whole block on the device
+  (``exact="device"``, ``qsx_partition_exact``): the norms
+  ||H0 + f(t) D - mu I||_1 of all steps of the block from H0's column data
+  (one pass over three vectors), then the theta branch of ``_fragment_3_1``,
+  then the block's steps as one launch sequence like the learned ones.
 
 Each step evaluates H at the step midpoint, t + dt/2.
 """
@@ -26,7 +32,7 @@ Each step evaluates H at the step midpoint, t + dt/2.
 import math
 
 import numpy as np
-from scipy.sparse.linalg._expm_multiply import _fragment_3_1
+from scipy.sparse.linalg._expm_multiply import _compute_p_max, _fragment_3_1, _theta
 
 from .. import _lib as L
 from .._lib import ptr, stream_handle
@@ -37,6 +43,23 @@ RECORD = np.dtype([("a", "<c16"), ("mu", "<c16"), ("dscale", "<c16"), ("tol", "<
                    ("pad", "<i4", (2,)), ("pad2", "<i4", (2,))])
 
 MLP_SHAPES = [(64, 4), (64,), (64, 64), (64,), (2, 64), (2,)]
+
+
+_THETA = {}
+
+
+def _theta_table():
+    """scipy's theta_m table on the device (values and m, in the order
+    ``_fragment_3_1`` visits them) and the bound of its theta branch on
+    |a| * ||A - mu I||_1 (n0 = 1, ell = 2, m_max = 55)."""
+    dev = L.torch().cuda.current_device()
+    if dev not in _THETA:
+        ms = np.array(list(_theta.keys()), np.int32)
+        th = np.array([_theta[m] for m in ms], np.float64)
+        p_max = _compute_p_max(55)
+        bound = 2 * 2 * p_max * (p_max + 3) * (_theta[55] / float(1 * 55))
+        _THETA[dev] = (L.to_device(th), L.to_device(ms), int(ms.size), float(bound))
+    return _THETA[dev]
 
 
 def _check_record():
@@ -197,6 +220,8 @@ class DrivenEvolution:
         self.tol = float(np.finfo(self.vdt).eps / 2)
         self.predictor = predictor
         self.m_margin, self.m_cap, self.s_cap, self.group = m_margin, m_cap, s_cap, group
+        self._h0 = as_device_operator(H0)
+        self._cols = None
         if form == "kron":
             self.csr, self.kron = None, self._assemble_kron(H0)
             self.N = self.kron.N
@@ -272,6 +297,50 @@ class DrivenEvolution:
         """Reference partition at step midpoint t."""
         return exact_partition(self.norm_1(t), -1j * self.dt, self.tol)
 
+    def _col_data(self):
+        """Column absolute sums and diagonal of H0 (device), for the norms of
+        every H0 + f(t) D - mu I: from the assembled CSR, else from the
+        matrix-free column pass."""
+        if self._cols is None:
+            if self.csr is not None:
+                c = self.csr
+                colabs = L.empty(max(self.N, 1), np.float64)
+                dg = L.empty((max(self.N, 1), 2), np.float64)
+                L.check(L.load().qsx_csr_col_split(ptr(c.indptr), ptr(c.indices), ptr(c.data), c.idx64,
+                                                   L.dtype_code(c.dtype), self.N, ptr(colabs), ptr(dg),
+                                                   stream_handle()), "qsx_csr_col_split")
+                self._cols = (colabs, dg)
+            else:
+                r = self._h0._column_norms(0.0) if isinstance(self._h0, MatrixFreeOp) else None
+                if r is None:
+                    raise NotImplementedError("device exact selection needs H0's column data")
+                self._cols = r
+        return self._cols
+
+    def _exact_block(self, psi, buf, ts, graph):
+        """Exact partitions of a block chosen on the device, then the block
+        as one launch sequence; False (nothing run) when a step needs the
+        host's other branch."""
+        nb = len(ts)
+        t = L.torch()
+        rec = buf["rec"][:nb * RECORD.itemsize]
+        rec.copy_(t.from_numpy(self._records(ts).view(np.uint8)))
+        colabs, dg = self._col_data()
+        th, mth, nth, bound = _theta_table()
+        fail = buf["fail"][:nb]
+        L.check(L.load().qsx_partition_exact(ptr(colabs), ptr(dg), ptr(self.D), self.N, ptr(rec), nb, ptr(th),
+                                             ptr(mth), nth, bound, ptr(buf["nmax"]), ptr(fail), stream_handle()),
+                "qsx_partition_exact")
+        parts = L.to_host(rec).view(RECORD)
+        if int(L.to_host(fail).sum()) != 0:
+            return False
+        s_b = max(int(parts["s"].max()), 1)
+        m_b = max(int(-(-int(parts["m"].max()) // 8) * 8), 8)
+        self._steps(psi, buf["work"], rec, nb, s_b, m_b, fail, graph)
+        self.stats["norm_passes"] += nb
+        self.stats["exact_steps"] += nb
+        return True
+
     def _records(self, ts, parts=None):
         r = np.zeros(len(ts), RECORD)
         r["a"] = -1j * self.dt
@@ -339,13 +408,17 @@ class DrivenEvolution:
             b = {"nb": nb, "rec": L.empty(nb * RECORD.itemsize, np.uint8), "X": L.empty((nb, 4), np.float64),
                  "fail": L.zeros(nb, np.int32), "work": L.empty(2 * self.N, self.vdt),
                  "backup": L.empty(self.N, self.sdt), "rec1": L.empty(RECORD.itemsize, np.uint8),
-                 "fail1": L.zeros(1, np.int32)}
+                 "fail1": L.zeros(1, np.int32), "nmax": L.zeros(nb, np.int64)}
             self._buf = b
         return b
 
-    def run(self, psi, t0, nsteps, learned=True, block=50, graph=True):
+    def run(self, psi, t0, nsteps, learned=True, block=50, graph=True, exact="host"):
         """Evolve the device vector ``psi`` (overwritten) by ``nsteps`` steps
-        from ``t0``; returns psi."""
+        from ``t0``; returns psi.  ``exact``: where the exact selection of
+        the steps that are not learned runs, "host" (per step, as the
+        reference) or "device" (per block)."""
+        if exact not in ("host", "device"):
+            raise ValueError("exact must be 'host' or 'device'")
         t = L.torch()
         if psi.numel() != self.N or L.np_dtype(psi.dtype) != self.sdt or not psi.is_contiguous():
             raise ValueError("psi must be a contiguous device vector of {} {}".format(self.N, self.sdt))
@@ -374,6 +447,8 @@ class DrivenEvolution:
                 else:
                     self.stats["fallback_blocks"] += 1
                     psi.copy_(buf["backup"])
+            if not done and exact == "device":
+                done = self._exact_block(psi, buf, ts, graph)
             if not done:
                 for tt in ts:
                     m, s = self.exact(tt)

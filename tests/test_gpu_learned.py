@@ -191,3 +191,46 @@ def test_fp32_terms_with_fp64_state_accumulation():
     emx = np.linalg.norm(out["mixed"] - out["fp64"])
     print(f"state error after {n} steps: complex64 {e32:.3e}, complex64 terms + complex128 state {emx:.3e}")
     assert emx < 1e-6 and emx < 0.7 * e32
+
+
+@pytest.mark.parametrize("form,dtype", [("csr", np.complex128), ("csr", np.complex64), ("kron", np.complex128)])
+def test_device_exact_selection_matches_host(form, dtype):
+    """The exact partition chosen on the device for a block of steps
+    (qsx_partition_exact: column data of H0, theta branch of _fragment_3_1)
+    equals the host's reference selection step by step, a step above the
+    branch bound is flagged, and evolving with it equals the host-selected
+    evolution."""
+    import torch
+    from quspin_amd import _lib as L
+    from quspin_amd._lib import ptr, stream_handle
+    from quspin_amd.tools.learned_step import RECORD, DrivenEvolution, _theta_table
+    print(f"case form={form} dtype={np.dtype(dtype).name}", flush=True)
+    B, hop, D, f, phase = _hubbard(8)
+    dt, nb = 0.05, 20
+    ev = DrivenEvolution(hop, D, f, dt, dtype=dtype, phase=phase, form=form, state_dtype=np.complex128)
+    ts = 0.3 + (np.arange(nb) + 0.5) * dt
+    buf = ev._buffers(nb)
+    recs = ev._records(ts)
+    recs["a"][-1] = -50j  # |a| * norm far above the theta branch's bound
+    rec = buf["rec"][:nb * RECORD.itemsize]
+    rec.copy_(torch.from_numpy(recs.view(np.uint8)))
+    colabs, dg = ev._col_data()
+    th, mth, nth, bound = _theta_table()
+    fail = buf["fail"][:nb]
+    L.check(L.load().qsx_partition_exact(ptr(colabs), ptr(dg), ptr(ev.D), ev.N, ptr(rec), nb, ptr(th), ptr(mth), nth,
+                                         bound, ptr(buf["nmax"]), ptr(fail), stream_handle()), "qsx_partition_exact")
+    parts = L.to_host(rec).view(RECORD)
+    flags = L.to_host(fail)
+    for k, t in enumerate(ts[:-1]):
+        assert (int(parts["m"][k]), int(parts["s"][k])) == tuple(ev.exact(t)), k
+        assert flags[k] == 0
+    assert flags[-1] == 1
+    rng = np.random.default_rng(8)
+    psi0 = rng.normal(size=B.Ns) + 1j * rng.normal(size=B.Ns)
+    psi0 /= np.linalg.norm(psi0)
+    ph, pd = L.to_device(psi0.copy()), L.to_device(psi0.copy())
+    ev.run(ph, 0.3, nb, learned=False)
+    ev.stats = {k: 0 for k in ev.stats}
+    ev.run(pd, 0.3, nb, learned=False, exact="device", block=10)
+    assert ev.stats["exact_steps"] == nb and ev.stats["norm_passes"] == nb
+    np.testing.assert_allclose(L.to_host(pd), L.to_host(ph), rtol=0, atol=1e-12)

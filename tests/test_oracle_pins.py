@@ -351,3 +351,32 @@ def test_ftlm_ltlm_beta_zero_any_krylov_size():
     np.testing.assert_allclose(l_res["O"], want, rtol=1e-10)
     np.testing.assert_allclose(c0, 1.0, rtol=1e-12)
     np.testing.assert_allclose(Id, 1.0, rtol=1e-12)
+
+
+def test_c_oracle_table_form_and_real_csr_taylor():
+    """The C restatement's timed forms: the sector-table H.v equals its
+    orbit-search H.v (same matrix elements through another lookup); the real
+    CSR (+ diagonal) 1-norm equals the dense column sum and its Taylor loop
+    equals dense expm."""
+    from scipy.linalg import expm
+    from oracle import c_oracle as C
+    L = 14
+    m = chain_maps(L)
+    g = C.Group(L, [m["P"], m["Z"], m["T"]], [2, 2, L], [0, 0, 0])
+    b, n = g.make_basis(L // 2)
+    tab = C.table_build(g, L // 2, b)
+    terms = flat_terms(xxz_static(L, Delta=0.6))
+    v = np.random.default_rng(1).normal(size=len(b))
+    np.testing.assert_allclose(C.hv_table_real(g, b, n, tab, terms, v), C.hv_real(g, b, n, terms, v), atol=1e-13)
+    rng = np.random.default_rng(2)
+    N = 50
+    A = sp.random(N, N, density=0.15, random_state=3)
+    A = (A + A.T).tocsr()
+    D = rng.normal(size=N)
+    M = (A + 0.4 * sp.diags(D)).toarray()
+    mu = np.trace(M) / N
+    for idx in (np.int32, np.int64):
+        R = C.RealCSR(A.indptr.astype(idx), A.indices.astype(idx), A.data, D)
+        np.testing.assert_allclose(R.one_norm(0.4, mu), np.abs(M - mu * np.eye(N)).sum(axis=0).max(), rtol=1e-13)
+        w = rng.normal(size=N) + 1j * rng.normal(size=N)
+        np.testing.assert_allclose(R.expm(w, 2, 40, 2.0 ** -53, mu, -0.6j, 0.4), expm(-0.6j * M) @ w, atol=1e-13)
