@@ -88,9 +88,9 @@ int qsx_sector_rank(qsx_core* core, const void* states, int64_t n, int64_t* out,
  * size query and build.  Trades HBM for the orbit search of the H.v; no
  * counterpart in the reference.  nrm / n_bytes: the basis n array (device).
  * When lut_len > 0 the code field is an id into a coefficient table of
- * lut_len double2 (chi(g) * sqrt(n) per group element and distinct n of the
- * basis), filled by qsx_table_build, so the H.v needs no read of the
- * target's n; otherwise it is the group element. */
+ * lut_len double2 (chi * sqrt(n) per distinct character chi(g) of the group
+ * and distinct n of the basis), filled by qsx_table_build, so the H.v needs
+ * no read of the target's n; otherwise it is the group element. */
 int qsx_table_info(const qsx_core* core, int64_t Ns, const void* nrm, int n_bytes, int64_t* entries, int* entry_bytes,
                    int* idx_bits, int64_t* lut_len);
 int qsx_table_build(qsx_core* core, const void* basis, int64_t Ns, const int64_t* begin, const int64_t* end, int N_p,
@@ -268,6 +268,29 @@ int qsx_csr_col_split(const void* indptr, const void* indices, const void* data,
 int qsx_partition_exact(const double* colabs, const double* dg, const double* D, int64_t N, void* es, int nsteps,
                         const double* theta, const int* mth, int nth, double bound, unsigned long long* nmax,
                         int* fail, void* stream);
+
+/* Device CSR algebra (hamiltonian_core.py:1686-1720 tocsr(time), and the
+ * hamiltonian's __add__, __mul__, astype, transpose, conj, getH, which the
+ * reference runs through scipy).  Inputs are canonical CSR (sorted, unique
+ * column indices per row); so are the results.
+ *
+ * qsx_csr_sum_create: sum_k coef_k op_k(A_k) of K <= 32 matrices with Nr rows
+ * (op_k = complex conjugate where conjs[k] = 1; coefs = K (re, im) pairs;
+ * per matrix: index width idx64s[k], value dtype dtypes[k]); exact-zero sums
+ * are dropped, as scipy's addition drops them.  Returns nnz; finish writes
+ * indptr (Nr + 1), indices and values (nnz) in the given widths. */
+typedef struct qsx_csr_sum qsx_csr_sum;
+int qsx_csr_sum_create(int K, const void* const* indptrs, const void* const* indices, const void* const* datas,
+                       const int* idx64s, const int* dtypes, const int* conjs, const double* coefs, int64_t Nr,
+                       qsx_csr_sum** out, int64_t* nnz, void* stream);
+int qsx_csr_sum_finish(qsx_csr_sum* h, int out_dtype, int idx_bytes, void* out_indptr, void* out_indices,
+                       void* out_data, void* stream);
+void qsx_csr_sum_destroy(qsx_csr_sum* h);
+/* (Conjugate) transpose into preallocated arrays: indptr (Nc + 1), indices and
+ * values (nnz).  Deterministic (radix sort on (column, row)). */
+int qsx_csr_transpose(const void* indptr, const void* indices, const void* data, int idx64, int dtype, int64_t Nr,
+                      int64_t Nc, int64_t nnz, int conj, int out_dtype, int idx_bytes, void* out_indptr,
+                      void* out_indices, void* out_data, void* stream);
 
 /* Replaces _wrapper_expm_multiply(indptr, indices, data, s, m_star, tol, mu,
  * a, v, work) and _wrapper_expm_multiply_batch (EMP.html:347-372): v <-

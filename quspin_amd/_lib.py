@@ -81,6 +81,10 @@ SIGNATURES = {
     "qsx_csr_1_norm": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _dbl, _dbl, _vp, _vp, _vp]),
     "qsx_csr_1_norm_diag": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _dbl, _dbl, _vp, _dbl, _dbl, _vp, _vp, _vp]),
     "qsx_csr_col_split": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _vp, _vp, _vp]),
+    "qsx_csr_sum_create": (_i32, [_i32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "qsx_csr_sum_finish": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp, _vp]),
+    "qsx_csr_sum_destroy": (None, [_vp]),
+    "qsx_csr_transpose": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _i64, _i32, _i32, _i32, _vp, _vp, _vp, _vp]),
     "qsx_partition_exact": (_i32, [_vp, _vp, _vp, _i64, _vp, _i32, _vp, _vp, _i32, _dbl, _vp, _vp, _vp]),
     "qsx_expm_multiply_csr": (_i32, [_vp, _vp, _vp, _i32, _i32, _i64, _i64, _i32, _i32, _dbl, _dbl, _dbl, _dbl,
                                      _dbl, _vp, _vp, _i32, _i64, _i32, _vp]),

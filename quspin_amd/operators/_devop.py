@@ -440,15 +440,92 @@ class _Ctx:
             pass
 
 
+def _idx_dtype(nnz, rows):
+    return np.int64 if (nnz >= 2 ** 31 - 1 or rows >= 2 ** 31 - 1) else np.int32
+
+
+def _combine_dtype(mats, coefs, conj):
+    dt = np.result_type(*[M.dtype for M in mats])
+    if dt.kind != "c" and any(complex(c).imag != 0 for c in coefs):
+        dt = np.result_type(dt, np.complex64)
+    return np.dtype(dt)
+
+
+def csr_combine(mats, coefs, dtype=None, conj=None):
+    """sum_k coefs[k] * op_k(mats[k]) of device CSR matrices of one shape,
+    op_k the complex conjugate where conj[k], on the device
+    (``qsx_csr_sum_*``); entries summing to exactly zero are dropped.  The
+    result dtype defaults to that of the matrices, complex when a
+    coefficient is."""
+    K = len(mats)
+    if K == 0:
+        raise ValueError("nothing to combine")
+    conj = [bool(c) for c in (conj if conj is not None else [False] * K)]
+    coefs = [complex(c) for c in coefs]
+    dt = np.dtype(dtype) if dtype is not None else _combine_dtype(mats, coefs, conj)
+    if K > 32:
+        head = csr_combine(mats[:32], coefs[:32], dt, conj[:32])
+        return csr_combine([head] + list(mats[32:]), [1.0] + coefs[32:], dt, [False] + conj[32:])
+    shape = mats[0].shape
+    if any(M.shape != shape for M in mats):
+        raise ValueError("matrices of different shapes")
+    lib = L.load()
+    vp = C.c_void_p * K
+    ia = C.c_int * K
+    coef_arr = (C.c_double * (2 * K))(*[x for c in coefs for x in (c.real, c.imag)])
+    h = L._vp()
+    nnz = L._i64()
+    rc = lib.qsx_csr_sum_create(K, vp(*[ptr(M.indptr) for M in mats]), vp(*[ptr(M.indices) for M in mats]),
+                                vp(*[ptr(M.data) for M in mats]), ia(*[M.idx64 for M in mats]),
+                                ia(*[L.dtype_code(M.dtype) for M in mats]), ia(*[int(c) for c in conj]), coef_arr,
+                                shape[0], C.byref(h), C.byref(nnz), stream_handle())
+    try:
+        L.check(rc, "qsx_csr_sum_create")
+        n = nnz.value
+        idx = _idx_dtype(n, shape[0])
+        indptr = L.empty(shape[0] + 1, idx)
+        indices = L.empty(max(n, 1), idx)
+        data = L.empty(max(n, 1), dt)
+        L.check(lib.qsx_csr_sum_finish(h, L.dtype_code(dt), np.dtype(idx).itemsize, ptr(indptr), ptr(indices),
+                                       ptr(data), stream_handle()), "qsx_csr_sum_finish")
+    finally:
+        lib.qsx_csr_sum_destroy(h)
+    return DeviceCSR(indptr, indices[:n], data[:n], shape, dt)
+
+
+def csr_transpose(M, conj=False, dtype=None):
+    """The transpose (conj: the conjugate transpose) of a device CSR, on the
+    device (``qsx_csr_transpose``)."""
+    dt = np.dtype(dtype) if dtype is not None else M.dtype
+    Nr, Nc = M.shape
+    idx = _idx_dtype(M.nnz, Nc)
+    indptr = L.empty(Nc + 1, idx)
+    indices = L.empty(max(M.nnz, 1), idx)
+    data = L.empty(max(M.nnz, 1), dt)
+    L.check(L.load().qsx_csr_transpose(ptr(M.indptr), ptr(M.indices), ptr(M.data), M.idx64, L.dtype_code(M.dtype),
+                                       Nr, Nc, M.nnz, int(bool(conj)), L.dtype_code(dt), np.dtype(idx).itemsize,
+                                       ptr(indptr), ptr(indices), ptr(data), stream_handle()), "qsx_csr_transpose")
+    return DeviceCSR(indptr, indices[:M.nnz], data[:M.nnz], (Nc, Nr), dt)
+
+
+def csr_diagonal(d, dtype):
+    """Diagonal device CSR with the entries of the device vector d."""
+    t = L.torch()
+    n = int(d.numel())
+    idx = _idx_dtype(n, n)
+    ar = t.arange(n + 1, dtype=L.torch_dtype(idx), device="cuda")
+    return DeviceCSR(ar, ar[:n], d.to(L.torch_dtype(np.dtype(dtype))).contiguous(), (n, n), dtype)
+
+
 def build_csr(basis, oplist, dtype, transposed=False, conjugated=False, a=1.0, diag=None):
     """Assemble a*Op as a canonical device CSR (``qsx_csr_builder_*``).
     Lists that do not commute with the symmetry group are assembled through
-    their transpose (rows gathered with the original terms)."""
+    their transpose (rows gathered with the original terms), transposed
+    back on the device."""
     if len(basis._pers) and not oplist.symmetric and not transposed:
-        MT = build_csr(basis, oplist, dtype, True, conjugated, a, None)
-        M = DeviceCSR.from_scipy(MT.to_scipy().T.tocsr(), dtype)
+        M = csr_transpose(build_csr(basis, oplist, dtype, True, conjugated, a, None))
         if diag is not None:
-            M = DeviceCSR.from_scipy(M.to_scipy() + sp.diags(L.to_host(diag).astype(dtype), format="csr"), dtype)
+            M = csr_combine([M, csr_diagonal(diag, dtype)], [1.0, 1.0], dtype)
         return M
     lib = L.load()
     dtype = np.dtype(dtype)
@@ -472,8 +549,7 @@ def build_csr(basis, oplist, dtype, transposed=False, conjugated=False, a=1.0, d
         lib.qsx_csr_builder_destroy(h)
     M = DeviceCSR(indptr, indices[:n], data[:n], (basis.Ns, basis.Ns), dtype)
     if diag is not None:
-        D = sp.diags(L.to_host(diag).astype(dtype), format="csr")
-        M = DeviceCSR.from_scipy(M.to_scipy() + D, dtype)
+        M = csr_combine([M, csr_diagonal(diag, dtype)], [1.0, 1.0], dtype)
     return M
 
 

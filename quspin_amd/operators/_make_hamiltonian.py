@@ -58,3 +58,34 @@ class function:
 
     def __repr__(self):
         return "function({}, {})".format(getattr(self._func, "__name__", self._func), self._args)
+
+    def conjugate(self):
+        """t -> conj(f(t)) (the drive of the conjugated operator, _functions.py conjugate_function)."""
+        return _conj_function(self)
+
+    conj = conjugate
+
+
+class _conj_function(function):
+    """Complex conjugate of a drive; its own conjugate is the drive again."""
+
+    def __init__(self, base):
+        function.__init__(self, base._func, base._args)
+        self._base = base
+
+    def __call__(self, t):
+        return np.conj(self._base(t))
+
+    def conjugate(self):
+        return self._base
+
+    conj = conjugate
+
+    def __eq__(self, other):
+        return isinstance(other, _conj_function) and self._base == other._base
+
+    def __hash__(self):
+        return hash(("conj", hash(self._base)))
+
+    def __repr__(self):
+        return "conj({!r})".format(self._base)

@@ -13,7 +13,7 @@ import numpy as np
 import scipy.sparse as sp
 
 from .. import _lib as L
-from ._devop import DeviceCSR
+from ._devop import DeviceCSR, csr_combine, csr_transpose
 from ._make_hamiltonian import _consolidate_static, _consolidate_dynamic, function
 
 supported_dtypes = (np.float32, np.float64, np.complex64, np.complex128)
@@ -87,8 +87,7 @@ class hamiltonian:
             func = function(f, f_args)
             Md = DeviceCSR.from_scipy(sp.csr_matrix(M, dtype=self._dtype), self._dtype)
             if func in self._dynamic:
-                self._dynamic[func] = DeviceCSR.from_scipy(self._dynamic[func].to_scipy() + Md.to_scipy(),
-                                                           self._dtype)
+                self._dynamic[func] = csr_combine([self._dynamic[func], Md], [1.0, 1.0], self._dtype)
             else:
                 self._dynamic[func] = Md
 
@@ -102,7 +101,7 @@ class hamiltonian:
         else:
             M = DeviceCSR.from_scipy(sp.csr_matrix((Ns, Ns), dtype=self._dtype), self._dtype)
         if host_extra is not None:
-            M = DeviceCSR.from_scipy(M.to_scipy() + host_extra, self._dtype)
+            M = csr_combine([M, DeviceCSR.from_scipy(host_extra, self._dtype)], [1.0, 1.0], self._dtype)
         return M
 
     @classmethod
@@ -159,23 +158,25 @@ class hamiltonian:
         return n
 
     # ------------------------------------------------------------ algebra
-    def _map(self, fstatic, fdyn=None):
-        fdyn = fdyn or fstatic
-        st = DeviceCSR.from_scipy(fstatic(self._static.to_scipy()), None)
-        dy = {f: DeviceCSR.from_scipy(fdyn(M.to_scipy()), None) for f, M in self._dynamic.items()}
-        dt = st.dtype
-        return hamiltonian._from_parts(st, dy, dt, self._shape[::-1], self._basis)
+    # every part stays a device CSR: transposes and linear combinations run
+    # on the device (csr_transpose, csr_combine)
+    def _map(self, fpart, shape=None, conj_funcs=False):
+        st = fpart(self._static)
+        dy = {(f.conj() if conj_funcs else f): fpart(M) for f, M in self._dynamic.items()}
+        return hamiltonian._from_parts(st, dy, st.dtype, shape or self._shape, self._basis)
 
     def transpose(self, copy=False):
-        return self._map(lambda M: M.T.tocsr())
+        return self._map(csr_transpose, self._shape[::-1])
 
     def conjugate(self):
-        return self._map(lambda M: M.conj().tocsr())
+        """Conjugated matrices and drives (hamiltonian_core.py:2058-2079)."""
+        return self._map(lambda M: csr_combine([M], [1.0], conj=[True]) if M.dtype.kind == "c" else M,
+                         conj_funcs=True)
 
     conj = conjugate
 
     def getH(self, copy=False):
-        return self.conj().transpose()
+        return self._map(lambda M: csr_transpose(M, conj=True), self._shape[::-1], conj_funcs=True)
 
     @property
     def T(self):
@@ -186,13 +187,12 @@ class hamiltonian:
         return self.getH()
 
     def __neg__(self):
-        return self._map(lambda M: -M)
+        return self._map(lambda M: csr_combine([M], [-1.0]))
 
     def __mul__(self, other):
         if np.isscalar(other):
             dt = np.result_type(self._dtype, np.min_scalar_type(other))
-            H = self._map(lambda M: (M * other).astype(dt))
-            return H
+            return self._map(lambda M: csr_combine([M], [other], dtype=np.result_type(M.dtype, dt)))
         return self.dot(other)
 
     def __rmul__(self, other):
@@ -207,13 +207,13 @@ class hamiltonian:
 
     def __add__(self, other):
         if ishamiltonian(other):
-            st = DeviceCSR.from_scipy(self._static.to_scipy() + other._static.to_scipy(), None)
+            st = csr_combine([self._static, other._static], [1.0, 1.0])
             dy = {f: M for f, M in self._dynamic.items()}
             for f, M in other._dynamic.items():
-                dy[f] = DeviceCSR.from_scipy(dy[f].to_scipy() + M.to_scipy(), None) if f in dy else M
+                dy[f] = csr_combine([dy[f], M], [1.0, 1.0]) if f in dy else M
             return hamiltonian._from_parts(st, dy, st.dtype, self._shape, self._basis)
         if sp.issparse(other) or isinstance(other, np.ndarray):
-            st = DeviceCSR.from_scipy(self._static.to_scipy() + sp.csr_matrix(other), None)
+            st = csr_combine([self._static, DeviceCSR.from_scipy(sp.csr_matrix(other))], [1.0, 1.0])
             return hamiltonian._from_parts(st, self._dynamic, st.dtype, self._shape, self._basis)
         return NotImplemented
 
@@ -223,17 +223,24 @@ class hamiltonian:
         return self.__add__(-other if ishamiltonian(other) else -other)
 
     def astype(self, dtype, copy=False):
-        st = DeviceCSR.from_scipy(self._static.to_scipy().astype(dtype), dtype)
-        dy = {f: DeviceCSR.from_scipy(M.to_scipy().astype(dtype), dtype) for f, M in self._dynamic.items()}
-        return hamiltonian._from_parts(st, dy, dtype, self._shape, self._basis)
+        H = self._map(lambda M: csr_combine([M], [1.0], dtype=dtype))
+        H._dtype = np.dtype(dtype)
+        return H
 
     # ------------------------------------------------------------- formats
+    def _at(self, time):
+        """Device CSR of H(time) = H_static + sum_f f(time) H_f, combined on
+        the device (hamiltonian_core.py:1686-1720)."""
+        if not self._dynamic:
+            return self._static
+        parts = [self._static] + list(self._dynamic.values())
+        coefs = [1.0] + [f(time) for f in self._dynamic]
+        return csr_combine(parts, coefs)
+
     def tocsr(self, time=0):
-        """Host CSR of H(time) (hamiltonian_core.py:1686-1720)."""
-        H = self._static.to_scipy().copy()
-        for f, M in self._dynamic.items():
-            H = H + f(time) * M.to_scipy()
-        return sp.csr_matrix(H)
+        """Host CSR of H(time) (hamiltonian_core.py:1686-1720): combined on the
+        device, copied once."""
+        return sp.csr_matrix(self._at(time).to_scipy(), copy=True)
 
     def tocsc(self, time=0):
         return self.tocsr(time).tocsc()
@@ -251,9 +258,7 @@ class hamiltonian:
 
     def _qsx_operator(self, time=0.0):
         """Device CSR of H(time) for the tools."""
-        if not self._dynamic:
-            return self._static
-        return DeviceCSR.from_scipy(self.tocsr(time), None)
+        return self._at(time)
 
     # ------------------------------------------------------------------ dot
     def dot(self, V, time=0, check=True, out=None, overwrite_out=True, a=1.0):
@@ -403,7 +408,7 @@ class hamiltonian:
         return np.linalg.eigvalsh(self.toarray(time))
 
     def trace(self, time=0):
-        return self.tocsr(time).diagonal().sum()
+        return self._at(time).trace()
 
     def __repr__(self):
         return "<{0}x{1} qspin dtype:{2}>".format(*(self._shape + (self._dtype,)))

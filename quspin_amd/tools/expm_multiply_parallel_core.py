@@ -20,7 +20,7 @@ from scipy.sparse.linalg import LinearOperator, onenormest
 from scipy.sparse.linalg._expm_multiply import _fragment_3_1
 
 from .. import _lib as L
-from ..operators._devop import DeviceCSR, MatrixFreeOp, SectorBatch, as_device_operator
+from ..operators._devop import DeviceCSR, MatrixFreeOp, SectorBatch, as_device_operator, csr_transpose
 
 
 class LazyOperatorNormInfo:
@@ -144,7 +144,7 @@ class expm_multiply_parallel:
             return MatrixFreeOp(op.basis, op.oplist, op.dtype, transposed=not op.transposed,
                                 conjugated=not op.conjugated, diag_conj=not op.diag_conj, diag=op.diag)
         if not hasattr(self, "_adj_csr"):
-            self._adj_csr = DeviceCSR.from_scipy(op.to_scipy().conj().T.tocsr())
+            self._adj_csr = csr_transpose(op, conj=True)
         return self._adj_csr
 
     def _calculate_partition(self):
