@@ -599,55 +599,23 @@ __global__ void __launch_bounds__(256) k_hv_rows_fast(HvArgs A) {
   }
 }
 
-// k_hv_rows_fast with the terms in batches of B: first the B ranks, then
-// the B table reads (TW-byte entries; TW = 0: direct rank, no table), then
-// the B gathers of x (and coefficient reads), then the B products, so a lane
-// keeps B independent requests in flight instead of one dependent chain per
-// term.  Rows a term does not reach read entry 0 of the table and of x and
-// add zero: the loads stay unconditional, so none waits behind a branch.
+// Off-diagonal sums of RPL rows (states s, ranks ra, 1/sqrt(n) rsq) over all
+// terms of A, the terms in batches of B: first the B ranks, then the B table
+// reads (TW-byte entries; TW = 0: direct rank, no table), then the B gathers
+// of x (and coefficient reads), then the B products, so a lane keeps B
+// independent requests in flight instead of one dependent chain per term.
+// Rows a term does not reach read entry 0 of the table and of x and add
+// zero: the loads stay unconditional, so none waits behind a branch.  sb:
+// the binomials in LDS; slut: the coefficient table in LDS (LUTL).
 template <typename I, typename VT, int M, int LK, int RPL, int B, int TW, bool LUTL>
-__global__ void __launch_bounds__(256) k_hv_rows_fast_batch(HvArgs A) {
+__device__ __forceinline__ void fast_rows_acc(const HvArgs& A, const int64_t* sb, const double2* slut,
+                                              const VT* __restrict__ vin, const I* s, const int64_t* ra,
+                                              const double* rsq, double2* acc) {
   static_assert(LK == LOOKUP_TABLE || LK == LOOKUP_RANK, "rank-based lookups only");
   static_assert((LK == LOOKUP_TABLE) == (TW == 4 || TW == 8), "table entries are 4 or 8 bytes");
   using E = typename std::conditional<TW == 8, uint64_t, uint32_t>::type;
-  // dynamic LDS: A.n_sb binomials (rows 0..N), then with LUTL the lut_len
-  // coefficients (read per element: from LDS rather than through the vector
-  // memory path)
-  extern __shared__ int64_t fast_smem[];
-  int64_t* sb = fast_smem;
-  double2* slut = (double2*)(fast_smem + A.n_sb);
-  if (A.skip && *A.skip) return;
-  {
-    const int n = min(A.n_sb, QSX_SB);
-    const int64_t* __restrict__ bsrc = &A.R->binom[0][0];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) sb[i] = bsrc[i];
-  }
-  if constexpr (LUTL)
-    for (int i = threadIdx.x; i < A.lut_len; i += blockDim.x) slut[i] = A.tlut[i];
-  __syncthreads();
-  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x * RPL + threadIdx.x;
-  if (r0 >= A.Ns) return;
   constexpr bool CPLX = VTr<VT>::cplx;
-  const VT* __restrict__ vin = (const VT*)A.v_in;
-  VT* __restrict__ vout = (VT*)A.v_out;
   const int Np = A.R->Np;
-  int64_t rr[RPL];
-  bool live[RPL];
-  I s[RPL];
-  int64_t ra[RPL];
-  double rsq[RPL];
-  double2 acc[RPL];
-#pragma unroll
-  for (int k = 0; k < RPL; ++k) {
-    rr[k] = r0 + (int64_t)k * blockDim.x;
-    live[k] = rr[k] < A.Ns;
-    const int64_t rc = live[k] ? rr[k] : r0;
-    s[k] = ((const I*)A.basis)[rc];
-    rsq[k] = 1.0;
-    if constexpr (M != 0) rsq[k] = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, rc));
-    ra[k] = comb_rank_sb<I>(sb, s[k]);
-    acc[k] = make_double2(0.0, 0.0);
-  }
   const int nt = A.n_off;
   for (int t0 = 0; t0 < nt; t0 += B) {
     // ranks of the images (rank 0, a valid index, where the term gives none)
@@ -746,6 +714,57 @@ __global__ void __launch_bounds__(256) k_hv_rows_fast_batch(HvArgs A) {
       }
     }
   }
+}
+
+// Stage the binomials (A.n_sb rows' worth) and, with LUTL, the coefficient
+// table into the dynamic LDS of a fast-row kernel; returns the two arrays.
+template <bool LUTL>
+__device__ __forceinline__ void fast_stage(const HvArgs& A, int64_t* smem, const int64_t*& sb, const double2*& slut) {
+  const int n = min(A.n_sb, QSX_SB);
+  const int64_t* __restrict__ bsrc = &A.R->binom[0][0];
+  for (int i = threadIdx.x; i < n; i += blockDim.x) smem[i] = bsrc[i];
+  double2* l = (double2*)(smem + A.n_sb);
+  if constexpr (LUTL)
+    for (int i = threadIdx.x; i < A.lut_len; i += blockDim.x) l[i] = A.tlut[i];
+  sb = smem;
+  slut = l;
+}
+
+// k_hv_rows_fast with the terms in batches of B (fast_rows_acc).
+template <typename I, typename VT, int M, int LK, int RPL, int B, int TW, bool LUTL>
+__global__ void __launch_bounds__(256) k_hv_rows_fast_batch(HvArgs A) {
+  // dynamic LDS: A.n_sb binomials (rows 0..N), then with LUTL the lut_len
+  // coefficients (read per element: from LDS rather than through the vector
+  // memory path)
+  extern __shared__ int64_t fast_smem[];
+  if (A.skip && *A.skip) return;
+  const int64_t* sb;
+  const double2* slut;
+  fast_stage<LUTL>(A, fast_smem, sb, slut);
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x * RPL + threadIdx.x;
+  if (r0 >= A.Ns) return;
+  constexpr bool CPLX = VTr<VT>::cplx;
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ vout = (VT*)A.v_out;
+  int64_t rr[RPL];
+  bool live[RPL];
+  I s[RPL];
+  int64_t ra[RPL];
+  double rsq[RPL];
+  double2 acc[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    rr[k] = r0 + (int64_t)k * blockDim.x;
+    live[k] = rr[k] < A.Ns;
+    const int64_t rc = live[k] ? rr[k] : r0;
+    s[k] = ((const I*)A.basis)[rc];
+    rsq[k] = 1.0;
+    if constexpr (M != 0) rsq[k] = 1.0 / sqrt(load_n(A.nrm, A.n_bytes, rc));
+    ra[k] = comb_rank_sb<I>(sb, s[k]);
+    acc[k] = make_double2(0.0, 0.0);
+  }
+  fast_rows_acc<I, VT, M, LK, RPL, B, TW, LUTL>(A, sb, slut, vin, s, ra, rsq, acc);
 #pragma unroll
   for (int k = 0; k < RPL; ++k) {
     if (!live[k]) continue;

@@ -414,31 +414,6 @@ extern "C" int qsx_lanczos_update(int dtype, const void* v, const double* b_prev
 // fixed order on every block (the same value everywhere, deterministic) and
 // block 0 records the scalar.
 // ---------------------------------------------------------------------------
-// Sum over the 256 lanes of a block, returned on every lane: a shuffle tree
-// inside each wavefront, then the four wavefront sums in a fixed order (the
-// same value on every block for the same inputs).
-__device__ __forceinline__ double block_sum256(double v) {
-  __shared__ double sh[4];
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-  __syncthreads();
-  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
-  __syncthreads();
-  return r;
-}
-
-__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nb) {
-  double a = 0.0;
-  for (int k = threadIdx.x; k < nb; k += blockDim.x) a += part[k];
-  return block_sum256(a);
-}
-
-__device__ __forceinline__ void block_partial(double v, double* __restrict__ part) {
-  const double r = block_sum256(v);
-  if (threadIdx.x == 0) part[blockIdx.x] = r;
-}
-
 // Lanczos recurrence grid: about two elements per lane, at most 1,024 blocks
 // (the partial slots), so a 361 k vector spreads over all CUs.
 static int lz_grid(int64_t n) {

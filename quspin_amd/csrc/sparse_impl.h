@@ -9,6 +9,33 @@ template <> struct MTr<float2> { static constexpr bool cplx = true; __device__ s
 template <> struct MTr<double2> { static constexpr bool cplx = true; __device__ static double2 ld(const double2* p) { return *p; } };
 
 
+// Sum over the 256 lanes of a block, returned on every lane: a shuffle tree
+// inside each wavefront, then the four wavefront sums in a fixed order (the
+// same value on every block for the same inputs).
+__device__ __forceinline__ double block_sum256(double v) {
+  __shared__ double sh[4];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  const double r = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+  __syncthreads();
+  return r;
+}
+
+// Sum of nb per-block partials, computed identically by every block.
+__device__ __forceinline__ double sum_partials(const double* __restrict__ part, int nb) {
+  double a = 0.0;
+  for (int k = threadIdx.x; k < nb; k += blockDim.x) a += part[k];
+  return block_sum256(a);
+}
+
+// This block's partial of a grid-wide sum into part[blockIdx.x].
+__device__ __forceinline__ void block_partial(double v, double* __restrict__ part) {
+  const double r = block_sum256(v);
+  if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
 // Atomic max of a non-negative double through its bit pattern.
 __device__ __forceinline__ void atomic_max_nonneg(double* addr, double v) {
   atomicMax((unsigned long long*)addr, (unsigned long long)__double_as_longlong(v));
