@@ -904,10 +904,12 @@ extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_
 // 0: one lane per row; 1: compacted work queue; 2/3/4: 2/4/8 lanes per row;
 // 5/6: batches of 4/8 terms with rank-based lookups (table or no symmetry);
 // 7: split-chain blocks (one hard-core-boson sector without symmetry maps;
-// the plan is built on the first request and owned by the context).
+// the plan is built on the first request and owned by the context);
+// 8/9: direct-rank rows with the terms in chunks of 4/8, gathers predicated
+// (a sector without symmetry maps; other products use schedule 0).
 // Single vector; several vectors always use one lane per row.
 extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
-  if (!X || sched < 0 || sched > 7) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
+  if (!X || sched < 0 || sched > 9) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
   if (sched == 7 && !X->split_d) {
     const int rc = split_build(X);
     if (rc) return rc;

@@ -599,6 +599,116 @@ __global__ void __launch_bounds__(256) k_hv_rows_fast(HvArgs A) {
   }
 }
 
+// k_hv_rows_fast for the direct rank (no table), the terms in chunks of T:
+// first the T terms' tests and ranks for the lane's RPL rows, then all the
+// gathers of x the chunk's applicable terms need, each under its own
+// predicate (no loads for terms that give a row nothing, so nothing is read
+// for them), then the products.  A chunk no lane of the wave needs is
+// skipped as a whole.
+template <typename I, typename VT, int RPL, int T>
+__global__ void __launch_bounds__(256) k_hv_rows_fast_chunk(HvArgs A) {
+  __shared__ int64_t sb[QSX_SB];
+  if (A.skip && *A.skip) return;
+  stage_binom<LOOKUP_RANK>(A.R, sb);
+  __syncthreads();
+  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x * RPL + threadIdx.x;
+  if (r0 >= A.Ns) return;
+  constexpr bool CPLX = VTr<VT>::cplx;
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ vout = (VT*)A.v_out;
+  const int Np = A.R->Np;
+  const int64_t last = A.R->Ns - 1;
+  int64_t rr[RPL];
+  bool live[RPL];
+  I s[RPL];
+  int64_t ra[RPL];
+  double2 acc[RPL];
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    rr[k] = r0 + (int64_t)k * blockDim.x;
+    live[k] = rr[k] < A.Ns;
+    s[k] = ((const I*)A.basis)[live[k] ? rr[k] : r0];
+    ra[k] = comb_rank_sb<I>(sb, s[k]);
+    acc[k] = make_double2(0.0, 0.0);
+  }
+  const int nt = A.n_off;
+  for (int t0 = 0; t0 < nt; t0 += T) {
+    bool ok[T][RPL];
+    int64_t c[T][RPL];
+    bool any = false;
+#pragma unroll
+    for (int b = 0; b < T; ++b) {
+      const bool tv = t0 + b < nt;
+      const TermDev* __restrict__ Tm = A.off + (tv ? t0 + b : nt - 1);
+      const I rq_m = (I)Tm->req_mask, rq_v = (I)Tm->req_val, rq_v2 = (I)Tm->req_val2, fl = (I)Tm->flip;
+      const bool has2 = Tm->has2 != 0;
+      const int lo = Tm->lo, hi = Tm->hi;
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const I sk = s[k];
+        const I v = sk & rq_m;
+        const I s2 = sk ^ fl;
+        const bool a = tv && live[k] && (v == rq_v || (has2 && v == rq_v2)) && popc<I>(s2) == Np;
+        int64_t q = 0;
+        if (a) {
+          if (hi == lo + 1) {
+            const int64_t dl = sb[lo * 33 + popc<I>(sk & ((((I)1) << lo) - (I)1))];
+            q = ((sk >> lo) & (I)1) ? ra[k] + dl : ra[k] - dl;
+          } else {
+            q = ra[k] - window_rank<I>(sb, sk, lo, hi) + window_rank<I>(sb, s2, lo, hi);
+          }
+        }
+        ok[b][k] = a;
+        c[b][k] = last - q;
+        any = any || a;
+      }
+    }
+    if (!__any(any)) continue;
+    double2 x[T][RPL];
+#pragma unroll
+    for (int b = 0; b < T; ++b)
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        x[b][k] = make_double2(0.0, 0.0);
+        if (ok[b][k]) x[b][k] = VTr<VT>::ld(vin + c[b][k]);
+      }
+#pragma unroll
+    for (int b = 0; b < T; ++b) {
+      if (t0 + b >= nt) break;
+      const TermDev* __restrict__ Tm = A.off + t0 + b;
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        if (!ok[b][k]) continue;
+        const double w = (popc<I>(s[k] & (I)Tm->smask) & 1) ? -1.0 : 1.0;
+        if constexpr (CPLX) {
+          double2 coef = make_double2(Tm->cr * w, Tm->ci * w);
+          if (A.off_conj) coef = cconj(coef);
+          acc[k] = cadd(acc[k], cmul(coef, x[b][k]));
+        } else {
+          acc[k].x += Tm->cr * w * x[b][k].x;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < RPL; ++k) {
+    if (!live[k]) continue;
+    const int64_t r = rr[k];
+    double2 d = make_double2(0.0, 0.0);
+    if (A.diag_arr) {
+      if (A.diag_arr_cplx) d = ((const double2*)A.diag_arr)[r];
+      else d.x = ((const double*)A.diag_arr)[r];
+    }
+    const double2 dsum = cadd(cmul(d, A.alpha_diag), A.shift);
+    const double2 xr = VTr<VT>::ld(vin + r);
+    double2 res;
+    if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc[k]), cmul(dsum, xr));
+    else res = make_double2(A.alpha_off.x * acc[k].x + dsum.x * xr.x, 0.0);
+    if (A.beta) res = cadd(res, VTr<VT>::ld(vout + r));
+    VTr<VT>::st(vout + r, res);
+  }
+}
+
 // Off-diagonal sums of RPL rows (states s, ranks ra, 1/sqrt(n) rsq) over all
 // terms of A, the terms in batches of B: first the B ranks, then the B table
 // reads (TW-byte entries; TW = 0: direct rank, no table), then the B gathers
@@ -1145,6 +1255,13 @@ void launch_hv_one(const HvArgs& A, hipStream_t st) {
     hipLaunchKernelGGL((k_hv_rows_g<I, VT, M, FERM, LK, 4>), dim3((unsigned)qsx_grid(A.Ns * 4, blk)), dim3(blk), 0, st, A);
   else if (A.nvec == 1 && A.sched == 4)
     hipLaunchKernelGGL((k_hv_rows_g<I, VT, M, FERM, LK, 8>), dim3((unsigned)qsx_grid(A.Ns * 8, blk)), dim3(blk), 0, st, A);
+  else if (A.nvec == 1 && A.fast_rows && (A.sched == 8 || A.sched == 9) && !FERM && LK == LOOKUP_RANK && M == 0) {
+    if constexpr (!FERM && LK == LOOKUP_RANK && M == 0) {
+      const dim3 g2((unsigned)qsx_grid(A.Ns, 2 * blk));
+      if (A.sched == 8) hipLaunchKernelGGL((k_hv_rows_fast_chunk<I, VT, 2, 4>), g2, dim3(blk), 0, st, A);
+      else hipLaunchKernelGGL((k_hv_rows_fast_chunk<I, VT, 2, 8>), g2, dim3(blk), 0, st, A);
+    }
+  }
   else if (A.nvec == 1 && A.fast_rows && (A.sched == 0 || A.sched == 5 || A.sched == 6) && !FERM &&
            (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
     if constexpr (!FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK))

@@ -25,7 +25,8 @@ from . import _lib as L
 _mem = {}
 
 # schedule name -> launch code of qsx_hv_ctx_set_schedule
-SCHEDULES = {"rows": 0, "queue": 1, "rows2": 2, "rows4": 3, "rows8": 4, "batch4": 5, "batch8": 6, "split": 7}
+SCHEDULES = {"rows": 0, "queue": 1, "rows2": 2, "rows4": 3, "rows8": 4, "batch4": 5, "batch8": 6, "split": 7,
+             "chunk4": 8, "chunk8": 9}
 
 
 def _cache_path():
@@ -92,6 +93,8 @@ def candidates(op, vdt, nvec):
             scheds += ["batch4", "batch8"]
         if lk == "search" and _split_layout(op):
             scheds.append("split")
+        if lk == "search" and _rank_lookup(op) and not len(b._pers):
+            scheds += ["chunk4", "chunk8"]
         out += [{"lookup": lk, "schedule": sc} for sc in scheds]
     return out
 

@@ -323,10 +323,11 @@ def test_merged_term_pairs_vs_oracle(lookup, monkeypatch):
 
 
 @pytest.mark.parametrize("lookup", ["table", "search"])
-@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8"])
+@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8", "chunk4", "chunk8"])
 def test_fast_rows_pauli_strings_vs_oracle(lookup, schedule, monkeypatch):
     """The fast-row kernels (rows: one term at a time with the direct rank,
-    batches sized from the term count with the table; batches of 4 and 8)
+    batches sized from the term count with the table; batches of 4 and 8;
+    chunks of 4 and 8 terms with predicated gathers for the direct rank)
     on x / y / z strings: odd and even numbers of y (factors i^(3 ny)), z
     factors on flipped and unflipped sites, complex couplings, a term count
     that is not a multiple of the batch, on a k-sector (sector table) and on
