@@ -201,7 +201,7 @@ def main():
         if "c3" in which:
             expm["precision_L28"] = run_precision(args.c3_steps, cpu=not args.no_cpu)
         if "c4" in which:
-            expm["batched_L26"] = run_batched()
+            expm["batched_L26"] = run_batched(cpu=not args.no_cpu)
         if "c5" in which:
             expm["learned_hubbard_L14"] = run_learned(args.c5_steps, runs=args.c5_runs, graph=not args.no_graph,
                                                       cpu=not args.no_cpu)
@@ -458,7 +458,34 @@ def _cpu_taylor_step(Ed, Ns, i0):
                       f"max|GPU-CPU| after the step = {err:.2e}"}
 
 
-def run_batched(L=26, nv=64, t=2.0, plain_sectors=2):
+def _cpu_sector_expm(op, V0, a, n_sectors, nv, n_cpu=2):
+    """The C/OpenMP complex128 CSR Taylor loop on the host for `n_cpu` of the
+    first sector's vectors (same partition, shift and tolerance as the
+    device), scaled to all sectors and vectors; compared with the device."""
+    import time
+    from quspin_amd import _lib
+    from quspin_amd.tools.evolution import expm_multiply_parallel
+    from oracle import c_oracle
+    E = expm_multiply_parallel(op, a=a, dtype=np.complex128)
+    A = E._op.to_csr(np.complex128).to_scipy()
+    m_star, s = (int(x) for x in E.partition)
+    Vh = _lib.to_host(V0[:, :n_cpu]).astype(np.complex128)
+    t0 = time.perf_counter()
+    F = np.stack([c_oracle.expm_c128(A, Vh[:, j], s, m_star, float(E._tol), complex(E._mu), a)
+                  for j in range(n_cpu)], axis=1)
+    per_vec = (time.perf_counter() - t0) / n_cpu
+    G = _lib.to_host(E.dot(V0[:, :n_cpu].contiguous().to(_lib.torch_dtype(np.complex128))))
+    err = float(np.abs(G - F).max())
+    total = per_vec * nv * n_sectors
+    return {"value": round(1.0 / total, 6), "unit": "batched expm steps/s", "cores": c_oracle.max_threads(),
+            "kind": "port",
+            "sample": f"{n_cpu} vectors of sector k=0 (partition m*={m_star}, s={s}) through the C/OpenMP complex128 "
+                      f"CSR Taylor loop (oracle/csrc/qsx_oracle.c qo_expm_c128), {per_vec:.2f} s each, scaled by "
+                      f"{nv} vectors x {n_sectors} sectors (estimate: {total:.0f} s per step); "
+                      f"max|GPU-CPU| = {err:.2e}"}
+
+
+def run_batched(L=26, nv=64, t=2.0, plain_sectors=2, cpu=True):
     """Heisenberg L=26, Sz=0, all 26 momentum sectors x 64 seeded states:
     batched expm to t=2 (one launch per Taylor term for every sector, hipGraph
     replay, autotuned form and schedule) in complex64 and complex128; the
@@ -534,6 +561,8 @@ def run_batched(L=26, nv=64, t=2.0, plain_sectors=2):
         torch.cuda.synchronize()
         tp += e0.elapsed_time(e1) / 1000.0
     plain = tp * L / plain_sectors
+    if cpu:
+        out["cpu_baseline"] = _cpu_sector_expm(ops[np.complex128][0], V[:bases[0].Ns], -1j * t, L, nv)
     out["plain_fp64_s_est"] = round(plain, 3)
     out["plain_fp64_sample"] = (f"estimate: {plain_sectors} of {L} sectors timed ({tp:.3f} s), scaled by "
                                 f"{L}/{plain_sectors} (the sectors hold 400,023 or 400,024 states each)")
