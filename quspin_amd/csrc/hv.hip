@@ -906,10 +906,11 @@ extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_
 // 7: split-chain blocks (one hard-core-boson sector without symmetry maps;
 // the plan is built on the first request and owned by the context);
 // 8/9: direct-rank rows with the terms in chunks of 4/8, gathers predicated
-// (a sector without symmetry maps; other products use schedule 0).
+// (a sector without symmetry maps; other products use schedule 0);
+// 10: batches of 14 terms where the fast-row kernels apply (as 6 elsewhere).
 // Single vector; several vectors always use one lane per row.
 extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
-  if (!X || sched < 0 || sched > 9) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
+  if (!X || sched < 0 || sched > 10) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
   if (sched == 7 && !X->split_d) {
     const int rc = split_build(X);
     if (rc) return rc;

@@ -1180,7 +1180,7 @@ __global__ void __launch_bounds__(256) k_hv_scatter(HvArgs A) {
 
 // Fast-row launch.  RPL = 2 rows per lane for the direct rank (no table
 // reads); with the table one row per lane keeps more waves in flight for the
-// lookups.  nb terms per batch (1: k_hv_rows_fast; 4..8:
+// lookups.  nb terms per batch (1: k_hv_rows_fast; 4..8 or 14:
 // k_hv_rows_fast_batch); nb = 0 picks the default measured on MI355X:
 // batches (fast_batch_default) with the table, 1 for the direct rank, where
 // the branch-free batches lose to skipping the gathers of terms that give a
@@ -1197,6 +1197,7 @@ void launch_hv_fast_l(const HvArgs& A, int nb, size_t smem, hipStream_t st) {
     case 6: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 6, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
     case 7: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 7, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
     case 8: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 8, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
+    case 14: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 14, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
     default: hipLaunchKernelGGL((k_hv_rows_fast_batch<I, VT, M, LK, RPL, 4, TW, LUTL>), grid, dim3(blk), smem, st, A); break;
   }
 }
@@ -1242,7 +1243,8 @@ void launch_hv_fast(const HvArgs& A, int nb, hipStream_t st) {
 
 // Dispatch over (maps, fermion sign, lookup, schedule, NV).
 // schedule codes: 0 one lane per row, 1 compacted queue, 2/3/4 = 2/4/8 lanes
-// per row, 5/6 = batches of 4/8 terms (rank-based lookups)
+// per row, 5/6 = batches of 4/8 terms (rank-based lookups), 10 = batches of
+// 14 terms on the fast-row path (elsewhere as 6)
 template <typename I, typename VT, int M, bool FERM, int LK>
 void launch_hv_one(const HvArgs& A, hipStream_t st) {
   const int blk = 256;
@@ -1262,12 +1264,12 @@ void launch_hv_one(const HvArgs& A, hipStream_t st) {
       else hipLaunchKernelGGL((k_hv_rows_fast_chunk<I, VT, 2, 8>), g2, dim3(blk), 0, st, A);
     }
   }
-  else if (A.nvec == 1 && A.fast_rows && (A.sched == 0 || A.sched == 5 || A.sched == 6) && !FERM &&
+  else if (A.nvec == 1 && A.fast_rows && (A.sched == 0 || A.sched == 5 || A.sched == 6 || A.sched == 10) && !FERM &&
            (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
     if constexpr (!FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK))
-      launch_hv_fast<I, VT, M, LK>(A, A.sched == 0 ? 0 : A.sched == 5 ? 4 : 8, st);
+      launch_hv_fast<I, VT, M, LK>(A, A.sched == 0 ? 0 : A.sched == 5 ? 4 : A.sched == 6 ? 8 : 14, st);
   }
-  else if (A.nvec == 1 && (A.sched == 5 || A.sched == 6) && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
+  else if (A.nvec == 1 && (A.sched == 5 || A.sched == 6 || A.sched == 10) && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
     if constexpr (LK == LOOKUP_TABLE || LK == LOOKUP_RANK) {
       if (A.sched == 5) hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 4>), grid, dim3(blk), 0, st, A);
       else hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 8>), grid, dim3(blk), 0, st, A);

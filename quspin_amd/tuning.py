@@ -5,9 +5,10 @@ For a matrix-free operator the tuner chooses the representative lookup
 table in HBM) and the schedule ("rows": one lane per row; "rows2/4/8": that
 many lanes per row with shuffle reduction; "queue": compacted LDS work queue;
 "batch4/8": terms in batches with all lookups of a batch in flight, for
-rank-based lookups; "split": blocks of the split-chain layout of one
-hard-core-boson sector without symmetry maps; all but "rows" for one vector
-only).  Candidates are timed on the operator's own
+rank-based lookups; "batch14": the same with 14 terms per batch, for the
+table; "split": blocks of the split-chain layout of one hard-core-boson
+sector without symmetry maps; "chunk4/8": direct-rank rows with the terms in
+chunks and predicated gathers; all but "rows" for one vector only).  Candidates are timed on the operator's own
 basis with HIP events (a few launches each) and the winner is cached per
 (basis signature, dtype, nvec) in memory and in a JSON file
 (``QSX_TUNE_CACHE``, default ``~/.cache/quspin_amd/tune.json``).
@@ -26,7 +27,7 @@ _mem = {}
 
 # schedule name -> launch code of qsx_hv_ctx_set_schedule
 SCHEDULES = {"rows": 0, "queue": 1, "rows2": 2, "rows4": 3, "rows8": 4, "batch4": 5, "batch8": 6, "split": 7,
-             "chunk4": 8, "chunk8": 9}
+             "chunk4": 8, "chunk8": 9, "batch14": 10}
 
 
 def _cache_path():
@@ -91,6 +92,8 @@ def candidates(op, vdt, nvec):
         scheds = ["rows", "rows2", "rows4", "queue"]
         if lk == "table" or _rank_lookup(op):
             scheds += ["batch4", "batch8"]
+        if lk == "table":
+            scheds.append("batch14")
         if lk == "search" and _split_layout(op):
             scheds.append("split")
         if lk == "search" and _rank_lookup(op) and not len(b._pers):
