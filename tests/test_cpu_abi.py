@@ -208,3 +208,28 @@ def test_bench_refuses_multi_gpu():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_spin_basis_1d_argument_checks():
+    """spin_basis_1d refuses what the reference refuses (unexpected keys,
+    spin inversion outside the zero-magnetisation sector, sublattice
+    inversion with a magnetisation sector, parity with sublattice inversion)
+    and the momentum-parity sectors at 0 < k < pi that the 1D core builds
+    from sigma = +-1 real states; all before any device work."""
+    from quspin_amd.basis import spin_basis_1d
+    with pytest.raises(ValueError):
+        spin_basis_1d(6, Nup=3, qblock=1)
+    with pytest.raises(ValueError):
+        spin_basis_1d(6, Nup=2, zblock=1)
+    with pytest.raises(ValueError):
+        spin_basis_1d(6, Nup=3, zAblock=1)
+    with pytest.raises(ValueError):
+        spin_basis_1d(6, pblock=1, zAblock=1)
+    with pytest.raises(ValueError):
+        spin_basis_1d(6, Nup=3, pblock=2)
+    with pytest.raises(TypeError):
+        spin_basis_1d(6, Nup=3, kblock=1.0)
+    for blocks in ({"kblock": 1, "pblock": 1}, {"kblock": 2, "pzblock": -1}, {"kblock": 1, "a": 2, "pblock": 1},
+                   {"kblock": -1, "pblock": 1}):
+        with pytest.raises(NotImplementedError):
+            spin_basis_1d(8, **blocks)
