@@ -150,13 +150,16 @@ __global__ void __launch_bounds__(256) k_kron(KronArgs K) {
 }
 
 // Row pass of the two-pass form: Z[u, :] = X[u, :] H_down^T.  A block stages
-// R up rows in LDS (grid-stride over row groups); each wave takes whole
-// 64-row chunks of H_down, so the chunk's width and offset are wave-uniform,
-// the sliced-ELL entries load coalesced, and each entry serves the R rows.
-// With EPI the launch runs the term's prologue, so that a finished series
-// skips both passes.
+// R up rows in LDS (grid-stride over row groups; the staging loads are issued
+// four per thread before their LDS stores); each wave takes whole 64-row
+// chunks of H_down, so the chunk's width and offset are wave-uniform, the
+// sliced-ELL entries load coalesced, and each entry serves the R rows.  The
+// staged form runs 1024-thread blocks: many waves share one staged group, so
+// a CU keeps 16-32 waves in flight while the H_down entries, which every row
+// group reads again from L2, are shared by R rows.  With EPI the launch runs
+// the term's prologue, so that a finished series skips both passes.
 template <typename MT, typename VT, bool EPI, bool STAGE, int R>
-__global__ void __launch_bounds__(256) k_kron_down(KronArgs K) {
+__global__ void __launch_bounds__(1024) k_kron_down(KronArgs K) {
   static_assert(STAGE || R == 1, "direct reads take one row at a time");
   extern __shared__ unsigned char kron_smem[];
   VT* srow = (VT*)kron_smem;
@@ -175,7 +178,17 @@ __global__ void __launch_bounds__(256) k_kron_down(KronArgs K) {
     const int nr = (int)(Nu - u0 < R ? Nu - u0 : R);
     const VT* __restrict__ xu = X + u0 * Nd;
     if constexpr (STAGE) {
-      for (int64_t i = threadIdx.x; i < nr * Nd; i += blockDim.x) srow[i] = xu[i];
+      constexpr int SU = 4;
+      const int64_t n = nr * Nd, bd = blockDim.x;
+      int64_t i = threadIdx.x;
+      for (; i + (SU - 1) * bd < n; i += SU * bd) {
+        VT t[SU];
+#pragma unroll
+        for (int q = 0; q < SU; ++q) t[q] = xu[i + q * bd];
+#pragma unroll
+        for (int q = 0; q < SU; ++q) srow[i + q * bd] = t[q];
+      }
+      for (; i < n; i += bd) srow[i] = xu[i];
       __syncthreads();
     }
     for (int64_t d0 = w0; d0 < Nd; d0 += blockDim.x) {
@@ -321,20 +334,44 @@ __global__ void __launch_bounds__(256) k_kron_up(KronArgs K, int64_t nunits, int
       }
     }
     if (!dok) continue;
+    // epilogue (spmv_out's arithmetic): every row's operands are requested
+    // before the first store, so the rows' loads overlap instead of each row
+    // waiting behind the previous row's stores (rows past nr load row u0)
+    double2 xr[RW], fo[RW];
+    double gd[RW], dx[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int64_t row = (u0 + (r < nr ? r : 0)) * Nd + d;
+      xr[r] = VTr<VT>::ld(X + row);
+      gd[r] = K.sdiag ? K.sdiag[row] : 0.0;
+      dx[r] = A.xdiag ? A.xdiag[row] : 0.0;
+      if constexpr (EPI) fo[r] = VTr<FT>::ld((const FT*)A.F + row);
+    }
+    VT* __restrict__ y = (VT*)A.y;
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
       if (r >= nr) break;
       const int64_t row = (u0 + r) * Nd + d;
-      const double2 xr = VTr<VT>::ld(X + row);
-      if (K.sdiag) {
-        const double g = K.sdiag[row];
-        sr[r] += g * xr.x;
-        si[r] += g * xr.y;
+      const double ar = sr[r] + gd[r] * xr[r].x, ai = si[r] + gd[r] * xr[r].y;
+      const double2 mu_r = make_double2(mu_t.x - xs.x * dx[r], mu_t.y - xs.y * dx[r]);
+      double2 res;
+      if constexpr (VTr<VT>::cplx) {
+        const double2 acc = make_double2(ar - (mu_r.x * xr[r].x - mu_r.y * xr[r].y),
+                                         ai - (mu_r.x * xr[r].y + mu_r.y * xr[r].x));
+        res = cmul(a_t, acc);
+      } else {
+        res = make_double2(a_t.x * (ar - mu_r.x * xr[r].x), 0.0);
       }
-      double curB = 0.0, curF = 0.0;
-      spmv_out<VT, EPI, FT>(A, row, row, make_double2(sr[r], si[r]), xr, a_t, mu_t, xs, curB, curF);
-      maxB = fmax(maxB, curB);
-      maxF = fmax(maxF, curF);
+      if (A.beta) res = cadd(res, VTr<VT>::ld(y + row));
+      VTr<VT>::st(y + row, res);
+      if constexpr (EPI) {
+        // the stored (rounded) B enters F, as in the reference
+        const double2 rs = VTr<VT>::rnd(res);
+        const double2 fs = VTr<FT>::rnd(cadd(fo[r], rs));
+        VTr<FT>::st((FT*)A.F + row, fs);
+        maxB = fmax(maxB, VTr<VT>::cplx ? cabs2(rs) : fabs(rs.x));
+        maxF = fmax(maxF, VTr<VT>::cplx ? cabs2(fs) : fabs(fs.x));
+      }
     }
   }
   if constexpr (EPI) block_norm_partials(A, maxB, maxF);
@@ -363,19 +400,19 @@ static void launch_kron_t(const KronArgs& KA, const qsx_kron* K, hipStream_t st)
   const bool staged = stage <= KRON_STAGE_BYTES;
   const dim3 blk(256);
   if (kron_two_pass(K, sizeof(VT))) {
-    // rows staged per row-pass block: up to 32 KB of LDS per block (five
-    // blocks per CU); more rows per block at lower occupancy measured slower
-    const int R = !staged ? 1 : (4 * stage <= 32 * 1024) ? 4 : (2 * stage <= 32 * 1024) ? 2 : 1;
+    // staged row pass: 1024-thread blocks with two rows each while they take
+    // at most 128 KB of LDS (measured at L=14, (7,7): complex64 337 -> 369
+    // and complex128 147 -> 182 driven steps/s against 256-thread blocks of
+    // one row; 768 threads within 1 %, three or four rows and one row slower)
+    const int R = !staged ? 1 : (2 * stage <= 128 * 1024) ? 2 : 1;
     const int64_t ng = (K->Nu + R - 1) / R;
     const dim3 gd((unsigned)(ng < KRON_MAX_BLOCKS ? ng : KRON_MAX_BLOCKS));
     if (!staged)
       hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, false, 1>), gd, blk, 0, st, KA);
-    else if (R == 4)
-      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, true, 4>), gd, blk, (size_t)(4 * stage), st, KA);
     else if (R == 2)
-      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, true, 2>), gd, blk, (size_t)(2 * stage), st, KA);
+      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, true, 2>), gd, dim3(1024), (size_t)(2 * stage), st, KA);
     else
-      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, true, 1>), gd, blk, (size_t)stage, st, KA);
+      hipLaunchKernelGGL((k_kron_down<MT, VT, EPI, true, 1>), gd, dim3(1024), (size_t)stage, st, KA);
     hipLaunchKernelGGL((k_kron_up<MT, VT, EPI, FT>), dim3((unsigned)kron_blocks(K, sizeof(VT))), blk, 0, st, KA,
                        kron_up_units(K), kron_up_groups(K), K->ind_u, (const MT*)K->dat_u);
     return;

@@ -88,6 +88,29 @@ def test_kron_product_matches_oracle(dtype, cplx, tol, mode):
                                atol=tol * np.abs(ref @ x).max())
 
 
+@pytest.mark.parametrize("dtype,tol", [(np.complex128, 1e-12), (np.complex64, 3e-6)])
+def test_kron_two_pass_odd_up_rows(dtype, tol):
+    """The row pass stages two up rows per block: an odd number of up rows
+    (Nf=(3,2) on 7 sites: 35 x 21) leaves the last group with one row."""
+    from quspin_amd import _lib as L
+    from quspin_amd.basis import spinful_fermion_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    from quspin_amd.operators._kron import KronOp
+    Lh = 7
+    B = spinful_fermion_basis_general(Lh, Nf=(3, 2))
+    st = _model(Lh, complex_hop=True)
+    K = KronOp(quantum_LinearOperator(st, basis=B, dtype=np.complex128, **_KW), dtype)
+    K.set_mode(2)
+    print(f"case: L={Lh} Nf=(3,2) dtype={np.dtype(dtype).name} Nu={K.Nu} Nd={K.Nd}", flush=True)
+    assert (K.Nu, K.Nd) == (35, 21)
+    ref = _oracle_H(B, st, Lh)
+    x = _vec(B.Ns, dtype, 5)
+    Y = L.empty(B.Ns, dtype)
+    K.apply(L.to_device(x), Y)
+    e = ref @ x.astype(np.complex128)
+    np.testing.assert_allclose(L.to_host(Y), e, rtol=0, atol=tol * np.abs(e).max())
+
+
 @pytest.mark.parametrize("mode", [1, 2])
 def test_kron_unstaged_rows_match_oracle(mode):
     """A down sector wider than the LDS row stage (C(15,7) complex128 rows:
