@@ -166,6 +166,8 @@ struct qsx_csr_sum {
   int64_t* offs = nullptr;
 };
 
+extern "C" void qsx_csr_sum_destroy(qsx_csr_sum* H);
+
 // sum_k coef_k op_k(A_k) of K canonical CSR matrices with Nr rows, op_k the
 // identity or (conjs[k] = 1) the complex conjugate; coefs holds K (re, im)
 // pairs.  Computes the row counts and returns nnz; qsx_csr_sum_finish writes
@@ -194,26 +196,38 @@ extern "C" int qsx_csr_sum_create(int K, const void* const* indptrs, const void*
     H->S.coef[k] = make_double2(coefs[2 * k], coefs[2 * k + 1]);
   }
   H->Nr = Nr;
-  *out = H;
+  *out = nullptr;
   *nnz = 0;
-  if (Nr == 0) return QSX_OK;
+  if (Nr == 0) {
+    *out = H;
+    return QSX_OK;
+  }
+  // every failure below frees what was allocated and returns no handle
   int64_t* cnt = nullptr;
   void* tmp = nullptr;
   size_t tb = 0;
-  QSX_CHECK_HIP(hipMalloc(&cnt, sizeof(int64_t) * Nr));
-  QSX_CHECK_HIP(hipMalloc(&H->offs, sizeof(int64_t) * Nr));
-  hipLaunchKernelGGL(k_sum_count, dim3(grid_for(Nr)), dim3(256), 0, st, H->S, Nr, cnt);
-  QSX_LAUNCH_CHECK();
-  QSX_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, H->offs, (int)Nr, st));
-  QSX_CHECK_HIP(hipMalloc(&tmp, tb + 16));
-  QSX_CHECK_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, H->offs, (int)Nr, st));
   int64_t lc = 0, lo = 0;
-  QSX_CHECK_HIP(hipMemcpyAsync(&lc, cnt + Nr - 1, 8, hipMemcpyDeviceToHost, st));
-  QSX_CHECK_HIP(hipMemcpyAsync(&lo, H->offs + Nr - 1, 8, hipMemcpyDeviceToHost, st));
-  QSX_CHECK_HIP(hipStreamSynchronize(st));
+  hipError_t e = hipMalloc(&cnt, sizeof(int64_t) * Nr);
+  if (e == hipSuccess) e = hipMalloc(&H->offs, sizeof(int64_t) * Nr);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_sum_count, dim3(grid_for(Nr)), dim3(256), 0, st, H->S, Nr, cnt);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(nullptr, tb, cnt, H->offs, (int)Nr, st);
+  if (e == hipSuccess) e = hipMalloc(&tmp, tb + 16);
+  if (e == hipSuccess) e = hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, H->offs, (int)Nr, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&lc, cnt + Nr - 1, 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(&lo, H->offs + Nr - 1, 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
   (void)hipFree(tmp);
   (void)hipFree(cnt);
+  if (e != hipSuccess) {
+    qsx_set_error(std::string("qsx_csr_sum_create: ") + hipGetErrorString(e));
+    qsx_csr_sum_destroy(H);
+    return QSX_ERR_HIP;
+  }
   H->nnz = lc + lo;
+  *out = H;
   *nnz = H->nnz;
   return QSX_OK;
 }
