@@ -710,23 +710,29 @@ struct qsx_hv_ctx {
   const qsx_oplist_full* L;
   int dtype, transposed, conjugated;
   uint64_t owner;  // id of the captures that hold A (renewed when A changes)
-  // split-chain plan of schedule 7 (device allocations owned here)
+  // split-chain plan of schedules 7 and 11 (device allocations owned here;
+  // split_z: the plan holds the row-pass buffer of schedule 11)
   std::vector<void*> split_bufs;
   SplitDev* split_d = nullptr;
+  bool split_z = false;
+  int split_nxt = 0;  // up + crossing terms of the plan
 };
 
 static void split_free(qsx_hv_ctx* X) {
   for (void* p : X->split_bufs) (void)hipFree(p);
   X->split_bufs.clear();
   X->split_d = nullptr;
+  X->split_z = false;
   X->A.split = nullptr;
   X->A.split_lds = 0;
   X->A.split_nblk = 0;
+  X->A.split_nunits = 0;
 }
 
 // Build the split-chain plan of a context (hard-core-boson sector of N <= 32
-// bits, no symmetry maps, every off-diagonal term in the bit-parallel form).
-static int split_build(qsx_hv_ctx* X) {
+// bits, no symmetry maps, every off-diagonal term in the bit-parallel form);
+// with_z also allocates the row-pass buffer of the two-pass form.
+static int split_build(qsx_hv_ctx* X, bool with_z) {
   const qsx_core* c = X->c;
   const qsx_oplist_full* L = X->L;
   const int N = c->h.N;
@@ -780,6 +786,19 @@ static int split_build(qsx_hv_ctx* X) {
     if (la != lb) return la > lb;
     return __builtin_popcount((uint32_t)a) < __builtin_popcount((uint32_t)b);
   });
+  // the two-pass form's order: by class, u descending inside a class, and
+  // its column-strip units
+  std::vector<int32_t> ord2;
+  std::vector<int32_t> ocls(K + 2, 0);
+  std::vector<int64_t> uoff(K + 2, 0);
+  for (int q = 0; q <= K; ++q) {
+    ocls[q] = (int32_t)ord2.size();
+    for (int64_t u = ((int64_t)1 << U) - 1; u >= 0; --u)
+      if (Np - __builtin_popcountll((uint64_t)u) == q) ord2.push_back((int32_t)u);
+    const int64_t cnt = (int64_t)ord2.size() - ocls[q];
+    uoff[q + 1] = uoff[q] + ((binom(K, q) + 63) / 64) * ((cnt + SPLIT2_UNIT_U - 1) / SPLIT2_UNIT_U);
+  }
+  ocls[K + 1] = (int32_t)ord2.size();
   // terms of the gather form this context uses, by the bits they flip and read
   const std::vector<TermDev>& terms = X->transposed ? L->base.off_h : L->off_hc_h;
   std::vector<TermDev> ell, low, up, upu, cross;
@@ -826,6 +845,50 @@ static int split_build(qsx_hv_ctx* X) {
     ell_off[q + 1] = ell_off[q] + (int64_t)km * nq;
   }
   if (lut.size() > 1024) { qsx_set_error("split-chain schedule: more than 1024 distinct coefficients"); return QSX_ERR_UNSUPPORTED; }
+  // two-pass form: per u of ord2, its first row and its applying upper-half
+  // terms that read upper bits only (partner first row, coefficient)
+  std::vector<int64_t> obase(ord2.size());
+  std::vector<int32_t> upn(ord2.size(), 0);
+  std::vector<std::vector<std::pair<int64_t, double2>>> upl(ord2.size());
+  int upw = 1;
+  for (size_t i = 0; i < ord2.size(); ++i) {
+    const uint32_t u = (uint32_t)ord2[i];
+    obase[i] = ublk[u];
+    const uint64_t su = (uint64_t)u << K;
+    for (const TermDev& T : upu) {
+      const uint64_t v = su & T.req_mask;
+      if (v != T.req_val && !(T.has2 && v == T.req_val2)) continue;
+      const uint32_t u2 = u ^ (uint32_t)(T.flip >> K);
+      if (__builtin_popcount(u2) != __builtin_popcount(u)) continue;
+      if (ublk[u2] < 0) continue;
+      const double w = (__builtin_popcountll(su & T.smask) & 1) ? -1.0 : 1.0;
+      double2 cf = make_double2(T.cr * w, T.ci * w);
+      if (oconj) cf.y = -cf.y;
+      upl[i].push_back({ublk[u2], cf});
+    }
+    upn[i] = (int32_t)upl[i].size();
+    upw = std::max(upw, (int)upl[i].size());
+  }
+  // the up and crossing terms in compact form, and their partner blocks per u
+  std::vector<SplitXT> xts;
+  for (const std::vector<TermDev>* lst : {&up, &cross})
+    for (const TermDev& T : *lst) {
+      SplitXT x;
+      x.flip = (uint32_t)T.flip; x.rm = (uint32_t)T.req_mask; x.rv = (uint32_t)T.req_val;
+      x.rv2 = (uint32_t)T.req_val2; x.sm = (uint32_t)T.smask; x.has2 = T.has2; x.cr = T.cr; x.ci = T.ci;
+      xts.push_back(x);
+    }
+  const int nxt = (int)xts.size();
+  std::vector<int64_t> xb(std::max<size_t>(ord2.size() * (size_t)nxt, 1), -1);
+  for (size_t i = 0; i < ord2.size(); ++i)
+    for (int t = 0; t < nxt; ++t) xb[i * nxt + t] = ublk[(uint32_t)ord2[i] ^ (xts[t].flip >> K)];
+  std::vector<int64_t> upb(ord2.size() * (size_t)upw, -1);
+  std::vector<double2> upc(ord2.size() * (size_t)upw, make_double2(0.0, 0.0));
+  for (size_t i = 0; i < ord2.size(); ++i)
+    for (size_t t = 0; t < upl[i].size(); ++t) {
+      upb[i * upw + t] = upl[i][t].first;
+      upc[i * upw + t] = upl[i][t].second;
+    }
   std::vector<uint32_t> ellv((size_t)std::max<int64_t>(ell_off[K + 1], 1), 0u);
   for (int q = 0; q <= K; ++q) {
     const int nq = cls_off[q + 1] - cls_off[q];
@@ -865,6 +928,21 @@ static int split_build(qsx_hv_ctx* X) {
   if (e == hipSuccess) { e = up_copy(cls_off.data(), sizeof(int32_t) * cls_off.size(), &p); H.cls_off = (const int32_t*)p; }
   if (e == hipSuccess) { e = up_copy(ublk.data(), sizeof(int64_t) * ublk.size(), &p); H.ublk = (const int64_t*)p; }
   if (e == hipSuccess) { e = up_copy(order.data(), sizeof(int32_t) * order.size(), &p); H.order = (const int32_t*)p; }
+  if (e == hipSuccess) { e = up_copy(ord2.data(), sizeof(int32_t) * ord2.size(), &p); H.ord2 = (const int32_t*)p; }
+  if (e == hipSuccess) { e = up_copy(ocls.data(), sizeof(int32_t) * ocls.size(), &p); H.ocls = (const int32_t*)p; }
+  if (e == hipSuccess) { e = up_copy(uoff.data(), sizeof(int64_t) * uoff.size(), &p); H.uoff = (const int64_t*)p; }
+  H.nunits = uoff[K + 1];
+  H.upw = upw;
+  if (e == hipSuccess) { e = up_copy(obase.data(), sizeof(int64_t) * obase.size(), &p); H.obase = (const int64_t*)p; }
+  if (e == hipSuccess) { e = up_copy(upn.data(), sizeof(int32_t) * upn.size(), &p); H.upn = (const int32_t*)p; }
+  if (e == hipSuccess) { e = up_copy(upb.data(), sizeof(int64_t) * upb.size(), &p); H.upb = (const int64_t*)p; }
+  if (e == hipSuccess) { e = up_copy(upc.data(), sizeof(double2) * upc.size(), &p); H.upc = (const double2*)p; }
+  if (e == hipSuccess && nxt > 0) { e = up_copy(xts.data(), sizeof(SplitXT) * xts.size(), &p); H.xt = (const SplitXT*)p; }
+  if (e == hipSuccess) { e = up_copy(xb.data(), sizeof(int64_t) * xb.size(), &p); H.xb = (const int64_t*)p; }
+  if (e == hipSuccess && with_z) {
+    e = hipMalloc(&H.z, (size_t)X->A.Ns * esz);
+    if (e == hipSuccess) X->split_bufs.push_back(H.z);
+  }
   if (e == hipSuccess) { e = up_copy(&H, sizeof(H), &p); X->split_d = (SplitDev*)p; }
   if (e != hipSuccess) {
     split_free(X);
@@ -874,6 +952,9 @@ static int split_build(qsx_hv_ctx* X) {
   X->A.split = X->split_d;
   X->A.split_lds = (int)(maxblk(K) * esz);
   X->A.split_nblk = H.nblk;
+  X->A.split_nunits = H.nunits;
+  X->split_z = with_z;
+  X->split_nxt = nxt;
   return QSX_OK;
 }
 
@@ -907,13 +988,20 @@ extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_
 // the plan is built on the first request and owned by the context);
 // 8/9: direct-rank rows with the terms in chunks of 4/8, gathers predicated
 // (a sector without symmetry maps; other products use schedule 0);
-// 10: batches of 14 terms where the fast-row kernels apply (as 6 elsewhere).
+// 10: batches of 14 terms where the fast-row kernels apply (as 6 elsewhere);
+// 11: the split-chain layout in two passes, a row pass over the lower-half
+// terms into a buffer of Ns elements owned by the context, then a
+// column-strip pass over the upper-half and crossing terms (sectors as 7).
 // Single vector; several vectors always use one lane per row.
 extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
-  if (!X || sched < 0 || sched > 10) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
-  if (sched == 7 && !X->split_d) {
-    const int rc = split_build(X);
+  if (!X || sched < 0 || sched > 11) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
+  if ((sched == 7 && !X->split_d) || (sched == 11 && !X->split_z)) {
+    const int rc = split_build(X, sched == 11);
     if (rc) return rc;
+  }
+  if (sched == 11 && X->split_nxt > SPLIT2_MAX_XT) {
+    qsx_set_error("split-chain two-pass schedule: more than 64 terms flip upper bits and read or flip lower bits");
+    return QSX_ERR_UNSUPPORTED;
   }
   X->A.sched = sched;
   ctx_changed(X);

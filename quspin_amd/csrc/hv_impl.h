@@ -122,6 +122,7 @@ struct HvArgs {
   const struct SplitDev* split;  // split-chain plan (schedule 7), device memory
   int split_lds;      // bytes of LDS the split kernel stages (largest block)
   int split_nblk;     // its workgroups (blocks of the layout)
+  int64_t split_nunits;  // column-strip units of the two-pass form (schedule 11)
 };
 
 // Split-chain layout of one hard-core-boson particle sector of N <= 32 bits
@@ -151,7 +152,45 @@ struct SplitDev {
   const int32_t* cls_off; // [K + 2] group offsets into dlist
   const int64_t* ublk;    // [2^(N-K)] first row of the block of u (-1: no such block)
   const int32_t* order;   // [nblk] u of each workgroup (largest blocks first)
+  // two-pass form (schedule 11): ord2 lists the u by class c = Np - popc(u)
+  // ascending, u descending inside a class, class c at ord2[ocls[c] ..
+  // ocls[c + 1]); the column-strip pass has units (class c, strip of 64
+  // positions, group of 16 u), uoff[c] .. uoff[c + 1] strip-major, nunits in
+  // all; z holds the row pass' partial sums (one vector in the vector dtype)
+  const int32_t* ord2;    // [nblk]
+  const int32_t* ocls;    // [K + 2]
+  const int64_t* uoff;    // [K + 2]
+  int64_t nunits;
+  void* z;
+  // per u of ord2 (index i): its first row, and its upper-half terms that
+  // read upper bits only, compacted: upn[i] of them at i * upw .. with the
+  // partner block's first row and the coefficient (conjugation applied)
+  const int64_t* obase;   // [nblk]
+  const int32_t* upn;     // [nblk]
+  const int64_t* upb;     // [nblk * upw]
+  const double2* upc;     // [nblk * upw]
+  int upw;
+  // the up and crossing terms (n_up + n_cross <= SPLIT2_MAX_XT) in compact
+  // form, and per u of ord2 the first row of each term's partner block
+  // (u ^ the term's upper flip; -1: none)
+  const struct SplitXT* xt;  // [n_up + n_cross]
+  const int64_t* xb;         // [nblk * (n_up + n_cross)]
 };
+
+// A term of the column-strip pass in compact form (N <= 32 bits).
+struct SplitXT {
+  uint32_t flip, rm, rv, rv2, sm;
+  int32_t has2;
+  double cr, ci;
+};
+constexpr int SPLIT2_MAX_XT = 64;
+
+// Block index with the blocks of each XCD (workgroups are dispatched
+// round-robin over the 8 XCDs) renumbered into one contiguous run.
+__device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t nb) {
+  const int64_t x = b & 7, pos = b >> 3, q = nb >> 3, r = nb & 7;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + pos;
+}
 
 __device__ __forceinline__ double load_n(const void* p, int nb, int64_t i) {
   return nb == 1 ? (double)((const uint8_t*)p)[i] : nb == 2 ? (double)((const uint16_t*)p)[i]
@@ -1440,6 +1479,282 @@ __global__ void __launch_bounds__(SPLIT_THREADS) k_hv_split(HvArgs A) {
   }
 }
 
+// Two-pass form of the split-chain layout (schedule 11).  Pass 1, one
+// workgroup per block u (largest first): the block is staged in LDS (four
+// loads per thread before their stores) and every row's lower-half terms are
+// summed in FP64 (the class's sliced-ELL matrix, then the lower-flip terms
+// that read upper bits) into Z, stored in the vector dtype.
+constexpr int SPLIT2_ROWS_THREADS = 512;
+constexpr int SPLIT2_UP_ROWS = 4;  // u per wave in the column-strip pass
+constexpr int SPLIT2_UNIT_U = 4 * SPLIT2_UP_ROWS;  // u per column-strip unit (four waves)
+
+// element coefficient of a bit-parallel term on state s (sign from smask)
+template <bool CPLX>
+__device__ __forceinline__ double2 split_coef(const HvArgs& A, const TermDev* __restrict__ T, uint32_t s) {
+  const double w = (__builtin_popcount(s & (uint32_t)T->smask) & 1) ? -1.0 : 1.0;
+  double2 coef = make_double2(T->cr * w, CPLX ? T->ci * w : 0.0);
+  if (CPLX && A.off_conj) coef = cconj(coef);
+  return coef;
+}
+
+__device__ __forceinline__ bool split_applies(const TermDev* __restrict__ T, uint32_t s) {
+  const uint32_t v = s & (uint32_t)T->req_mask;
+  return v == (uint32_t)T->req_val || (T->has2 && v == (uint32_t)T->req_val2);
+}
+
+template <typename VT>
+__global__ void __launch_bounds__(SPLIT2_ROWS_THREADS) k_split2_rows(HvArgs A) {
+  extern __shared__ unsigned char split_smem[];
+  __shared__ int64_t sbk[SPLIT_MAX_K * 33];  // C(n, k) for n < K (the ranks of d)
+  __shared__ double2 lut[SPLIT_MAX_LUT];
+  if (A.skip && *A.skip) return;
+  const SplitDev S = *A.split;
+  const uint32_t u = (uint32_t)S.order[blockIdx.x];
+  const int K = S.K;
+  const int Np = A.R->Np;
+  const int c = Np - __builtin_popcount(u);
+  const int n = S.cls_off[c + 1] - S.cls_off[c];
+  const int64_t base = S.ublk[u];
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ z = (VT*)S.z;
+  VT* xs = (VT*)split_smem;
+  constexpr bool CPLX = VTr<VT>::cplx;
+  {
+    const int bd = blockDim.x;
+    int p = threadIdx.x;
+    for (; p + 3 * bd < n; p += 4 * bd) {
+      const VT t0 = vin[base + p], t1 = vin[base + p + bd], t2 = vin[base + p + 2 * bd], t3 = vin[base + p + 3 * bd];
+      xs[p] = t0;
+      xs[p + bd] = t1;
+      xs[p + 2 * bd] = t2;
+      xs[p + 3 * bd] = t3;
+    }
+    for (; p < n; p += bd) xs[p] = vin[base + p];
+  }
+  if (S.n_low > 0)
+    for (int i = threadIdx.x; i < K * 33; i += blockDim.x) sbk[i] = (&A.R->binom[0][0])[i];
+  for (int i = threadIdx.x; i < S.n_lut; i += blockDim.x) lut[i] = S.lut[i];
+  __syncthreads();
+  const uint16_t* __restrict__ dl = S.dlist + S.cls_off[c];
+  const int ke = S.ell_k[c];
+  const uint32_t* __restrict__ el = S.ell + S.ell_off[c];
+  for (int p = threadIdx.x; p < n; p += blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int k0 = 0; k0 < ke; k0 += 8) {
+      uint32_t ev[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) ev[q] = (k0 + q < ke) ? el[(int64_t)(k0 + q) * n + p] : (uint32_t)p;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const double2 x = VTr<VT>::ld(xs + (ev[q] & 0xFFFFu));
+        const double2 cf = lut[ev[q] >> 16];
+        if constexpr (CPLX) acc = cadd(acc, cmul(cf, x));
+        else acc.x += cf.x * x.x;
+      }
+    }
+    if (S.n_low > 0) {
+      const uint32_t d = dl[p];
+      const uint32_t s = (u << K) | d;
+      const int64_t ra = (int64_t)n - 1 - p;  // ascending rank of d in its class
+      for (int t = 0; t < S.n_low; ++t) {
+        const TermDev* __restrict__ T = S.low + t;
+        if (!split_applies(T, s)) continue;
+        const uint32_t d2 = d ^ (uint32_t)T->flip;
+        if (__builtin_popcount(d2) != c) continue;
+        const int lo = T->lo, hi = T->hi;
+        int64_t rk;
+        if (hi == lo + 1) {
+          const int64_t dl1 = sbk[lo * 33 + __builtin_popcount(d & ((1u << lo) - 1u))];
+          rk = ((d >> lo) & 1u) ? ra + dl1 : ra - dl1;
+        } else {
+          rk = ra - window_rank<uint32_t>(sbk, d, lo, hi) + window_rank<uint32_t>(sbk, d2, lo, hi);
+        }
+        const double2 x = VTr<VT>::ld(xs + (n - 1 - rk));
+        const double2 cf = split_coef<CPLX>(A, T, s);
+        if constexpr (CPLX) acc = cadd(acc, cmul(cf, x));
+        else acc.x += cf.x * x.x;
+      }
+    }
+    VTr<VT>::st(z + base + p, acc);
+  }
+}
+
+// Pass 2, the column-strip pass: unit = (class c, strip of 64 positions, 16
+// u of the class), strip-major and each XCD walking a contiguous run of
+// units, so the strip of every u of the class (C(K, c) u x 64 elements) stays
+// in that XCD's L2 while the upper-half terms read their partner blocks at
+// the same positions.  Wave w takes four u (wave-uniform: their partner
+// blocks and coefficients are scalar), lane l position 64 s + l.  Then the
+// upper-flip terms that read lower bits and the crossing terms (per lane),
+// the diagonal and the output: y = alpha_off (Z + those sums) + (alpha_diag
+// diag + shift) x (+ y).
+__device__ __forceinline__ int64_t rl_i64(int64_t x, int t) {
+  const int lo = __builtin_amdgcn_readlane((int)(x & 0xffffffffll), t);
+  const int hi = __builtin_amdgcn_readlane((int)(x >> 32), t);
+  return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+
+__device__ __forceinline__ double rl_dbl(double x, int t) { return __longlong_as_double(rl_i64(__double_as_longlong(x), t)); }
+
+template <typename VT>
+__global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
+  if (A.skip && *A.skip) return;
+  const SplitDev S = *A.split;
+  constexpr int RW = SPLIT2_UP_ROWS;
+  constexpr bool CPLX = VTr<VT>::cplx;
+  const int K = S.K, Np = A.R->Np;
+  const VT* __restrict__ vin = (const VT*)A.v_in;
+  VT* __restrict__ vout = (VT*)A.v_out;
+  const VT* __restrict__ z = (const VT*)S.z;
+  // class tables and the up / crossing terms in LDS (one round per workgroup)
+  __shared__ int64_t s_uoff[SPLIT_MAX_K + 2];
+  __shared__ int32_t s_cls[SPLIT_MAX_K + 2], s_ocl[SPLIT_MAX_K + 2];
+  __shared__ SplitXT s_xt[SPLIT2_MAX_XT];
+  const int nxt = S.n_up + S.n_cross;
+  if (threadIdx.x < K + 2) {
+    s_uoff[threadIdx.x] = S.uoff[threadIdx.x];
+    s_cls[threadIdx.x] = S.cls_off[threadIdx.x];
+    s_ocl[threadIdx.x] = S.ocls[threadIdx.x];
+  }
+  if (threadIdx.x < nxt) s_xt[threadIdx.x] = S.xt[threadIdx.x];
+  __syncthreads();
+  // the wave index as a scalar, so the rows' bases and coefficients stay scalar
+  const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
+  const uint32_t dmask = (1u << K) - 1u;
+  const int upw = S.upw;
+  for (int64_t unit = xcd_contiguous(blockIdx.x, gridDim.x); unit < S.nunits; unit += gridDim.x) {
+    int c = 0;
+    while (c < K && s_uoff[c + 1] <= unit) ++c;
+    const int n = s_cls[c + 1] - s_cls[c];
+    const int cnt = s_ocl[c + 1] - s_ocl[c];
+    const int64_t ngrp = (cnt + SPLIT2_UNIT_U - 1) / SPLIT2_UNIT_U;
+    const int64_t loc = unit - s_uoff[c];
+    const int64_t sp = loc / ngrp, g = loc - sp * ngrp;
+    const int i0 = (int)(g * SPLIT2_UNIT_U) + w * RW;
+    if (i0 >= cnt) continue;
+    const int nr = cnt - i0 < RW ? cnt - i0 : RW;
+    const int p = (int)(sp * 64) + lane;
+    const bool pok = p < n;
+    const int ps = pok ? p : n - 1;  // lanes past the class read in bounds, store nothing
+    // one round: the position's d, the rows' u, first rows and partner lists
+    // (lane t holds entry t of each row's list)
+    const uint32_t d = S.dlist[s_cls[c] + ps];
+    uint32_t uu[RW];
+    int64_t bs[RW], pb[RW], xbl[RW];
+    int npart[RW];
+    double2 pc[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int64_t ii = (int64_t)s_ocl[c] + i0 + (r < nr ? r : 0);
+      uu[r] = (uint32_t)S.ord2[ii];
+      bs[r] = S.obase[ii];
+      npart[r] = r < nr ? S.upn[ii] : 0;
+      pb[r] = lane < upw ? S.upb[ii * upw + lane] : -1;
+      pc[r] = lane < upw ? S.upc[ii * upw + lane] : make_double2(0.0, 0.0);
+      xbl[r] = lane < nxt ? S.xb[ii * nxt + lane] : -1;
+    }
+    double2 acc[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) acc[r] = VTr<VT>::ld(z + bs[r] + ps);
+    int nmax = 0;
+#pragma unroll
+    for (int r = 0; r < RW; ++r) nmax = npart[r] > nmax ? npart[r] : nmax;
+    // upper-half terms reading upper bits only: partner block, same position,
+    // eight loads in flight
+    for (int t0 = 0; t0 < nmax; t0 += 2) {
+      double2 v[2][RW], cf[2][RW];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          const int t = t0 + h;
+          const bool on = t < npart[r];  // wave-uniform
+          const int tl = on ? t : 0;
+          const int64_t b2 = on ? rl_i64(pb[r], tl) : bs[r];
+          cf[h][r] = on ? make_double2(rl_dbl(pc[r].x, tl), rl_dbl(pc[r].y, tl)) : make_double2(0.0, 0.0);
+          v[h][r] = VTr<VT>::ld(vin + b2 + ps);
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          if constexpr (CPLX) acc[r] = cadd(acc[r], cmul(cf[h][r], v[h][r]));
+          else acc[r].x += cf[h][r].x * v[h][r].x;
+        }
+    }
+    // upper-flip terms reading lower bits (partner position = own position)
+    // and crossing terms (partner position from dpos): per-lane conditions;
+    // for each term the rows' position reads, then their partner reads
+    for (int t = 0; t < nxt; ++t) {
+      const SplitXT T = s_xt[t];
+      const bool upt = t < S.n_up;
+      int64_t a[RW];
+      bool on[RW];
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        const uint32_t s = (uu[r] << K) | d;
+        const uint32_t s2 = s ^ T.flip;
+        const uint32_t vv = s & T.rm;
+        const int64_t b2 = rl_i64(xbl[r], t);  // wave-uniform
+        on[r] = r < nr && pok && b2 >= 0 && (vv == T.rv || (T.has2 && vv == T.rv2)) && __builtin_popcount(s2) == Np;
+        a[r] = upt ? (int64_t)ps : (int64_t)S.dpos[s2 & dmask];
+        a[r] += on[r] ? b2 : bs[r];
+        if (!on[r]) a[r] = bs[r] + ps;
+      }
+      double2 v[RW];
+#pragma unroll
+      for (int r = 0; r < RW; ++r) v[r] = VTr<VT>::ld(vin + a[r]);
+#pragma unroll
+      for (int r = 0; r < RW; ++r) {
+        const uint32_t s = (uu[r] << K) | d;
+        const double wt = on[r] ? ((__builtin_popcount(s & T.sm) & 1) ? -1.0 : 1.0) : 0.0;
+        double2 cf = make_double2(T.cr * wt, CPLX ? T.ci * wt : 0.0);
+        if (CPLX && A.off_conj) cf = cconj(cf);
+        if constexpr (CPLX) acc[r] = cadd(acc[r], cmul(cf, v[r]));
+        else acc[r].x += cf.x * v[r].x;
+      }
+    }
+    if (!pok) continue;
+    double2 xr[RW], dg[RW];
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      const int64_t row = bs[r] + p;
+      xr[r] = VTr<VT>::ld(vin + row);
+      dg[r] = make_double2(0.0, 0.0);
+      if (A.diag_arr) {
+        if (A.diag_arr_cplx) dg[r] = ((const double2*)A.diag_arr)[row];
+        else dg[r].x = ((const double*)A.diag_arr)[row];
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RW; ++r) {
+      if (r >= nr) break;
+      const int64_t row = bs[r] + p;
+      const double2 dsum = cadd(cmul(dg[r], A.alpha_diag), A.shift);
+      double2 res;
+      if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc[r]), cmul(dsum, xr[r]));
+      else res = make_double2(A.alpha_off.x * acc[r].x + dsum.x * xr[r].x, 0.0);
+      if (A.beta) res = cadd(res, VTr<VT>::ld(vout + row));
+      VTr<VT>::st(vout + row, res);
+    }
+  }
+}
+
+template <typename VT>
+int launch_hv_split2(const HvArgs& A, hipStream_t st) {
+  hipLaunchKernelGGL(k_split2_rows<VT>, dim3((unsigned)A.split_nblk), dim3(SPLIT2_ROWS_THREADS), (size_t)A.split_lds,
+                     st, A);
+  const int64_t nb = A.split_nunits < 65536 ? (A.split_nunits > 0 ? A.split_nunits : 1) : 65536;
+  hipLaunchKernelGGL(k_split2_strips<VT>, dim3((unsigned)nb), dim3(256), 0, st, A);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    qsx_set_error(std::string("split-chain launch failed: ") + hipGetErrorString(e));
+    return QSX_ERR_HIP;
+  }
+  return QSX_OK;
+}
+
 template <typename VT>
 int launch_hv_split(const HvArgs& A, int nblk, hipStream_t st) {
   hipLaunchKernelGGL(k_hv_split<VT>, dim3((unsigned)nblk), dim3(SPLIT_THREADS), (size_t)A.split_lds, st, A);
@@ -1508,6 +1823,8 @@ int qsx_hv_launch_c128(const HvArgs& A, int N, int nmaps, bool fm, int lookup, i
   int NAME(const HvArgs& A, int N, int nmaps, bool fm, int lookup, int scatter, hipStream_t st) { \
     if (!scatter && A.split && A.sched == 7 && A.fast_rows && A.nvec == 1)                     \
       return launch_hv_split<VT>(A, A.split_nblk, st);                                        \
+    if (!scatter && A.split && A.sched == 11 && A.fast_rows && A.nvec == 1)                    \
+      return launch_hv_split2<VT>(A, st);                                                     \
     if (scatter) {                                                                             \
       if (N <= 32) return launch_hv_scatter<uint32_t, VT>(A, nmaps, fm, lookup, st);           \
       return launch_hv_scatter<uint64_t, VT>(A, nmaps, fm, lookup, st);                        \

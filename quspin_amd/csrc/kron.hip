@@ -235,13 +235,6 @@ __global__ void __launch_bounds__(1024) k_kron_down(KronArgs K) {
   }
 }
 
-// Block index with the blocks of each XCD (workgroups are dispatched
-// round-robin over the 8 XCDs) renumbered into one contiguous run.
-__device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t nb) {
-  const int64_t x = b & 7, pos = b >> 3, q = nb >> 3, r = nb & 7;
-  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + pos;
-}
-
 constexpr int KRON_UP_ROWS = 4;  // up rows per wave in the column-strip pass
 
 // Column-strip pass: Y = a_t (Z + H_up X - (mu - diag) X) with the Taylor

@@ -7,8 +7,10 @@ many lanes per row with shuffle reduction; "queue": compacted LDS work queue;
 "batch4/8": terms in batches with all lookups of a batch in flight, for
 rank-based lookups; "batch14": the same with 14 terms per batch, for the
 table; "split": blocks of the split-chain layout of one hard-core-boson
-sector without symmetry maps; "chunk4/8": direct-rank rows with the terms in
-chunks and predicated gathers; all but "rows" for one vector only).  Candidates are timed on the operator's own
+sector without symmetry maps; "split2": the same layout in two passes, lower
+half-chain terms per block, then upper-half and crossing terms over column
+strips; "chunk4/8": direct-rank rows with the terms in chunks and predicated
+gathers; all but "rows" for one vector only).  Candidates are timed on the operator's own
 basis with HIP events (a few launches each) and the winner is cached per
 (basis signature, dtype, nvec) in memory and in a JSON file
 (``QSX_TUNE_CACHE``, default ``~/.cache/quspin_amd/tune.json``).
@@ -27,7 +29,7 @@ _mem = {}
 
 # schedule name -> launch code of qsx_hv_ctx_set_schedule
 SCHEDULES = {"rows": 0, "queue": 1, "rows2": 2, "rows4": 3, "rows8": 4, "batch4": 5, "batch8": 6, "split": 7,
-             "chunk4": 8, "chunk8": 9, "batch14": 10}
+             "chunk4": 8, "chunk8": 9, "batch14": 10, "split2": 11}
 
 
 def _cache_path():
@@ -95,7 +97,7 @@ def candidates(op, vdt, nvec):
         if lk == "table":
             scheds.append("batch14")
         if lk == "search" and _split_layout(op):
-            scheds.append("split")
+            scheds += ["split", "split2"]
         if lk == "search" and _rank_lookup(op) and not len(b._pers):
             scheds += ["chunk4", "chunk8"]
         out += [{"lookup": lk, "schedule": sc} for sc in scheds]

@@ -32,12 +32,13 @@ def _model(L, cplx, seed):
     return st
 
 
+@pytest.mark.parametrize("sched", ["split", "split2"])
 @pytest.mark.parametrize("dt", [np.float64, np.complex128, np.float32, np.complex64])
-def test_split_schedule_vs_oracle(dt, monkeypatch):
+def test_split_schedule_vs_oracle(dt, sched, monkeypatch):
     from quspin_amd import _lib as Lb
     from quspin_amd.basis import spin_basis_general
     from quspin_amd.operators import quantum_LinearOperator
-    monkeypatch.setenv("QSX_SCHEDULE", "split")
+    monkeypatch.setenv("QSX_SCHEDULE", sched)
     cplx = np.dtype(dt).kind == "c"
     tol = 1e-12 if np.dtype(dt) in (np.float64, np.complex128) else 3e-6
     for L, Nup in ((12, 6), (17, 6), (20, 10), (20, 3)):
@@ -53,7 +54,7 @@ def test_split_schedule_vs_oracle(dt, monkeypatch):
         got = Lb.to_host(Q.dot(Lb.to_device(v)))
         cfg = next(iter(Q._qsx_operator().config.values()))
         print(f"case L={L} Nup={Nup} Ns={B.Ns} dtype={np.dtype(dt).name} cfg={cfg}", flush=True)
-        assert cfg["schedule"] == "split"
+        assert cfg["schedule"] == sched
         ref = A @ v.astype(np.complex128 if cplx else np.float64)
         assert np.abs(got - ref).max() <= tol * max(1.0, np.abs(ref).max())
         # transposed and conjugated products (reference quirk B.1 included)
@@ -66,7 +67,8 @@ def test_split_schedule_vs_oracle(dt, monkeypatch):
                                        atol=tol * max(1.0, np.abs(ref).max()))
 
 
-def test_split_schedule_expm_and_lanczos(monkeypatch):
+@pytest.mark.parametrize("sched", ["split", "split2"])
+def test_split_schedule_expm_and_lanczos(sched, monkeypatch):
     """The Taylor loop (expm_multiply_parallel on the matrix-free operator) and
     a captured Lanczos recurrence on the split schedule, against the oracle's
     expm and dense ground energy."""
@@ -77,7 +79,7 @@ def test_split_schedule_expm_and_lanczos(monkeypatch):
     from quspin_amd.operators import quantum_LinearOperator
     from quspin_amd.tools.evolution import expm_multiply_parallel
     from quspin_amd.tools.lanczos import lanczos_full
-    monkeypatch.setenv("QSX_SCHEDULE", "split")
+    monkeypatch.setenv("QSX_SCHEDULE", sched)
     L = 14
     B = spin_basis_general(L, Nup=7, pauli=0)
     st = _model(L, True, 3)
