@@ -727,6 +727,9 @@ static void split_free(qsx_hv_ctx* X) {
   X->A.split_lds = 0;
   X->A.split_nblk = 0;
   X->A.split_nunits = 0;
+  X->A.split_nrp = 0;
+  X->A.split_lds2 = 0;
+  X->A.split_pairs = 0;
 }
 
 // Build the split-chain plan of a context (hard-core-boson sector of N <= 32
@@ -869,6 +872,23 @@ static int split_build(qsx_hv_ctx* X, bool with_z) {
     upn[i] = (int32_t)upl[i].size();
     upw = std::max(upw, (int)upl[i].size());
   }
+  // row-pass pairs: consecutive blocks of order with one class, while two
+  // staged blocks take at most 64 KB of LDS (larger pairs cost more in
+  // occupancy than the shared sliced-ELL entries save)
+  const bool pairs = 2 * maxblk(K) * esz <= 64 * 1024;
+  std::vector<int32_t> rpair;
+  for (size_t i = 0; i < order.size();) {
+    const int c1 = Np - __builtin_popcount((uint32_t)order[i]);
+    if (pairs && i + 1 < order.size() && Np - __builtin_popcount((uint32_t)order[i + 1]) == c1) {
+      rpair.push_back(order[i]);
+      rpair.push_back(order[i + 1]);
+      i += 2;
+    } else {
+      rpair.push_back(order[i]);
+      rpair.push_back(-1);
+      i += 1;
+    }
+  }
   // the up and crossing terms in compact form, and their partner blocks per u
   std::vector<SplitXT> xts;
   for (const std::vector<TermDev>* lst : {&up, &cross})
@@ -939,6 +959,8 @@ static int split_build(qsx_hv_ctx* X, bool with_z) {
   if (e == hipSuccess) { e = up_copy(upc.data(), sizeof(double2) * upc.size(), &p); H.upc = (const double2*)p; }
   if (e == hipSuccess && nxt > 0) { e = up_copy(xts.data(), sizeof(SplitXT) * xts.size(), &p); H.xt = (const SplitXT*)p; }
   if (e == hipSuccess) { e = up_copy(xb.data(), sizeof(int64_t) * xb.size(), &p); H.xb = (const int64_t*)p; }
+  H.nrp = (int)(rpair.size() / 2);
+  if (e == hipSuccess) { e = up_copy(rpair.data(), sizeof(int32_t) * rpair.size(), &p); H.rpair = (const int32_t*)p; }
   if (e == hipSuccess && with_z) {
     e = hipMalloc(&H.z, (size_t)X->A.Ns * esz);
     if (e == hipSuccess) X->split_bufs.push_back(H.z);
@@ -953,6 +975,9 @@ static int split_build(qsx_hv_ctx* X, bool with_z) {
   X->A.split_lds = (int)(maxblk(K) * esz);
   X->A.split_nblk = H.nblk;
   X->A.split_nunits = H.nunits;
+  X->A.split_nrp = H.nrp;
+  X->A.split_lds2 = (int)((pairs ? 2 : 1) * maxblk(K) * esz);
+  X->A.split_pairs = pairs ? 1 : 0;
   X->split_z = with_z;
   X->split_nxt = nxt;
   return QSX_OK;

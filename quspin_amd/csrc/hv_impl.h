@@ -123,6 +123,9 @@ struct HvArgs {
   int split_lds;      // bytes of LDS the split kernel stages (largest block)
   int split_nblk;     // its workgroups (blocks of the layout)
   int64_t split_nunits;  // column-strip units of the two-pass form (schedule 11)
+  int split_nrp;         // its row-pass workgroups (pairs of blocks, or single blocks)
+  int split_lds2;        // bytes of LDS its row pass stages
+  int split_pairs;       // its row pass takes two blocks per workgroup
 };
 
 // Split-chain layout of one hard-core-boson particle sector of N <= 32 bits
@@ -175,6 +178,10 @@ struct SplitDev {
   // (u ^ the term's upper flip; -1: none)
   const struct SplitXT* xt;  // [n_up + n_cross]
   const int64_t* xb;         // [nblk * (n_up + n_cross)]
+  // row pass: workgroup b takes the blocks rpair[2b] and rpair[2b + 1] (-1:
+  // none), two u of one class, so each sliced-ELL entry serves both
+  const int32_t* rpair;      // [2 * nrp]
+  int nrp;
 };
 
 // A term of the column-strip pass in compact form (N <= 32 bits).
@@ -1480,10 +1487,11 @@ __global__ void __launch_bounds__(SPLIT_THREADS) k_hv_split(HvArgs A) {
 }
 
 // Two-pass form of the split-chain layout (schedule 11).  Pass 1, one
-// workgroup per block u (largest first): the block is staged in LDS (four
-// loads per thread before their stores) and every row's lower-half terms are
-// summed in FP64 (the class's sliced-ELL matrix, then the lower-flip terms
-// that read upper bits) into Z, stored in the vector dtype.
+// workgroup per two blocks u of one class (largest first): both blocks are
+// staged in LDS (four loads per thread before their stores) and every row's
+// lower-half terms are summed in FP64 (the class's sliced-ELL matrix, each
+// entry serving both blocks, then the lower-flip terms that read upper bits)
+// into Z, stored in the vector dtype.
 constexpr int SPLIT2_ROWS_THREADS = 512;
 constexpr int SPLIT2_UP_ROWS = 4;  // u per wave in the column-strip pass
 constexpr int SPLIT2_UNIT_U = 4 * SPLIT2_UP_ROWS;  // u per column-strip unit (four waves)
@@ -1502,34 +1510,40 @@ __device__ __forceinline__ bool split_applies(const TermDev* __restrict__ T, uin
   return v == (uint32_t)T->req_val || (T->has2 && v == (uint32_t)T->req_val2);
 }
 
-template <typename VT>
+template <typename VT, int R>
 __global__ void __launch_bounds__(SPLIT2_ROWS_THREADS) k_split2_rows(HvArgs A) {
   extern __shared__ unsigned char split_smem[];
   __shared__ int64_t sbk[SPLIT_MAX_K * 33];  // C(n, k) for n < K (the ranks of d)
   __shared__ double2 lut[SPLIT_MAX_LUT];
   if (A.skip && *A.skip) return;
   const SplitDev S = *A.split;
-  const uint32_t u = (uint32_t)S.order[blockIdx.x];
+  const int u1 = S.rpair[2 * blockIdx.x], u2 = R > 1 ? S.rpair[2 * blockIdx.x + 1] : -1;
+  const int nb = u2 >= 0 ? 2 : 1;
+  const uint32_t uq[2] = {(uint32_t)u1, (uint32_t)(u2 >= 0 ? u2 : u1)};
   const int K = S.K;
   const int Np = A.R->Np;
-  const int c = Np - __builtin_popcount(u);
+  const int c = Np - __builtin_popcount(uq[0]);
   const int n = S.cls_off[c + 1] - S.cls_off[c];
-  const int64_t base = S.ublk[u];
+  const int64_t base[2] = {S.ublk[uq[0]], S.ublk[uq[1]]};
   const VT* __restrict__ vin = (const VT*)A.v_in;
   VT* __restrict__ z = (VT*)S.z;
-  VT* xs = (VT*)split_smem;
+  VT* xs = (VT*)split_smem;  // block 0 at xs[0 .. n), block 1 at xs[n .. 2n)
   constexpr bool CPLX = VTr<VT>::cplx;
   {
-    const int bd = blockDim.x;
-    int p = threadIdx.x;
-    for (; p + 3 * bd < n; p += 4 * bd) {
-      const VT t0 = vin[base + p], t1 = vin[base + p + bd], t2 = vin[base + p + 2 * bd], t3 = vin[base + p + 3 * bd];
-      xs[p] = t0;
-      xs[p + bd] = t1;
-      xs[p + 2 * bd] = t2;
-      xs[p + 3 * bd] = t3;
+    // staging: four loads per thread before their LDS stores
+    const int bd = blockDim.x, tot = nb * n;
+    int i = threadIdx.x;
+    for (; i + 3 * bd < tot; i += 4 * bd) {
+      VT t[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int j = i + q * bd;
+        t[q] = vin[(j < n ? base[0] : base[1] - n) + j];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) xs[i + q * bd] = t[q];
     }
-    for (; p < n; p += bd) xs[p] = vin[base + p];
+    for (; i < tot; i += bd) xs[i] = vin[(i < n ? base[0] : base[1] - n) + i];
   }
   if (S.n_low > 0)
     for (int i = threadIdx.x; i < K * 33; i += blockDim.x) sbk[i] = (&A.R->binom[0][0])[i];
@@ -1538,44 +1552,56 @@ __global__ void __launch_bounds__(SPLIT2_ROWS_THREADS) k_split2_rows(HvArgs A) {
   const uint16_t* __restrict__ dl = S.dlist + S.cls_off[c];
   const int ke = S.ell_k[c];
   const uint32_t* __restrict__ el = S.ell + S.ell_off[c];
+  const int xo = nb > 1 ? n : 0;  // second block's offset in LDS (the first again when alone)
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
-    double2 acc = make_double2(0.0, 0.0);
+    double2 acc0 = make_double2(0.0, 0.0), acc1 = make_double2(0.0, 0.0);
     for (int k0 = 0; k0 < ke; k0 += 8) {
       uint32_t ev[8];
 #pragma unroll
       for (int q = 0; q < 8; ++q) ev[q] = (k0 + q < ke) ? el[(int64_t)(k0 + q) * n + p] : (uint32_t)p;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        const double2 x = VTr<VT>::ld(xs + (ev[q] & 0xFFFFu));
+        const int pos = (int)(ev[q] & 0xFFFFu);
         const double2 cf = lut[ev[q] >> 16];
-        if constexpr (CPLX) acc = cadd(acc, cmul(cf, x));
-        else acc.x += cf.x * x.x;
+        const double2 x0 = VTr<VT>::ld(xs + pos);
+        if constexpr (CPLX) acc0 = cadd(acc0, cmul(cf, x0));
+        else acc0.x += cf.x * x0.x;
+        if constexpr (R > 1) {
+          const double2 x1 = VTr<VT>::ld(xs + xo + pos);
+          if constexpr (CPLX) acc1 = cadd(acc1, cmul(cf, x1));
+          else acc1.x += cf.x * x1.x;
+        }
       }
     }
     if (S.n_low > 0) {
       const uint32_t d = dl[p];
-      const uint32_t s = (u << K) | d;
       const int64_t ra = (int64_t)n - 1 - p;  // ascending rank of d in its class
-      for (int t = 0; t < S.n_low; ++t) {
-        const TermDev* __restrict__ T = S.low + t;
-        if (!split_applies(T, s)) continue;
-        const uint32_t d2 = d ^ (uint32_t)T->flip;
-        if (__builtin_popcount(d2) != c) continue;
-        const int lo = T->lo, hi = T->hi;
-        int64_t rk;
-        if (hi == lo + 1) {
-          const int64_t dl1 = sbk[lo * 33 + __builtin_popcount(d & ((1u << lo) - 1u))];
-          rk = ((d >> lo) & 1u) ? ra + dl1 : ra - dl1;
-        } else {
-          rk = ra - window_rank<uint32_t>(sbk, d, lo, hi) + window_rank<uint32_t>(sbk, d2, lo, hi);
+      for (int b = 0; b < nb; ++b) {
+        const uint32_t s = (uq[b] << K) | d;
+        double2 acc = b ? acc1 : acc0;
+        for (int t = 0; t < S.n_low; ++t) {
+          const TermDev* __restrict__ T = S.low + t;
+          if (!split_applies(T, s)) continue;
+          const uint32_t d2 = d ^ (uint32_t)T->flip;
+          if (__builtin_popcount(d2) != c) continue;
+          const int lo = T->lo, hi = T->hi;
+          int64_t rk;
+          if (hi == lo + 1) {
+            const int64_t dl1 = sbk[lo * 33 + __builtin_popcount(d & ((1u << lo) - 1u))];
+            rk = ((d >> lo) & 1u) ? ra + dl1 : ra - dl1;
+          } else {
+            rk = ra - window_rank<uint32_t>(sbk, d, lo, hi) + window_rank<uint32_t>(sbk, d2, lo, hi);
+          }
+          const double2 x = VTr<VT>::ld(xs + (b ? xo : 0) + (n - 1 - rk));
+          const double2 cf = split_coef<CPLX>(A, T, s);
+          if constexpr (CPLX) acc = cadd(acc, cmul(cf, x));
+          else acc.x += cf.x * x.x;
         }
-        const double2 x = VTr<VT>::ld(xs + (n - 1 - rk));
-        const double2 cf = split_coef<CPLX>(A, T, s);
-        if constexpr (CPLX) acc = cadd(acc, cmul(cf, x));
-        else acc.x += cf.x * x.x;
+        if (b) acc1 = acc; else acc0 = acc;
       }
     }
-    VTr<VT>::st(z + base + p, acc);
+    VTr<VT>::st(z + base[0] + p, acc0);
+    if (nb > 1) VTr<VT>::st(z + base[1] + p, acc1);
   }
 }
 
@@ -1596,7 +1622,7 @@ __device__ __forceinline__ int64_t rl_i64(int64_t x, int t) {
 
 __device__ __forceinline__ double rl_dbl(double x, int t) { return __longlong_as_double(rl_i64(__double_as_longlong(x), t)); }
 
-template <typename VT>
+template <typename VT, int HD, int XB>
 __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
   if (A.skip && *A.skip) return;
   const SplitDev S = *A.split;
@@ -1661,58 +1687,73 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
     for (int r = 0; r < RW; ++r) nmax = npart[r] > nmax ? npart[r] : nmax;
     // upper-half terms reading upper bits only: partner block, same position,
     // eight loads in flight
-    for (int t0 = 0; t0 < nmax; t0 += 2) {
-      double2 v[2][RW], cf[2][RW];
+    for (int t0 = 0; t0 < nmax; t0 += HD) {
+      VT v[HD][RW];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int h = 0; h < HD; ++h) {
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
           const int t = t0 + h;
           const bool on = t < npart[r];  // wave-uniform
-          const int tl = on ? t : 0;
-          const int64_t b2 = on ? rl_i64(pb[r], tl) : bs[r];
-          cf[h][r] = on ? make_double2(rl_dbl(pc[r].x, tl), rl_dbl(pc[r].y, tl)) : make_double2(0.0, 0.0);
-          v[h][r] = VTr<VT>::ld(vin + b2 + ps);
+          const int64_t b2 = on ? rl_i64(pb[r], t) : bs[r];
+          v[h][r] = vin[b2 + ps];
         }
       }
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
+      for (int h = 0; h < HD; ++h)
 #pragma unroll
         for (int r = 0; r < RW; ++r) {
-          if constexpr (CPLX) acc[r] = cadd(acc[r], cmul(cf[h][r], v[h][r]));
-          else acc[r].x += cf[h][r].x * v[h][r].x;
+          const int t = t0 + h;
+          if (t >= npart[r]) continue;  // wave-uniform
+          const double2 cf = make_double2(rl_dbl(pc[r].x, t), rl_dbl(pc[r].y, t));
+          const double2 x = VTr<VT>::ld(&v[h][r]);
+          if constexpr (CPLX) acc[r] = cadd(acc[r], cmul(cf, x));
+          else acc[r].x += cf.x * x.x;
         }
     }
     // upper-flip terms reading lower bits (partner position = own position)
     // and crossing terms (partner position from dpos): per-lane conditions;
     // for each term the rows' position reads, then their partner reads
-    for (int t = 0; t < nxt; ++t) {
-      const SplitXT T = s_xt[t];
-      const bool upt = t < S.n_up;
-      int64_t a[RW];
-      bool on[RW];
+    for (int t0 = 0; t0 < nxt; t0 += XB) {
+      int64_t a[XB][RW];
+      bool on[XB][RW];
 #pragma unroll
-      for (int r = 0; r < RW; ++r) {
-        const uint32_t s = (uu[r] << K) | d;
-        const uint32_t s2 = s ^ T.flip;
-        const uint32_t vv = s & T.rm;
-        const int64_t b2 = rl_i64(xbl[r], t);  // wave-uniform
-        on[r] = r < nr && pok && b2 >= 0 && (vv == T.rv || (T.has2 && vv == T.rv2)) && __builtin_popcount(s2) == Np;
-        a[r] = upt ? (int64_t)ps : (int64_t)S.dpos[s2 & dmask];
-        a[r] += on[r] ? b2 : bs[r];
-        if (!on[r]) a[r] = bs[r] + ps;
+      for (int h = 0; h < XB; ++h) {
+        const int t = t0 + h < nxt ? t0 + h : nxt - 1;
+        const SplitXT T = s_xt[t];
+        const bool upt = t < S.n_up;
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          const uint32_t s = (uu[r] << K) | d;
+          const uint32_t s2 = s ^ T.flip;
+          const uint32_t vv = s & T.rm;
+          const int64_t b2 = rl_i64(xbl[r], t);  // wave-uniform
+          on[h][r] = t0 + h < nxt && r < nr && pok && b2 >= 0 && (vv == T.rv || (T.has2 && vv == T.rv2)) &&
+                     __builtin_popcount(s2) == Np;
+          a[h][r] = upt ? (int64_t)ps : (int64_t)S.dpos[s2 & dmask];
+          a[h][r] += on[h][r] ? b2 : bs[r];
+          if (!on[h][r]) a[h][r] = bs[r] + ps;
+        }
       }
-      double2 v[RW];
+      VT v[XB][RW];
 #pragma unroll
-      for (int r = 0; r < RW; ++r) v[r] = VTr<VT>::ld(vin + a[r]);
+      for (int h = 0; h < XB; ++h)
 #pragma unroll
-      for (int r = 0; r < RW; ++r) {
-        const uint32_t s = (uu[r] << K) | d;
-        const double wt = on[r] ? ((__builtin_popcount(s & T.sm) & 1) ? -1.0 : 1.0) : 0.0;
-        double2 cf = make_double2(T.cr * wt, CPLX ? T.ci * wt : 0.0);
-        if (CPLX && A.off_conj) cf = cconj(cf);
-        if constexpr (CPLX) acc[r] = cadd(acc[r], cmul(cf, v[r]));
-        else acc[r].x += cf.x * v[r].x;
+        for (int r = 0; r < RW; ++r) v[h][r] = vin[a[h][r]];
+#pragma unroll
+      for (int h = 0; h < XB; ++h) {
+        const int t = t0 + h < nxt ? t0 + h : nxt - 1;
+        const SplitXT T = s_xt[t];
+#pragma unroll
+        for (int r = 0; r < RW; ++r) {
+          const uint32_t s = (uu[r] << K) | d;
+          const double wt = on[h][r] ? ((__builtin_popcount(s & T.sm) & 1) ? -1.0 : 1.0) : 0.0;
+          double2 cf = make_double2(T.cr * wt, CPLX ? T.ci * wt : 0.0);
+          if (CPLX && A.off_conj) cf = cconj(cf);
+          const double2 x = VTr<VT>::ld(&v[h][r]);
+          if constexpr (CPLX) acc[r] = cadd(acc[r], cmul(cf, x));
+          else acc[r].x += cf.x * x.x;
+        }
       }
     }
     if (!pok) continue;
@@ -1743,10 +1784,16 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
 
 template <typename VT>
 int launch_hv_split2(const HvArgs& A, hipStream_t st) {
-  hipLaunchKernelGGL(k_split2_rows<VT>, dim3((unsigned)A.split_nblk), dim3(SPLIT2_ROWS_THREADS), (size_t)A.split_lds,
-                     st, A);
+  if (A.split_pairs)
+    hipLaunchKernelGGL((k_split2_rows<VT, 2>), dim3((unsigned)A.split_nrp), dim3(SPLIT2_ROWS_THREADS),
+                       (size_t)A.split_lds2, st, A);
+  else
+    hipLaunchKernelGGL((k_split2_rows<VT, 1>), dim3((unsigned)A.split_nrp), dim3(SPLIT2_ROWS_THREADS),
+                       (size_t)A.split_lds2, st, A);
   const int64_t nb = A.split_nunits < 65536 ? (A.split_nunits > 0 ? A.split_nunits : 1) : 65536;
-  hipLaunchKernelGGL(k_split2_strips<VT>, dim3((unsigned)nb), dim3(256), 0, st, A);
+  // two upper-half terms and two crossing terms per round of loads (measured
+  // at L=28 against one crossing term per round, and four upper-half terms)
+  hipLaunchKernelGGL((k_split2_strips<VT, 2, 2>), dim3((unsigned)nb), dim3(256), 0, st, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     qsx_set_error(std::string("split-chain launch failed: ") + hipGetErrorString(e));
