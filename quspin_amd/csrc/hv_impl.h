@@ -45,6 +45,42 @@ template <> struct VTr<double2> {
   __device__ static double2 rnd(double2 v) { return v; }
 };
 
+// Streaming (non-temporal) element loads and stores: data read or written
+// once by a pass, marked so that it does not push reused data out of L2.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+template <typename VT> __device__ __forceinline__ double2 ld_stream(const VT* p);
+template <> __device__ __forceinline__ double2 ld_stream<float>(const float* p) {
+  return make_double2((double)__builtin_nontemporal_load(p), 0.0);
+}
+template <> __device__ __forceinline__ double2 ld_stream<double>(const double* p) {
+  return make_double2(__builtin_nontemporal_load(p), 0.0);
+}
+template <> __device__ __forceinline__ double2 ld_stream<float2>(const float2* p) {
+  const uint64_t b = __builtin_nontemporal_load((const uint64_t*)p);
+  return make_double2((double)__uint_as_float((uint32_t)b), (double)__uint_as_float((uint32_t)(b >> 32)));
+}
+template <> __device__ __forceinline__ double2 ld_stream<double2>(const double2* p) {
+  const nt_d2 v = __builtin_nontemporal_load((const nt_d2*)p);
+  return make_double2(v.x, v.y);
+}
+template <typename VT> __device__ __forceinline__ void st_stream(VT* p, double2 v);
+template <> __device__ __forceinline__ void st_stream<float>(float* p, double2 v) {
+  __builtin_nontemporal_store((float)v.x, p);
+}
+template <> __device__ __forceinline__ void st_stream<double>(double* p, double2 v) {
+  __builtin_nontemporal_store(v.x, p);
+}
+template <> __device__ __forceinline__ void st_stream<float2>(float2* p, double2 v) {
+  const uint64_t b = (uint64_t)__float_as_uint((float)v.x) | ((uint64_t)__float_as_uint((float)v.y) << 32);
+  __builtin_nontemporal_store(b, (uint64_t*)p);
+}
+template <> __device__ __forceinline__ void st_stream<double2>(double2* p, double2 v) {
+  nt_d2 w;
+  w.x = v.x;
+  w.y = v.y;
+  __builtin_nontemporal_store(w, (nt_d2*)p);
+}
+
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -1600,8 +1636,8 @@ __global__ void __launch_bounds__(SPLIT2_ROWS_THREADS) k_split2_rows(HvArgs A) {
         if (b) acc1 = acc; else acc0 = acc;
       }
     }
-    VTr<VT>::st(z + base[0] + p, acc0);
-    if (nb > 1) VTr<VT>::st(z + base[1] + p, acc1);
+    st_stream<VT>(z + base[0] + p, acc0);
+    if (nb > 1) st_stream<VT>(z + base[1] + p, acc1);
   }
 }
 
@@ -1681,7 +1717,7 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
     }
     double2 acc[RW];
 #pragma unroll
-    for (int r = 0; r < RW; ++r) acc[r] = VTr<VT>::ld(z + bs[r] + ps);
+    for (int r = 0; r < RW; ++r) acc[r] = ld_stream<VT>(z + bs[r] + ps);
     int nmax = 0;
 #pragma unroll
     for (int r = 0; r < RW; ++r) nmax = npart[r] > nmax ? npart[r] : nmax;
@@ -1764,8 +1800,8 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
       xr[r] = VTr<VT>::ld(vin + row);
       dg[r] = make_double2(0.0, 0.0);
       if (A.diag_arr) {
-        if (A.diag_arr_cplx) dg[r] = ((const double2*)A.diag_arr)[row];
-        else dg[r].x = ((const double*)A.diag_arr)[row];
+        if (A.diag_arr_cplx) dg[r] = ld_stream<double2>((const double2*)A.diag_arr + row);
+        else dg[r] = ld_stream<double>((const double*)A.diag_arr + row);
       }
     }
 #pragma unroll
@@ -1777,7 +1813,7 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
       if constexpr (CPLX) res = cadd(cmul(A.alpha_off, acc[r]), cmul(dsum, xr[r]));
       else res = make_double2(A.alpha_off.x * acc[r].x + dsum.x * xr[r].x, 0.0);
       if (A.beta) res = cadd(res, VTr<VT>::ld(vout + row));
-      VTr<VT>::st(vout + row, res);
+      st_stream<VT>(vout + row, res);
     }
   }
 }
