@@ -228,7 +228,7 @@ __global__ void __launch_bounds__(1024) k_kron_down(KronArgs K) {
       if (d < Nd) {
 #pragma unroll
         for (int r = 0; r < R; ++r)
-          if (r < nr) VTr<VT>::st(Z + (u0 + r) * Nd + d, make_double2(sr[r], si[r]));
+          if (r < nr) st_stream<VT>(Z + (u0 + r) * Nd + d, make_double2(sr[r], si[r]));
       }
     }
     if constexpr (STAGE) __syncthreads();
@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(256) k_kron_up(KronArgs K, int64_t nunits, int
     double sr[RW], si[RW];
 #pragma unroll
     for (int r = 0; r < RW; ++r) {
-      const double2 z = r < nr ? VTr<VT>::ld(Z + (u0 + r) * Nd + ds) : make_double2(0.0, 0.0);
+      const double2 z = r < nr ? ld_stream<VT>(Z + (u0 + r) * Nd + ds) : make_double2(0.0, 0.0);
       sr[r] = z.x;
       si[r] = z.y;
     }
@@ -336,9 +336,9 @@ __global__ void __launch_bounds__(256) k_kron_up(KronArgs K, int64_t nunits, int
     for (int r = 0; r < RW; ++r) {
       const int64_t row = (u0 + (r < nr ? r : 0)) * Nd + d;
       xr[r] = VTr<VT>::ld(X + row);
-      gd[r] = K.sdiag ? K.sdiag[row] : 0.0;
-      dx[r] = A.xdiag ? A.xdiag[row] : 0.0;
-      if constexpr (EPI) fo[r] = VTr<FT>::ld((const FT*)A.F + row);
+      gd[r] = K.sdiag ? ld_stream<double>(K.sdiag + row).x : 0.0;
+      dx[r] = A.xdiag ? ld_stream<double>(A.xdiag + row).x : 0.0;
+      if constexpr (EPI) fo[r] = ld_stream<FT>((const FT*)A.F + row);
     }
     VT* __restrict__ y = (VT*)A.y;
 #pragma unroll
@@ -356,12 +356,12 @@ __global__ void __launch_bounds__(256) k_kron_up(KronArgs K, int64_t nunits, int
         res = make_double2(a_t.x * (ar - mu_r.x * xr[r].x), 0.0);
       }
       if (A.beta) res = cadd(res, VTr<VT>::ld(y + row));
-      VTr<VT>::st(y + row, res);
+      st_stream<VT>(y + row, res);
       if constexpr (EPI) {
         // the stored (rounded) B enters F, as in the reference
         const double2 rs = VTr<VT>::rnd(res);
         const double2 fs = VTr<FT>::rnd(cadd(fo[r], rs));
-        VTr<FT>::st((FT*)A.F + row, fs);
+        st_stream<FT>((FT*)A.F + row, fs);
         maxB = fmax(maxB, VTr<VT>::cplx ? cabs2(rs) : fabs(rs.x));
         maxF = fmax(maxF, VTr<VT>::cplx ? cabs2(fs) : fabs(fs.x));
       }
