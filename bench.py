@@ -44,7 +44,9 @@ MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 88.4e6, (28, "table", "bat
                     ("csr", 34, "float64"): 14.68e9}
 # The same for the precision workload's term kernels (full launches, per
 # launch): profiles/k_csr_spmv_expm_hubbard_L14_xxz_L28.md
-C3_TRAFFIC = {("mf", "complex64"): 3.76e9, ("mf", "complex128"): 7.44e9}
+C3_TRAFFIC = {("mf", "complex64", "rows"): 3.76e9, ("mf", "complex128", "rows"): 7.44e9,
+              ("mf", "complex64", "split2"): 2.96e9, ("mf", "complex128", "split2"): 5.36e9,
+              ("csr", "complex64"): 9.95e9, ("csr", "complex128"): 18.76e9}
 # ... and of one batched H.V of the 26-sector workload (CSR form, all sectors):
 # profiles/k_csr_wave_batched_heisenberg_L26_64vec.md
 C4_TRAFFIC = {("csr", "fp32"): 70.1e9, ("csr", "fp64"): 161.6e9}
@@ -355,8 +357,21 @@ def _kernel_roofline(op, x, form, Ns, traffic):
         alg = op.indptr.element_size() * (Ns + 1) + op.nnz * (op.indices.element_size() + op.data.element_size()) \
             + 2 * Ns * esz
     gbs = alg / ms / 1e6
-    return {"ms": round(ms, 4), "alg_bytes": int(alg), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic}
+    out = {"ms": round(ms, 4), "alg_bytes": int(alg), "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
+           "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    if form == "mf":
+        cfg = next((c for (dt_, nv), c in op.config.items() if np.dtype(dt_) == np.dtype(_np_dtype(x)) and nv == 1),
+                   {})
+        out["schedule"] = cfg.get("schedule")
+        traffic = C3_TRAFFIC.get(("mf", np.dtype(_np_dtype(x)).name, cfg.get("schedule")))
+    out["traffic"] = traffic
+    return out
+
+
+def _np_dtype(t):
+    import torch
+    return {torch.complex64: np.complex64, torch.complex128: np.complex128, torch.float32: np.float32,
+            torch.float64: np.float64}[t.dtype]
 
 
 def _run_precision(steps, form, L=28, dt=0.1, cpu=False):
