@@ -29,6 +29,12 @@ def _model(L, cplx, seed):
     st += [["+-", [[J2, i, (i + 2) % L] for i in range(L)]],
            ["-+", [[np.conj(J2), i, (i + 2) % L] for i in range(L)]],
            ["zz", [[0.2, i, (i + 3) % L] for i in range(L)]]]
+    # a hop beside a z: where the z sits in the other half-chain the term
+    # flips bits of one half and reads the other (the "low" and "up" term
+    # classes of the split layout)
+    J3 = 0.15 - (0.1j if cplx else 0.0)
+    st += [["z+-", [[J3, i, (i + 1) % L, (i + 2) % L] for i in range(L)]],
+           ["z-+", [[np.conj(J3), i, (i + 1) % L, (i + 2) % L] for i in range(L)]]]
     return st
 
 
