@@ -1193,6 +1193,21 @@ int qsx_hv_expm_term(void* ctx, const void* x, void* y, double2 coef, double2 sh
   return hv_launch(X->c, X->L, A, X->transposed, X->conjugated, X->dtype, st);
 }
 
+// The same term with the Taylor epilogue (F += the stored term, the norms of
+// the term and of F into nB, nF) fused into the product.  Only the two-pass
+// split schedule fuses it; otherwise QSX_ERR_UNSUPPORTED is returned before
+// anything is launched, and the caller runs the separate epilogue.
+int qsx_hv_expm_term_epi(void* ctx, const void* x, void* y, double2 coef, double2 shift, const int* skip, void* F,
+                         double* nB, double* nF, hipStream_t st) {
+  qsx_hv_ctx* X = (qsx_hv_ctx*)ctx;
+  if (!X->split_z || X->A.sched != 11 || X->A.nvec != 1) return QSX_ERR_UNSUPPORTED;
+  HvArgs A = X->A;
+  A.v_in = x; A.v_out = y;
+  A.alpha_off = coef; A.alpha_diag = coef; A.shift = shift; A.skip = skip; A.beta = 0;
+  A.epi_F = F; A.epi_nB = nB; A.epi_nF = nF;
+  return hv_launch(X->c, X->L, A, X->transposed, X->conjugated, X->dtype, st);
+}
+
 // ---------------------------------------------------------------------------
 // sector rank / basis index of arbitrary states (LOOKUP_RANK path), used by
 // tests and by callers that index states without a search

@@ -162,6 +162,12 @@ struct HvArgs {
   int split_nrp;         // its row-pass workgroups (pairs of blocks, or single blocks)
   int split_lds2;        // bytes of LDS its row pass stages
   int split_pairs;       // its row pass takes two blocks per workgroup
+  // expm Taylor epilogue fused into the two-pass form's strip pass (one
+  // vector): F += the stored term, and the maxima of |term| and |F| go to
+  // the two norm slots (null: a plain product)
+  void* epi_F;
+  double* epi_nB;
+  double* epi_nF;
 };
 
 // Split-chain layout of one hard-core-boson particle sector of N <= 32 bits
@@ -1658,7 +1664,7 @@ __device__ __forceinline__ int64_t rl_i64(int64_t x, int t) {
 
 __device__ __forceinline__ double rl_dbl(double x, int t) { return __longlong_as_double(rl_i64(__double_as_longlong(x), t)); }
 
-template <typename VT, int HD, int XB>
+template <typename VT, int HD, int XB, bool EPI>
 __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
   if (A.skip && *A.skip) return;
   const SplitDev S = *A.split;
@@ -1684,6 +1690,7 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = threadIdx.x & 63;
   const uint32_t dmask = (1u << K) - 1u;
   const int upw = S.upw;
+  double maxB = 0.0, maxF = 0.0;  // EPI: this thread's largest |term| and |F|
   for (int64_t unit = xcd_contiguous(blockIdx.x, gridDim.x); unit < S.nunits; unit += gridDim.x) {
     int c = 0;
     while (c < K && s_uoff[c + 1] <= unit) ++c;
@@ -1814,6 +1821,35 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
       else res = make_double2(A.alpha_off.x * acc[r].x + dsum.x * xr[r].x, 0.0);
       if (A.beta) res = cadd(res, VTr<VT>::ld(vout + row));
       st_stream<VT>(vout + row, res);
+      if constexpr (EPI) {
+        // the stored (rounded) term enters F, as the separate epilogue does
+        VT* __restrict__ F = (VT*)A.epi_F;
+        const double2 rs = VTr<VT>::rnd(res);
+        const double2 f = VTr<VT>::rnd(cadd(ld_stream<VT>(F + row), rs));
+        st_stream<VT>(F + row, f);
+        maxB = fmax(maxB, CPLX ? sqrt(rs.x * rs.x + rs.y * rs.y) : fabs(rs.x));
+        maxF = fmax(maxF, CPLX ? sqrt(f.x * f.x + f.y * f.y) : fabs(f.x));
+      }
+    }
+  }
+  if constexpr (EPI) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      maxB = fmax(maxB, __shfl_xor(maxB, o, 64));
+      maxF = fmax(maxF, __shfl_xor(maxF, o, 64));
+    }
+    __shared__ double s_m[2][4];
+    if (lane == 0) {
+      s_m[0][w] = maxB;
+      s_m[1][w] = maxF;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const double mb = fmax(fmax(s_m[0][0], s_m[0][1]), fmax(s_m[0][2], s_m[0][3]));
+      const double mf = fmax(fmax(s_m[1][0], s_m[1][1]), fmax(s_m[1][2], s_m[1][3]));
+      // non-negative doubles order as their bit patterns
+      atomicMax((unsigned long long*)A.epi_nB, (unsigned long long)__double_as_longlong(mb));
+      atomicMax((unsigned long long*)A.epi_nF, (unsigned long long)__double_as_longlong(mf));
     }
   }
 }
@@ -1829,7 +1865,10 @@ int launch_hv_split2(const HvArgs& A, hipStream_t st) {
   const int64_t nb = A.split_nunits < 65536 ? (A.split_nunits > 0 ? A.split_nunits : 1) : 65536;
   // two upper-half terms and two crossing terms per round of loads (measured
   // at L=28 against one crossing term per round, and four upper-half terms)
-  hipLaunchKernelGGL((k_split2_strips<VT, 2, 2>), dim3((unsigned)nb), dim3(256), 0, st, A);
+  if (A.epi_F)
+    hipLaunchKernelGGL((k_split2_strips<VT, 2, 2, true>), dim3((unsigned)nb), dim3(256), 0, st, A);
+  else
+    hipLaunchKernelGGL((k_split2_strips<VT, 2, 2, false>), dim3((unsigned)nb), dim3(256), 0, st, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     qsx_set_error(std::string("split-chain launch failed: ") + hipGetErrorString(e));
@@ -1908,6 +1947,10 @@ int qsx_hv_launch_c128(const HvArgs& A, int N, int nmaps, bool fm, int lookup, i
       return launch_hv_split<VT>(A, A.split_nblk, st);                                        \
     if (!scatter && A.split && A.sched == 11 && A.fast_rows && A.nvec == 1)                    \
       return launch_hv_split2<VT>(A, st);                                                     \
+    if (A.epi_F) {                                                                             \
+      qsx_set_error("the fused expm epilogue needs the two-pass split schedule");              \
+      return QSX_ERR_UNSUPPORTED;                                                              \
+    }                                                                                          \
     if (scatter) {                                                                             \
       if (N <= 32) return launch_hv_scatter<uint32_t, VT>(A, nmaps, fm, lookup, st);           \
       return launch_hv_scatter<uint64_t, VT>(A, nmaps, fm, lookup, st);                        \

@@ -1041,6 +1041,8 @@ extern "C" int qsx_expm_multiply_csr_noalloc(const void* indptr, const void* ind
 // Matrix-free Taylor loop: same recurrence with H.v from the fused kernel.
 int qsx_hv_expm_term(void* hv_ctx, const void* x, void* y, double2 coef, double2 shift, const int* skip,
                      hipStream_t st);
+int qsx_hv_expm_term_epi(void* hv_ctx, const void* x, void* y, double2 coef, double2 shift, const int* skip,
+                         void* F, double* nB, double* nF, hipStream_t st);
 
 extern "C" int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star, double tol, double mu_re,
                                     double mu_im, double a_re, double a_im, void* v, void* work, int vec_dtype,
@@ -1062,6 +1064,7 @@ extern "C" int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star,
   const double er = a_re * mu_re - a_im * mu_im, ei = a_re * mu_im + a_im * mu_re;
   const double mag = std::exp(er / s);
   const double2 eta = make_double2(mag * std::cos(ei / s), mag * std::sin(ei / s));
+  bool fuse = nvec == 1;
   QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_start<VT>, dim3(rg), dim3(256), 0, st, (const VT*)v, (VT*)Bb[0], Ns,
                                               nvec, normB, 1));
   QSX_LAUNCH_CHECK();
@@ -1075,6 +1078,15 @@ extern "C" int qsx_expm_multiply_mf(void* hv_ctx, int64_t Ns, int s, int m_star,
       const double2 coef = make_double2(a_re * cf, a_im * cf);
       // shift = -coef * mu
       const double2 shift = make_double2(-(coef.x * mu_re - coef.y * mu_im), -(coef.x * mu_im + coef.y * mu_re));
+      if (fuse) {
+        // the two-pass split product writes the term and updates F and the
+        // norms in its strip pass (no separate epilogue pass)
+        rc = qsx_hv_expm_term_epi(hv_ctx, Bb[j & 1], Bb[(j + 1) & 1], coef, shift, &ctl->done, v,
+                                  normB + i * stride + j + 1, normF + i * stride + j + 1, st);
+        if (rc == QSX_OK) continue;
+        if (rc != QSX_ERR_UNSUPPORTED) return rc;
+        fuse = false;
+      }
       rc = qsx_hv_expm_term(hv_ctx, Bb[j & 1], Bb[(j + 1) & 1], coef, shift, &ctl->done, st);
       if (rc) return rc;
       QSX_VT_SWITCH(vec_dtype, hipLaunchKernelGGL(k_expm_epi<VT>, dim3(rg), dim3(256), 0, st, (const VT*)Bb[(j + 1) & 1],
