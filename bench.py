@@ -80,8 +80,8 @@ def main():
                     help="driven-Hubbard steps launched directly, without capturing step sequences")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build measures the single-GPU kernel "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        print("bench.py runs on one GPU only: it measures the single-GPU kernels; the multi-GPU "
+              "sharding of these workloads is a design, not a run", file=sys.stderr)
         sys.exit(2)
 
     import torch

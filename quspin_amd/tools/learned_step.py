@@ -15,7 +15,7 @@ and the partition from scipy's ``_fragment_3_1`` on the host
   written straight into device step records (``qsx_mlp_partition``).  The
   block's Taylor steps then run as one launch sequence (one hipGraph replay;
   ``qsx_evolve_csr_steps``) with no norm pass and no host round trip per step.
-  The tail check is the judge: a stage that uses all of its m terms without
+  The tail check decides: a stage that uses all of its m terms without
   passing the truncation test flags its step.  The block is then restored
   and redone with the exact selection (``learned=False`` runs that exact
   method throughout).
