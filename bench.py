@@ -37,10 +37,12 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.0, write
 # x1.0); keyed by (L, lookup, schedule, dtype).  Source and commands:
 # profiles/k_hv_rows_heisenberg_L28_kpz.md, profiles/k_hv_rows_heisenberg_L34_kpz.md
-# For these operators the "rows", "batch4" and "batch8" schedules all launch
-# k_hv_rows_fast_batch (batches of 7 or 6, 4 and 8 terms).
+# For these operators the "rows", "batch4", "batch8" and "batch14" schedules
+# all launch k_hv_rows_fast_batch (batches of 7 or 6, 4, 8 and 14 terms).
 MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 88.4e6, (28, "table", "batch4", "float64"): 87.0e6,
-                    (28, "table", "batch8", "float64"): 89.1e6, (34, "table", "rows", "float64"): 9.24e9,
+                    (28, "table", "batch8", "float64"): 89.1e6, (28, "table", "batch14", "float64"): 85.3e6,
+                    (34, "table", "rows", "float64"): 9.24e9, (34, "table", "batch4", "float64"): 9.10e9,
+                    (34, "table", "batch8", "float64"): 9.00e9, (34, "table", "batch14", "float64"): 9.10e9,
                     ("csr", 34, "float64"): 14.68e9}
 # The same for the precision workload's term kernels (full launches, per
 # launch): profiles/k_csr_spmv_expm_hubbard_L14_xxz_L28.md
