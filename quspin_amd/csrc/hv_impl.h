@@ -1534,7 +1534,18 @@ __global__ void __launch_bounds__(SPLIT_THREADS) k_hv_split(HvArgs A) {
 // lower-half terms are summed in FP64 (the class's sliced-ELL matrix, each
 // entry serving both blocks, then the lower-flip terms that read upper bits)
 // into Z, stored in the vector dtype.
+//
+// Two forms of the ELL loop.  KE > 0 (paired blocks, 512 threads): a row's
+// entries are loaded in one round, the next row's while this one is summed,
+// and the first row's before the staging barrier; a class wider than KE
+// takes the KE = 0 loop.  KE = 0 (one block, 1,024 threads): entries in
+// rounds of eight; about 60 VGPRs, so two workgroups of 16 waves fit a CU
+// beside their LDS.  Measured on the XXZ L = 28 row pass (13 entries per
+// row): complex64 pairs 311 -> 263-283 us with KE = 16, complex128 single
+// blocks 448 -> 386 us with 1,024 threads.
 constexpr int SPLIT2_ROWS_THREADS = 512;
+constexpr int SPLIT2_ROWS_THREADS1 = 1024;  // one block per workgroup
+constexpr int SPLIT2_ROWS_KE = 16;          // ELL entries per row loaded in one round (pairs)
 constexpr int SPLIT2_UP_ROWS = 4;  // u per wave in the column-strip pass
 constexpr int SPLIT2_UNIT_U = 4 * SPLIT2_UP_ROWS;  // u per column-strip unit (four waves)
 
@@ -1552,8 +1563,8 @@ __device__ __forceinline__ bool split_applies(const TermDev* __restrict__ T, uin
   return v == (uint32_t)T->req_val || (T->has2 && v == (uint32_t)T->req_val2);
 }
 
-template <typename VT, int R>
-__global__ void __launch_bounds__(SPLIT2_ROWS_THREADS) k_split2_rows(HvArgs A) {
+template <typename VT, int R, int KE, int NT>
+__global__ void __launch_bounds__(NT) k_split2_rows(HvArgs A) {
   extern __shared__ unsigned char split_smem[];
   __shared__ int64_t sbk[SPLIT_MAX_K * 33];  // C(n, k) for n < K (the ranks of d)
   __shared__ double2 lut[SPLIT_MAX_LUT];
@@ -1571,6 +1582,16 @@ __global__ void __launch_bounds__(SPLIT2_ROWS_THREADS) k_split2_rows(HvArgs A) {
   VT* __restrict__ z = (VT*)S.z;
   VT* xs = (VT*)split_smem;  // block 0 at xs[0 .. n), block 1 at xs[n .. 2n)
   constexpr bool CPLX = VTr<VT>::cplx;
+  const int ke = S.ell_k[c];
+  const uint32_t* __restrict__ el = S.ell + S.ell_off[c];
+  constexpr int KA = KE > 0 ? KE : 1;
+  const bool pipe = KE > 0 && ke <= KE;
+  uint32_t en[KA];  // the ELL entries of this thread's next row
+  if (pipe) {
+#pragma unroll
+    for (int q = 0; q < KA; ++q)
+      en[q] = (q < ke && (int)threadIdx.x < n) ? el[(int64_t)q * n + threadIdx.x] : 0u;
+  }
   {
     // staging: four loads per thread before their LDS stores
     const int bd = blockDim.x, tot = nb * n;
@@ -1592,11 +1613,32 @@ __global__ void __launch_bounds__(SPLIT2_ROWS_THREADS) k_split2_rows(HvArgs A) {
   for (int i = threadIdx.x; i < S.n_lut; i += blockDim.x) lut[i] = S.lut[i];
   __syncthreads();
   const uint16_t* __restrict__ dl = S.dlist + S.cls_off[c];
-  const int ke = S.ell_k[c];
-  const uint32_t* __restrict__ el = S.ell + S.ell_off[c];
   const int xo = nb > 1 ? n : 0;  // second block's offset in LDS (the first again when alone)
   for (int p = threadIdx.x; p < n; p += blockDim.x) {
     double2 acc0 = make_double2(0.0, 0.0), acc1 = make_double2(0.0, 0.0);
+    if (pipe) {
+      uint32_t ev[KA];
+      const int pn = p + (int)blockDim.x;
+#pragma unroll
+      for (int q = 0; q < KA; ++q) {
+        ev[q] = en[q];
+        en[q] = (q < ke && pn < n) ? el[(int64_t)q * n + pn] : 0u;
+      }
+#pragma unroll
+      for (int q = 0; q < KA; ++q) {
+        if (q >= ke) break;
+        const int pos = (int)(ev[q] & 0xFFFFu);
+        const double2 cf = lut[ev[q] >> 16];
+        const double2 x0 = VTr<VT>::ld(xs + pos);
+        if constexpr (CPLX) acc0 = cadd(acc0, cmul(cf, x0));
+        else acc0.x += cf.x * x0.x;
+        if constexpr (R > 1) {
+          const double2 x1 = VTr<VT>::ld(xs + xo + pos);
+          if constexpr (CPLX) acc1 = cadd(acc1, cmul(cf, x1));
+          else acc1.x += cf.x * x1.x;
+        }
+      }
+    } else
     for (int k0 = 0; k0 < ke; k0 += 8) {
       uint32_t ev[8];
 #pragma unroll
@@ -1857,11 +1899,11 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
 template <typename VT>
 int launch_hv_split2(const HvArgs& A, hipStream_t st) {
   if (A.split_pairs)
-    hipLaunchKernelGGL((k_split2_rows<VT, 2>), dim3((unsigned)A.split_nrp), dim3(SPLIT2_ROWS_THREADS),
-                       (size_t)A.split_lds2, st, A);
+    hipLaunchKernelGGL((k_split2_rows<VT, 2, SPLIT2_ROWS_KE, SPLIT2_ROWS_THREADS>), dim3((unsigned)A.split_nrp),
+                       dim3(SPLIT2_ROWS_THREADS), (size_t)A.split_lds2, st, A);
   else
-    hipLaunchKernelGGL((k_split2_rows<VT, 1>), dim3((unsigned)A.split_nrp), dim3(SPLIT2_ROWS_THREADS),
-                       (size_t)A.split_lds2, st, A);
+    hipLaunchKernelGGL((k_split2_rows<VT, 1, 0, SPLIT2_ROWS_THREADS1>), dim3((unsigned)A.split_nrp),
+                       dim3(SPLIT2_ROWS_THREADS1), (size_t)A.split_lds2, st, A);
   const int64_t nb = A.split_nunits < 65536 ? (A.split_nunits > 0 ? A.split_nunits : 1) : 65536;
   // two upper-half terms and two crossing terms per round of loads (measured
   // at L=28 against one crossing term per round, and four upper-half terms)
