@@ -1546,8 +1546,7 @@ __global__ void __launch_bounds__(SPLIT_THREADS) k_hv_split(HvArgs A) {
 constexpr int SPLIT2_ROWS_THREADS = 512;
 constexpr int SPLIT2_ROWS_THREADS1 = 1024;  // one block per workgroup
 constexpr int SPLIT2_ROWS_KE = 16;          // ELL entries per row loaded in one round (pairs)
-constexpr int SPLIT2_UP_ROWS = 4;  // u per wave in the column-strip pass
-constexpr int SPLIT2_UNIT_U = 4 * SPLIT2_UP_ROWS;  // u per column-strip unit (four waves)
+constexpr int SPLIT2_UNIT_U = 16;  // u per column-strip unit (two or four u per wave)
 
 // element coefficient of a bit-parallel term on state s (sign from smask)
 template <bool CPLX>
@@ -1693,7 +1692,7 @@ __global__ void __launch_bounds__(NT) k_split2_rows(HvArgs A) {
 // u of the class), strip-major and each XCD walking a contiguous run of
 // units, so the strip of every u of the class (C(K, c) u x 64 elements) stays
 // in that XCD's L2 while the upper-half terms read their partner blocks at
-// the same positions.  Wave w takes four u (wave-uniform: their partner
+// the same positions.  Wave w takes RW u (wave-uniform: their partner
 // blocks and coefficients are scalar), lane l position 64 s + l.  Then the
 // upper-flip terms that read lower bits and the crossing terms (per lane),
 // the diagonal and the output: y = alpha_off (Z + those sums) + (alpha_diag
@@ -1706,11 +1705,11 @@ __device__ __forceinline__ int64_t rl_i64(int64_t x, int t) {
 
 __device__ __forceinline__ double rl_dbl(double x, int t) { return __longlong_as_double(rl_i64(__double_as_longlong(x), t)); }
 
-template <typename VT, int HD, int XB, bool EPI>
-__global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
+template <typename VT, int HD, int XB, bool EPI, int RW>
+__global__ void __launch_bounds__(64 * SPLIT2_UNIT_U / RW) k_split2_strips(HvArgs A) {
   if (A.skip && *A.skip) return;
   const SplitDev S = *A.split;
-  constexpr int RW = SPLIT2_UP_ROWS;
+  constexpr int NW = SPLIT2_UNIT_U / RW;  // waves per workgroup
   constexpr bool CPLX = VTr<VT>::cplx;
   const int K = S.K, Np = A.R->Np;
   const VT* __restrict__ vin = (const VT*)A.v_in;
@@ -1880,15 +1879,18 @@ __global__ void __launch_bounds__(256) k_split2_strips(HvArgs A) {
       maxB = fmax(maxB, __shfl_xor(maxB, o, 64));
       maxF = fmax(maxF, __shfl_xor(maxF, o, 64));
     }
-    __shared__ double s_m[2][4];
+    __shared__ double s_m[2][NW];
     if (lane == 0) {
       s_m[0][w] = maxB;
       s_m[1][w] = maxF;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      const double mb = fmax(fmax(s_m[0][0], s_m[0][1]), fmax(s_m[0][2], s_m[0][3]));
-      const double mf = fmax(fmax(s_m[1][0], s_m[1][1]), fmax(s_m[1][2], s_m[1][3]));
+      double mb = s_m[0][0], mf = s_m[1][0];
+      for (int i = 1; i < NW; ++i) {
+        mb = fmax(mb, s_m[0][i]);
+        mf = fmax(mf, s_m[1][i]);
+      }
       // non-negative doubles order as their bit patterns
       atomicMax((unsigned long long*)A.epi_nB, (unsigned long long)__double_as_longlong(mb));
       atomicMax((unsigned long long*)A.epi_nF, (unsigned long long)__double_as_longlong(mf));
@@ -1905,12 +1907,27 @@ int launch_hv_split2(const HvArgs& A, hipStream_t st) {
     hipLaunchKernelGGL((k_split2_rows<VT, 1, 0, SPLIT2_ROWS_THREADS1>), dim3((unsigned)A.split_nrp),
                        dim3(SPLIT2_ROWS_THREADS1), (size_t)A.split_lds2, st, A);
   const int64_t nb = A.split_nunits < 65536 ? (A.split_nunits > 0 ? A.split_nunits : 1) : 65536;
-  // two upper-half terms and two crossing terms per round of loads (measured
-  // at L=28 against one crossing term per round, and four upper-half terms)
-  if (A.epi_F)
-    hipLaunchKernelGGL((k_split2_strips<VT, 2, 2, true>), dim3((unsigned)nb), dim3(256), 0, st, A);
-  else
-    hipLaunchKernelGGL((k_split2_strips<VT, 2, 2, false>), dim3((unsigned)nb), dim3(256), 0, st, A);
+  // Two u per wave and eight waves (68-76 VGPRs, against 105-127 with four
+  // u per wave and four waves), four upper-half terms per round of loads for
+  // elements of at most 8 bytes and two for complex128, two crossing terms
+  // per round.  Measured on the XXZ L = 28 strip pass: complex64 716.7 ->
+  // 679.9 us (fused epilogue 911.4 -> 841.2 us), complex128 912.1 -> 892.6
+  // us (four upper-half terms: 1,006 us).  The complex128 fused epilogue
+  // needs 81 VGPRs with two u per wave, which leaves each CU as many waves
+  // with half the rows in flight (1,293 against 1,179 us), so it keeps four
+  // u per wave.
+  constexpr int HD = sizeof(VT) <= 8 ? 4 : 2;
+  if (A.epi_F) {
+    if constexpr (sizeof(VT) > 8)
+      hipLaunchKernelGGL((k_split2_strips<VT, 2, 2, true, 4>), dim3((unsigned)nb), dim3(64 * SPLIT2_UNIT_U / 4), 0,
+                         st, A);
+    else
+      hipLaunchKernelGGL((k_split2_strips<VT, HD, 2, true, 2>), dim3((unsigned)nb), dim3(64 * SPLIT2_UNIT_U / 2), 0,
+                         st, A);
+  } else {
+    hipLaunchKernelGGL((k_split2_strips<VT, HD, 2, false, 2>), dim3((unsigned)nb), dim3(64 * SPLIT2_UNIT_U / 2), 0,
+                       st, A);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) {
     qsx_set_error(std::string("split-chain launch failed: ") + hipGetErrorString(e));
