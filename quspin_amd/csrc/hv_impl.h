@@ -1705,8 +1705,9 @@ __device__ __forceinline__ int64_t rl_i64(int64_t x, int t) {
 
 __device__ __forceinline__ double rl_dbl(double x, int t) { return __longlong_as_double(rl_i64(__double_as_longlong(x), t)); }
 
-template <typename VT, int HD, int XB, bool EPI, int RW>
-__global__ void __launch_bounds__(64 * SPLIT2_UNIT_U / RW) k_split2_strips(HvArgs A) {
+template <typename VT, int HD, int XB, bool EPI, int RW, int WPE = 1>
+__global__ void __launch_bounds__(64 * SPLIT2_UNIT_U / RW) __attribute__((amdgpu_waves_per_eu(WPE)))
+k_split2_strips(HvArgs A) {
   if (A.skip && *A.skip) return;
   const SplitDev S = *A.split;
   constexpr int NW = SPLIT2_UNIT_U / RW;  // waves per workgroup
@@ -1913,13 +1914,13 @@ int launch_hv_split2(const HvArgs& A, hipStream_t st) {
   // per round.  Measured on the XXZ L = 28 strip pass: complex64 716.7 ->
   // 679.9 us (fused epilogue 911.4 -> 841.2 us), complex128 912.1 -> 892.6
   // us (four upper-half terms: 1,006 us).  The complex128 fused epilogue
-  // needs 81 VGPRs with two u per wave, which leaves each CU as many waves
-  // with half the rows in flight (1,293 against 1,179 us), so it keeps four
-  // u per wave.
+  // needs 81 VGPRs with two u per wave (two workgroups per CU: 1,293 us,
+  // against 1,179 us with four u per wave); held to 80 VGPRs (six waves per
+  // SIMD, three workgroups) it takes 1,134 us.  Eight waves per SIMD spill.
   constexpr int HD = sizeof(VT) <= 8 ? 4 : 2;
   if (A.epi_F) {
     if constexpr (sizeof(VT) > 8)
-      hipLaunchKernelGGL((k_split2_strips<VT, 2, 2, true, 4>), dim3((unsigned)nb), dim3(64 * SPLIT2_UNIT_U / 4), 0,
+      hipLaunchKernelGGL((k_split2_strips<VT, 2, 2, true, 2, 6>), dim3((unsigned)nb), dim3(64 * SPLIT2_UNIT_U / 2), 0,
                          st, A);
     else
       hipLaunchKernelGGL((k_split2_strips<VT, HD, 2, true, 2>), dim3((unsigned)nb), dim3(64 * SPLIT2_UNIT_U / 2), 0,
