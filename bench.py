@@ -47,7 +47,7 @@ MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 88.4e6, (28, "table", "bat
 # The same for the precision workload's term kernels (full launches, per
 # launch): profiles/k_csr_spmv_expm_hubbard_L14_xxz_L28.md
 C3_TRAFFIC = {("mf", "complex64", "rows"): 3.76e9, ("mf", "complex128", "rows"): 7.44e9,
-              ("mf", "complex64", "split2"): 2.96e9, ("mf", "complex128", "split2"): 5.36e9,
+              ("mf", "complex64", "split2"): 2.95e9, ("mf", "complex128", "split2"): 5.17e9,
               ("csr", "complex64"): 9.95e9, ("csr", "complex128"): 18.76e9}
 # ... and of one batched H.V of the 26-sector workload (CSR form, all sectors):
 # profiles/k_csr_wave_batched_heisenberg_L26_64vec.md
