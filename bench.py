@@ -37,10 +37,12 @@ HBM_PEAK_GBS = 8000.0
 # WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.0, write
 # x1.0); keyed by (L, lookup, schedule, dtype).  Source and commands:
 # profiles/k_hv_rows_heisenberg_L28_kpz.md, profiles/k_hv_rows_heisenberg_L34_kpz.md
-# For these operators the "rows", "batch4", "batch8" and "batch14" schedules
-# all launch k_hv_rows_fast_batch (batches of 7 or 6, 4, 8 and 14 terms).
+# For these operators the "rows", "batch4", "batch8", "batch14" and "batch14x"
+# schedules all launch k_hv_rows_fast_batch (batches of 7 or 6, 4, 8 and 14
+# terms; batch14x with each XCD's workgroups on one contiguous run of rows).
 MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 88.4e6, (28, "table", "batch4", "float64"): 87.0e6,
                     (28, "table", "batch8", "float64"): 89.1e6, (28, "table", "batch14", "float64"): 85.3e6,
+                    (28, "table", "batch14x", "float64"): 45.8e6, (34, "table", "batch14x", "float64"): 7.05e9,
                     (34, "table", "rows", "float64"): 9.24e9, (34, "table", "batch4", "float64"): 9.10e9,
                     (34, "table", "batch8", "float64"): 9.00e9, (34, "table", "batch14", "float64"): 9.10e9,
                     ("csr", 34, "float64"): 14.68e9}
@@ -230,7 +232,9 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic,
                      "alg_bytes_per_state": s_i + 1 + 16,
                      "achieved_with_folded_diag": round(achieved_fd, 2),
-                     "limit": ("cache-resident working set: latency of the L2 misses of the sector-table reads "
+                     "limit": ("cache-resident working set: divergent gathers (about one cache line per lane "
+                               "for each applied term's table entry and x element) through each CU's vector "
+                               "memory path; halving the L2 misses (schedule batch14x) leaves the time unchanged "
                                "(profiles/k_hv_rows_heisenberg_L28_kpz.md); the HBM-bound size is "
                                "c2_scaled_L34") if L <= 30 else "sector-table reads from HBM"},
         "cpu_baseline": cpu,

@@ -140,7 +140,8 @@ int qsx_hv(qsx_core* core, qsx_oplist* l, const void* basis, const void* n, int 
  * queue, 2/3/4 = 2/4/8 lanes per row, 5/6 = batches of 4/8 terms, 7 =
  * split-chain blocks, 8/9 = direct-rank chunks of 4/8 terms with predicated
  * gathers, 10 = batches of 14 terms with the table, 11 = split-chain layout
- * in two passes through a buffer the context allocates), apply. */
+ * in two passes through a buffer the context allocates, 12 = as 10 with the
+ * workgroups of each XCD on one contiguous run of rows), apply. */
 int qsx_hv_ctx_create(qsx_core* core, qsx_oplist* l, const void* basis, const void* n, int n_bytes, int64_t Ns,
                       const int64_t* begin, const int64_t* end, int N_p, int dtype, int64_t nvec, int transposed,
                       int conjugated, int diag_conj, const void* diag_arr, int diag_arr_cplx, qsx_hv_ctx** out);

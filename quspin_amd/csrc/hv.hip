@@ -1014,12 +1014,13 @@ extern "C" int qsx_hv_ctx_set_table(qsx_hv_ctx* X, const void* table, int entry_
 // 8/9: direct-rank rows with the terms in chunks of 4/8, gathers predicated
 // (a sector without symmetry maps; other products use schedule 0);
 // 10: batches of 14 terms where the fast-row kernels apply (as 6 elsewhere);
+// 12: as 10, with the workgroups of each XCD on one contiguous run of rows;
 // 11: the split-chain layout in two passes, a row pass over the lower-half
 // terms into a buffer of Ns elements owned by the context, then a
 // column-strip pass over the upper-half and crossing terms (sectors as 7).
 // Single vector; several vectors always use one lane per row.
 extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
-  if (!X || sched < 0 || sched > 11) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
+  if (!X || sched < 0 || sched > 12) { qsx_set_error("qsx_hv_ctx_set_schedule: bad arguments"); return QSX_ERR_ARG; }
   if ((sched == 7 && !X->split_d) || (sched == 11 && !X->split_z)) {
     const int rc = split_build(X, sched == 11);
     if (rc) return rc;
@@ -1029,6 +1030,7 @@ extern "C" int qsx_hv_ctx_set_schedule(qsx_hv_ctx* X, int sched) {
     return QSX_ERR_UNSUPPORTED;
   }
   X->A.sched = sched;
+  X->A.xcd_rows = sched == 12 ? 1 : 0;
   ctx_changed(X);
   return QSX_OK;
 }

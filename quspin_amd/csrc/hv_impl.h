@@ -162,6 +162,7 @@ struct HvArgs {
   int split_nrp;         // its row-pass workgroups (pairs of blocks, or single blocks)
   int split_lds2;        // bytes of LDS its row pass stages
   int split_pairs;       // its row pass takes two blocks per workgroup
+  int xcd_rows;          // k_hv_rows_fast_batch: each XCD's workgroups take one contiguous run of rows
   // expm Taylor epilogue fused into the two-pass form's strip pass (one
   // vector): F += the stored term, and the maxima of |term| and |F| go to
   // the two norm slots (null: a plain product)
@@ -940,7 +941,11 @@ __global__ void __launch_bounds__(256) k_hv_rows_fast_batch(HvArgs A) {
   const double2* slut;
   fast_stage<LUTL>(A, fast_smem, sb, slut);
   __syncthreads();
-  const int64_t r0 = (int64_t)blockIdx.x * blockDim.x * RPL + threadIdx.x;
+  // with xcd_rows the rows of one XCD share their high bits, so most of
+  // their targets (and table entries) do too: fewer L2 misses while the table
+  // sits in the MALL, more HBM time when it does not (the tuner times both)
+  const int64_t bi = A.xcd_rows ? xcd_contiguous(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  const int64_t r0 = bi * blockDim.x * RPL + threadIdx.x;
   if (r0 >= A.Ns) return;
   constexpr bool CPLX = VTr<VT>::cplx;
   const VT* __restrict__ vin = (const VT*)A.v_in;
@@ -1332,7 +1337,8 @@ void launch_hv_fast(const HvArgs& A, int nb, hipStream_t st) {
 // Dispatch over (maps, fermion sign, lookup, schedule, NV).
 // schedule codes: 0 one lane per row, 1 compacted queue, 2/3/4 = 2/4/8 lanes
 // per row, 5/6 = batches of 4/8 terms (rank-based lookups), 10 = batches of
-// 14 terms on the fast-row path (elsewhere as 6)
+// 14 terms on the fast-row path (elsewhere as 6), 12 = as 10 with each XCD's
+// workgroups on one contiguous run of rows (A.xcd_rows)
 template <typename I, typename VT, int M, bool FERM, int LK>
 void launch_hv_one(const HvArgs& A, hipStream_t st) {
   const int blk = 256;
@@ -1352,12 +1358,13 @@ void launch_hv_one(const HvArgs& A, hipStream_t st) {
       else hipLaunchKernelGGL((k_hv_rows_fast_chunk<I, VT, 2, 8>), g2, dim3(blk), 0, st, A);
     }
   }
-  else if (A.nvec == 1 && A.fast_rows && (A.sched == 0 || A.sched == 5 || A.sched == 6 || A.sched == 10) && !FERM &&
-           (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
+  else if (A.nvec == 1 && A.fast_rows && (A.sched == 0 || A.sched == 5 || A.sched == 6 || A.sched == 10 || A.sched == 12) &&
+           !FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
     if constexpr (!FERM && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK))
       launch_hv_fast<I, VT, M, LK>(A, A.sched == 0 ? 0 : A.sched == 5 ? 4 : A.sched == 6 ? 8 : 14, st);
   }
-  else if (A.nvec == 1 && (A.sched == 5 || A.sched == 6 || A.sched == 10) && (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
+  else if (A.nvec == 1 && (A.sched == 5 || A.sched == 6 || A.sched == 10 || A.sched == 12) &&
+           (LK == LOOKUP_TABLE || LK == LOOKUP_RANK)) {
     if constexpr (LK == LOOKUP_TABLE || LK == LOOKUP_RANK) {
       if (A.sched == 5) hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 4>), grid, dim3(blk), 0, st, A);
       else hipLaunchKernelGGL((k_hv_rows_batch<I, VT, M, FERM, LK, 8>), grid, dim3(blk), 0, st, A);

@@ -6,7 +6,8 @@ table in HBM) and the schedule ("rows": one lane per row; "rows2/4/8": that
 many lanes per row with shuffle reduction; "queue": compacted LDS work queue;
 "batch4/8": terms in batches with all lookups of a batch in flight, for
 rank-based lookups; "batch14": the same with 14 terms per batch, for the
-table; "split": blocks of the split-chain layout of one hard-core-boson
+table; "batch14x": batch14 with the workgroups of each XCD on one contiguous
+run of rows; "split": blocks of the split-chain layout of one hard-core-boson
 sector without symmetry maps; "split2": the same layout in two passes, lower
 half-chain terms per block, then upper-half and crossing terms over column
 strips; "chunk4/8": direct-rank rows with the terms in chunks and predicated
@@ -29,7 +30,7 @@ _mem = {}
 
 # schedule name -> launch code of qsx_hv_ctx_set_schedule
 SCHEDULES = {"rows": 0, "queue": 1, "rows2": 2, "rows4": 3, "rows8": 4, "batch4": 5, "batch8": 6, "split": 7,
-             "chunk4": 8, "chunk8": 9, "batch14": 10, "split2": 11}
+             "chunk4": 8, "chunk8": 9, "batch14": 10, "split2": 11, "batch14x": 12}
 
 
 def _cache_path():
@@ -95,7 +96,7 @@ def candidates(op, vdt, nvec):
         if lk == "table" or _rank_lookup(op):
             scheds += ["batch4", "batch8"]
         if lk == "table":
-            scheds.append("batch14")
+            scheds += ["batch14", "batch14x"]
         if lk == "search" and _split_layout(op):
             scheds += ["split", "split2"]
         if lk == "search" and _rank_lookup(op) and not len(b._pers):

@@ -190,6 +190,7 @@ def test_tuning_static_rule_and_candidates():
     assert tuning.static_rule(sym, np.float64, 4)["schedule"] == "rows"
     names = {(c["lookup"], c["schedule"]) for c in tuning.candidates(sym, np.float64, 1)}
     assert ("table", "batch8") in names and ("search", "queue") in names and ("table", "batch14") in names
+    assert ("table", "batch14x") in names
     assert all(c["schedule"] in tuning.SCHEDULES for c in tuning.candidates(sym, np.float64, 1))
 
 

@@ -250,7 +250,8 @@ def test_large_C2_hermiticity_and_linearity():
 
 
 @pytest.mark.parametrize("lookup", ["search", "table"])
-@pytest.mark.parametrize("schedule", ["rows", "rows2", "rows4", "rows8", "queue", "batch4", "batch8", "batch14"])
+@pytest.mark.parametrize("schedule", ["rows", "rows2", "rows4", "rows8", "queue", "batch4", "batch8", "batch14",
+                                      "batch14x"])
 def test_forced_configs_vs_oracle(lookup, schedule, monkeypatch):
     """Every (lookup, schedule) pair of the fused kernel against the oracle:
     complex k-sector, real k=0/p/z sector, fermion k-sector."""
@@ -323,7 +324,7 @@ def test_merged_term_pairs_vs_oracle(lookup, monkeypatch):
 
 
 @pytest.mark.parametrize("lookup", ["table", "search"])
-@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8", "batch14", "chunk4", "chunk8"])
+@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8", "batch14", "batch14x", "chunk4", "chunk8"])
 def test_fast_rows_pauli_strings_vs_oracle(lookup, schedule, monkeypatch):
     """The fast-row kernels (rows: one term at a time with the direct rank,
     batches sized from the term count with the table; batches of 4, 8 and 14;
@@ -357,7 +358,7 @@ def test_fast_rows_pauli_strings_vs_oracle(lookup, schedule, monkeypatch):
             np.testing.assert_allclose(op.dot(v), M @ v, atol=1e-11, rtol=0)
 
 
-@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8", "batch14"])
+@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8", "batch14", "batch14x"])
 def test_fast_rows_u64_table_vs_oracle(schedule, monkeypatch):
     """Sector tables with 8-byte entries (index | coefficient id << 40, the
     layout large bases need), forced on small bases: the fast-row kernels
