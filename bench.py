@@ -39,12 +39,12 @@ HBM_PEAK_GBS = 8000.0
 # profiles/k_hv_rows_heisenberg_L28_kpz.md, profiles/k_hv_rows_heisenberg_L34_kpz.md
 # For these operators the "rows", "batch4", "batch8", "batch14" and "batch14x"
 # schedules all launch k_hv_rows_fast_batch (batches of 7 or 6, 4, 8 and 14
-# terms; batch14x with each XCD's workgroups on one contiguous run of rows).
-MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 88.4e6, (28, "table", "batch4", "float64"): 87.0e6,
-                    (28, "table", "batch8", "float64"): 89.1e6, (28, "table", "batch14", "float64"): 85.3e6,
+# terms; each XCD takes runs of 64 or 256 blocks, batch14x one run per XCD).
+MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 68.5e6, (28, "table", "batch4", "float64"): 67.9e6,
+                    (28, "table", "batch8", "float64"): 67.8e6, (28, "table", "batch14", "float64"): 71.3e6,
                     (28, "table", "batch14x", "float64"): 45.8e6, (34, "table", "batch14x", "float64"): 7.05e9,
-                    (34, "table", "rows", "float64"): 9.24e9, (34, "table", "batch4", "float64"): 9.10e9,
-                    (34, "table", "batch8", "float64"): 9.00e9, (34, "table", "batch14", "float64"): 9.10e9,
+                    (34, "table", "rows", "float64"): 7.42e9, (34, "table", "batch4", "float64"): 7.35e9,
+                    (34, "table", "batch8", "float64"): 7.32e9, (34, "table", "batch14", "float64"): 7.30e9,
                     ("csr", 34, "float64"): 14.68e9}
 # The same for the precision workload's term kernels (full launches, per
 # launch): profiles/k_csr_spmv_expm_hubbard_L14_xxz_L28.md

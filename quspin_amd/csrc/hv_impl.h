@@ -242,6 +242,16 @@ __device__ __forceinline__ int64_t xcd_contiguous(int64_t b, int64_t nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + pos;
 }
 
+// Block index with each XCD taking runs of G consecutive blocks inside a
+// window of 8 G that all XCDs share (G = 1: dispatch order; one run per XCD:
+// xcd_contiguous).  Blocks past the last full window keep their index.
+__device__ __forceinline__ int64_t xcd_chunked(int64_t b, int64_t nb, int64_t G) {
+  const int64_t nfull = (nb / (8 * G)) * 8 * G;
+  if (b >= nfull) return b;
+  const int64_t pos = b >> 3;
+  return (pos / G) * 8 * G + (b & 7) * G + pos % G;
+}
+
 __device__ __forceinline__ double load_n(const void* p, int nb, int64_t i) {
   return nb == 1 ? (double)((const uint8_t*)p)[i] : nb == 2 ? (double)((const uint16_t*)p)[i]
                                                               : (double)((const uint32_t*)p)[i];
@@ -941,10 +951,16 @@ __global__ void __launch_bounds__(256) k_hv_rows_fast_batch(HvArgs A) {
   const double2* slut;
   fast_stage<LUTL>(A, fast_smem, sb, slut);
   __syncthreads();
-  // with xcd_rows the rows of one XCD share their high bits, so most of
-  // their targets (and table entries) do too: fewer L2 misses while the table
-  // sits in the MALL, more HBM time when it does not (the tuner times both)
-  const int64_t bi = A.xcd_rows ? xcd_contiguous(blockIdx.x, gridDim.x) : (int64_t)blockIdx.x;
+  // Rows in runs per XCD: consecutive rows share their high bits, and so do
+  // most of their targets and table entries, so each XCD's L2 sees fewer
+  // distinct lines.  By default each XCD takes runs of 64 blocks (256 on
+  // grids of 16,384 blocks or more) inside a window all XCDs share: measured
+  // on the Heisenberg chain, L = 28 (1,412 blocks) 28.4 -> 28.1 us and L = 34
+  // (67,108 blocks) 1.30 -> 1.155 ms against dispatch order.  xcd_rows (the
+  // batch14x schedule) gives each XCD one run: fewest L2 misses, but 1.42 ms
+  // at L = 34; the tuner times both.
+  const int64_t bi = A.xcd_rows ? xcd_contiguous(blockIdx.x, gridDim.x)
+                                : xcd_chunked(blockIdx.x, gridDim.x, gridDim.x >= 16384 ? 256 : 64);
   const int64_t r0 = bi * blockDim.x * RPL + threadIdx.x;
   if (r0 >= A.Ns) return;
   constexpr bool CPLX = VTr<VT>::cplx;

@@ -249,6 +249,30 @@ def test_large_C2_hermiticity_and_linearity():
     assert torch.max(torch.abs(Hv2 - Hv)).item() < 1e-11
 
 
+@pytest.mark.parametrize("schedule", ["rows", "batch4", "batch8", "batch14", "batch14x"])
+def test_large_C2_forced_schedules_vs_csr(schedule, monkeypatch):
+    """Full C2 size (1,412 blocks of 256 rows, so the per-XCD block orders
+    renumber most of the grid): every table schedule of the batched kernel
+    against the device CSR, which is assembled through the orbit search."""
+    import torch
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import hamiltonian, quantum_LinearOperator
+    monkeypatch.setenv("QSX_LOOKUP", "table")
+    monkeypatch.setenv("QSX_SCHEDULE", schedule)
+    L = 28
+    m = chain_maps(L)
+    print(f"case L={L} schedule={schedule}", flush=True)
+    B = spin_basis_general(L, Nup=14, pauli=0, kblock=(m["T"], 0), pblock=(m["P"], 0), zblock=(m["Z"], 0))
+    static = xxz_static(L, J=1.0, Delta=1.0)
+    kw = dict(check_symm=False, check_herm=False, check_pcon=False)
+    Q = quantum_LinearOperator(static, basis=B, dtype=np.float64, **kw)
+    v = torch.randn(B.Ns, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(3))
+    Hv = Q.dot(v)
+    assert all(c["schedule"] == schedule for c in Q._qsx_operator().config.values())
+    Hv2 = hamiltonian(static, [], basis=B, dtype=np.float64, **kw).dot(v)
+    assert torch.max(torch.abs(Hv2 - Hv)).item() <= 1e-12 * torch.max(torch.abs(Hv2)).item()
+
+
 @pytest.mark.parametrize("lookup", ["search", "table"])
 @pytest.mark.parametrize("schedule", ["rows", "rows2", "rows4", "rows8", "queue", "batch4", "batch8", "batch14",
                                       "batch14x"])
