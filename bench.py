@@ -45,12 +45,12 @@ MEASURED_TRAFFIC = {(28, "table", "rows", "float64"): 68.5e6, (28, "table", "bat
                     (28, "table", "batch14x", "float64"): 45.8e6, (34, "table", "batch14x", "float64"): 7.05e9,
                     (34, "table", "rows", "float64"): 7.42e9, (34, "table", "batch4", "float64"): 7.35e9,
                     (34, "table", "batch8", "float64"): 7.32e9, (34, "table", "batch14", "float64"): 7.30e9,
-                    ("csr", 34, "float64"): 14.68e9}
+                    ("csr", 34, "float64"): 11.99e9}
 # The same for the precision workload's term kernels (full launches, per
 # launch): profiles/k_csr_spmv_expm_hubbard_L14_xxz_L28.md
 C3_TRAFFIC = {("mf", "complex64", "rows"): 3.76e9, ("mf", "complex128", "rows"): 7.44e9,
               ("mf", "complex64", "split2"): 2.95e9, ("mf", "complex128", "split2"): 5.17e9,
-              ("csr", "complex64"): 9.95e9, ("csr", "complex128"): 18.76e9}
+              ("csr", "complex64"): 8.22e9, ("csr", "complex128"): 15.03e9}
 # ... and of one batched H.V of the 26-sector workload (CSR form, all sectors):
 # profiles/k_csr_wave_batched_heisenberg_L26_64vec.md
 C4_TRAFFIC = {("csr", "fp32"): 70.1e9, ("csr", "fp64"): 161.6e9}
