@@ -53,7 +53,7 @@ C3_TRAFFIC = {("mf", "complex64", "rows"): 3.76e9, ("mf", "complex128", "rows"):
               ("csr", "complex64"): 8.22e9, ("csr", "complex128"): 15.03e9}
 # ... and of one batched H.V of the 26-sector workload (CSR form, all sectors):
 # profiles/k_csr_wave_batched_heisenberg_L26_64vec.md
-C4_TRAFFIC = {("csr", "fp32"): 70.1e9, ("csr", "fp64"): 161.6e9}
+C4_TRAFFIC = {("csr", "fp32"): 59.1e9, ("csr", "fp64"): 161.6e9}
 
 
 def chain_static(L, J=1.0, Delta=1.0):

@@ -78,14 +78,24 @@ __device__ __forceinline__ double wave_sum(double v) {
 // xcd_blocks > 0: block b of the XCD-interleaved dispatch (block b runs on
 // XCD b % 8) is given the chunk range of its XCD's slice, so neighbouring
 // rows share an L2.
-// xcd_blocks == 0: chunks in dispatch order (all XCDs work on neighbouring
-// chunks, i.e. inside one sector's vectors at a time).
+// xcd_blocks == 0, WIN (elements of at most 8 bytes): each XCD takes runs of
+// 64 consecutive blocks (256 on grids of 16,384 blocks or more) inside a
+// window all XCDs share (xcd_chunked), so all XCDs work inside one sector's
+// vectors at a time and each XCD's rows are neighbours; without WIN, chunks
+// in dispatch order.  Measured on the 26 L = 26 Heisenberg sectors x 64
+// vectors (CSR form, 650 k blocks): complex64 H.V 9.45 -> 9.12 ms and expm
+// 1.87-2.00 -> 1.57 s.  complex128 H.V went 21.3 -> 16.1 ms (16.98 ms with
+// xcd_blocks), but its Taylor-term kernel lost (expm 3.70 s against 3.12 s
+// with xcd_blocks, which the tuner then no longer picked: it times the plain
+// H.V), so complex128 keeps dispatch order.
 // The result is wave-uniform and is marked so (readfirstlane): chunk data,
 // sector descriptors and row offsets then live in scalar registers and the
 // gathers address vector rows as scalar base + lane offset.
+template <bool WIN>
 __device__ __forceinline__ int64_t wave_chunk_index(const BatchArgs& B) {
   const int b = blockIdx.x;
-  const int64_t slice_block = B.xcd_blocks ? (int64_t)(b & 7) * B.xcd_blocks + (b >> 3) : (int64_t)b;
+  int64_t slice_block = B.xcd_blocks ? (int64_t)(b & 7) * B.xcd_blocks + (b >> 3) : (int64_t)b;
+  if (WIN && !B.xcd_blocks) slice_block = xcd_chunked(b, gridDim.x, gridDim.x >= 16384 ? 256 : 64);
   const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   return slice_block * (blockDim.x >> 6) + w;
 }
@@ -213,7 +223,7 @@ __global__ void __launch_bounds__(256) k_hv_wave(BatchArgs B) {
   __shared__ int64_t sb[(LK == LOOKUP_TABLE || LK == LOOKUP_RANK) ? QSX_SB : 1];
   __shared__ int32_t s_col[4][64];
   __shared__ double2 s_cf[4][64];
-  const int64_t ci = wave_chunk_index(B);
+  const int64_t ci = wave_chunk_index<(sizeof(VT) <= 8)>(B);
   const WaveChunk ch = uniform_chunk(ci < B.nchunk ? B.chunk[ci] : WaveChunk{0, 0, 0});
   {
     // the binomial table is the same for every sector of a batch
@@ -330,7 +340,7 @@ __device__ __forceinline__ int64_t rlane64(int64_t v, int l) {
 // current row's gathers are consumed.  Chunks hold at most 63 rows.
 template <typename VT, bool EPI>
 __global__ void __launch_bounds__(256) k_csr_wave(BatchArgs B) {
-  const int64_t ci = wave_chunk_index(B);
+  const int64_t ci = wave_chunk_index<(sizeof(VT) <= 8)>(B);
   if (ci >= B.nchunk) return;
   const WaveChunk ch = uniform_chunk(B.chunk[ci]);
   const int k = ch.sec;
@@ -423,7 +433,7 @@ __global__ void __launch_bounds__(256) k_csr_wave(BatchArgs B) {
 //                ||F|| -> nB[k][i+1][0] when another stage follows.
 template <typename VT, bool SCALE>
 __global__ void __launch_bounds__(256) k_batch_stage(BatchArgs B) {
-  const int64_t ci = wave_chunk_index(B);
+  const int64_t ci = wave_chunk_index<(sizeof(VT) <= 8)>(B);
   if (ci >= B.nchunk) return;
   const WaveChunk ch = uniform_chunk(B.chunk[ci]);
   const int k = ch.sec;

@@ -67,6 +67,38 @@ def test_batched_apply_all_k_sectors(nvec, dt, chunk, form):
         np.testing.assert_allclose(L.to_host(Bt.unstack(Y2)[k]), ref, rtol=0, atol=tol * max(1.0, np.abs(ref).max()))
 
 
+@pytest.mark.parametrize("form", ["mf", "csr"])
+@pytest.mark.parametrize("xcd_slices", [0, 1])
+def test_batched_apply_large_grid_block_orders(form, xcd_slices):
+    """All 20 momentum sectors of an L=20 chain (184,756 rows, 4-row chunks:
+    about 11,500 blocks, so the per-XCD block orders renumber most of the
+    grid) against each sector's own single-sector product."""
+    import torch
+    from quspin_amd import _lib as L
+    from quspin_amd.basis import spin_basis_general
+    from quspin_amd.operators import quantum_LinearOperator
+    from quspin_amd.operators._devop import SectorBatch
+    L_, nvec = 20, 2
+    print(f"case L={L_} nvec={nvec} form={form} xcd_slices={xcd_slices}", flush=True)
+    m = chain_maps(L_)
+    static = xxz_static(L_, Delta=0.7)
+    ops = [quantum_LinearOperator(static, basis=spin_basis_general(L_, Nup=L_ // 2, pauli=0, kblock=(m["T"], k)),
+                                  dtype=np.complex128, check_symm=False, check_herm=False, check_pcon=False)
+           for k in range(L_)]
+    Bt = SectorBatch([op._qsx_operator() for op in ops], np.complex128, nvec, form=form, chunk_rows=4)
+    Bt.set_schedule(4, xcd_slices)
+    assert Bt.rows == 184756
+    rng = np.random.default_rng(11)
+    xs = [_rand(rng, (op.shape[0], nvec), np.complex128) for op in ops]
+    X = Bt.stack(xs)
+    Y = Bt.empty()
+    Bt.apply(X, Y)
+    got = Bt.unstack(Y)
+    for k, op in enumerate(ops):
+        ref = op.dot(torch.from_numpy(xs[k]).cuda())
+        np.testing.assert_allclose(L.to_host(got[k]), L.to_host(ref), rtol=0, atol=1e-12 * max(1.0, float(ref.abs().max())))
+
+
 def test_batched_matches_single_sector_kernel_with_parity_and_inversion():
     """k=0 and k=L/2 sectors with parity and spin inversion (real characters),
     float64: batched product == the single-operator launch, bit for bit up to
